@@ -1,0 +1,343 @@
+// k_cut.hip -- boundary cut (graphcut.jl:5-84) and paste (iqsim.jl:252-281).
+//
+// One 256-wide workgroup per overlap slab.  The first slice along the cut
+// axis is tied to the source and the last to the sink with infinite capacity
+// (graphcut.jl:56-70), so both are contracted into the terminals and a
+// preflow push-relabel runs on the interior slices, its state resident in LDS
+// (global scratch beyond IQ_PR_NLDS interior nodes).  The schedule is
+// synchronous: a relabel phase, a push phase (only the higher endpoint of an
+// edge can push on it, so every residual pair has one writer) and a gather
+// phase that adds incoming flow in a fixed neighbour order; a global relabel
+// (exact residual BFS distance to the sink) runs every IQ_GR_FREQ rounds and
+// decides termination.  The result is deterministic and matches the oracle's
+// restatement bit for bit.  Cut mask = voxels that cannot reach the sink in
+// the residual graph (graphcut.jl:79-81).
+#include "iq_internal.h"
+
+#define CB 256
+
+template <typename T>
+__device__ inline double cut_cap(const T* A, const T* B, int64_t u, int64_t v, int64_t w, bool wok);
+
+template <>
+__device__ inline double cut_cap<float>(const float* A, const float* B, int64_t u, int64_t v, int64_t w, bool wok) {
+  float Au = A[u], Av = A[v], Bu = B[u], Bv = B[v];
+  float Du = __builtin_fabsf(Au - Bu), Dv = __builtin_fabsf(Av - Bv);
+  float gAu = __builtin_fabsf(Av - Au), gBu = __builtin_fabsf(Bv - Bu);
+  float gAv = wok ? __builtin_fabsf(A[w] - Av) : gAu;
+  float gBv = wok ? __builtin_fabsf(B[w] - Bv) : gBu;
+  float num = Du + Dv;
+  float g = ((gAu + gAv) + gBu) + gBv;
+  return (double)num / ((double)g + 0x1p-52);
+}
+
+template <>
+__device__ inline double cut_cap<double>(const double* A, const double* B, int64_t u, int64_t v, int64_t w, bool wok) {
+  double Du = __builtin_fabs(A[u] - B[u]), Dv = __builtin_fabs(A[v] - B[v]);
+  double gAu = __builtin_fabs(A[v] - A[u]), gBu = __builtin_fabs(B[v] - B[u]);
+  double gAv = wok ? __builtin_fabs(A[w] - A[v]) : gAu;
+  double gBv = wok ? __builtin_fabs(B[w] - B[v]) : gBu;
+  return (Du + Dv) / ((((gAu + gAv) + gBu) + gBv) + 0x1p-52);
+}
+
+// Slab view: element (x,y,z) of the slab lives at base + x*sx + y*sy + z*sz.
+template <typename T>
+struct SlabView {
+  const T* base;
+  int64_t st[3];
+  __device__ int64_t at(int x, int y, int z) const { return x * st[0] + y * st[1] + z * st[2]; }
+};
+
+template <typename T>
+__device__ double slab_cap(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int c0, int c1, int c2, int d) {
+  int c[3] = {c0, c1, c2};
+  int cv[3] = {c0, c1, c2};
+  cv[d] += 1;
+  int cw[3] = {c0, c1, c2};
+  cw[d] += 2;
+  bool wok = cw[d] < s[d];
+  int64_t u = A.at(c[0], c[1], c[2]), v = A.at(cv[0], cv[1], cv[2]);
+  int64_t w = wok ? A.at(cw[0], cw[1], cw[2]) : 0;
+  // A and B share strides only when both live in the same array; use per-view offsets
+  int64_t ub = B.at(c[0], c[1], c[2]), vb = B.at(cv[0], cv[1], cv[2]);
+  int64_t wb = wok ? B.at(cw[0], cw[1], cw[2]) : 0;
+  // evaluate with the element order of graphcut.jl:37-52
+  T Au = A.base[u], Av = A.base[v], Bu = B.base[ub], Bv = B.base[vb];
+  T Aw = wok ? A.base[w] : (T)0, Bw = wok ? B.base[wb] : (T)0;
+  T a3[3] = {Au, Av, Aw}, b3[3] = {Bu, Bv, Bw};
+  return cut_cap<T>(a3, b3, 0, 1, 2, wok);
+}
+
+struct CutState {
+  double *e, *rt, *rf, *rb, *pr;
+  int32_t *h, *hn, *fl;
+};
+
+__device__ inline void cut_bind(CutState& S, char* mem, int n) {
+  S.e = (double*)mem;
+  S.rt = S.e + n;
+  S.rf = S.rt + n;
+  S.rb = S.rf + 3 * n;
+  S.pr = S.rb + 3 * n;
+  S.h = (int32_t*)(S.pr + 3 * n);
+  S.hn = S.h + n;
+  S.fl = S.hn + n;
+}
+
+// bytes of state per interior node
+#define CUT_NODE_BYTES (8 * 11 + 4 * 3)
+
+template <typename T>
+__global__ void __launch_bounds__(CB)
+k_graphcut(const T* __restrict__ sim, const T* __restrict__ ti, IqGeo g, const IqStep* __restrict__ step,
+           const int64_t* __restrict__ pick, uint8_t* __restrict__ keep, char* __restrict__ gscratch,
+           int64_t gscratch_per_slab, int32_t* __restrict__ err, int64_t* __restrict__ rounds) {
+  extern __shared__ char cut_lds[];
+  const int64_t pk = *pick;
+  if (pk < 0) return;
+  const IqSlab sl = step->slab[blockIdx.x];
+  const int dim = sl.dim;
+  const int s[3] = {sl.sz[0], sl.sz[1], sl.sz[2]};
+  const int L = s[dim];
+  // slab views: A = current grid over the tile, B = chosen training-image window
+  const int rc[3] = {(int)(pk % g.dist[0]), (int)((pk / g.dist[0]) % g.dist[1]),
+                     (int)(pk / ((int64_t)g.dist[0] * g.dist[1]))};
+  SlabView<T> A, B;
+  A.base = sim + iq_lin3(g.pad, step->origin[0] + sl.lo[0], step->origin[1] + sl.lo[1], step->origin[2] + sl.lo[2]);
+  A.st[0] = 1; A.st[1] = g.pad[0]; A.st[2] = (int64_t)g.pad[0] * g.pad[1];
+  B.base = ti + iq_lin3(g.ti, rc[0] + sl.lo[0], rc[1] + sl.lo[1], rc[2] + sl.lo[2]);
+  B.st[0] = 1; B.st[1] = g.ti[0]; B.st[2] = (int64_t)g.ti[0] * g.ti[1];
+  uint8_t* out = keep + (int64_t)blockIdx.x * g.tilevox;
+  const int nvox = s[0] * s[1] * s[2];
+  const uint8_t side = (uint8_t)sl.side;
+  // terminal slices
+  for (int p = threadIdx.x; p < nvox; p += CB) {
+    int c[3] = {p % s[0], (p / s[0]) % s[1], p / (s[0] * s[1])};
+    uint8_t m = (c[dim] == L - 1) ? 0 : 1;
+    out[p] = m ^ side;
+  }
+  if (L <= 2) return;
+  int si[3] = {s[0], s[1], s[2]};
+  si[dim] = L - 2;
+  const int n = si[0] * si[1] * si[2];
+  const int HMAX = n + 1;
+  const int ist[3] = {1, si[0], si[0] * si[1]};
+  CutState S;
+  if (n <= IQ_PR_NLDS) cut_bind(S, cut_lds, n);
+  else cut_bind(S, gscratch + (int64_t)blockIdx.x * gscratch_per_slab, n);
+
+  // build: capacities of graphcut.jl:22-54 on interior edges; saturated source edges
+  for (int u = threadIdx.x; u < n; u += CB) {
+    int ic[3] = {u % si[0], (u / si[0]) % si[1], u / (si[0] * si[1])};
+    int pc[3] = {ic[0], ic[1], ic[2]};
+    pc[dim] += 1;
+    int f = 0;
+    double e = 0.0, rt = 0.0;
+    for (int d = 0; d < 3; d++) {
+      double c = 0.0;
+      if (ic[d] + 1 < si[d]) { f |= 1 << d; c = slab_cap<T>(A, B, s, pc[0], pc[1], pc[2], d); }
+      S.rf[3 * u + d] = c;
+      S.rb[3 * u + d] = c;
+      S.pr[3 * u + d] = 0.0;
+    }
+    if (pc[dim] == 1) {
+      int q[3] = {pc[0], pc[1], pc[2]};
+      q[dim] -= 1;
+      e = e + slab_cap<T>(A, B, s, q[0], q[1], q[2], dim);
+    }
+    if (pc[dim] == L - 2) { f |= 8; rt = slab_cap<T>(A, B, s, pc[0], pc[1], pc[2], dim); }
+    S.e[u] = e;
+    S.rt[u] = rt;
+    S.fl[u] = f;
+  }
+  __syncthreads();
+
+  int64_t total = 0;
+  for (;;) {
+    // global relabel
+    for (int u = threadIdx.x; u < n; u += CB) S.h[u] = ((S.fl[u] & 8) && S.rt[u] > 0.0) ? 1 : HMAX;
+    __syncthreads();
+    for (int lev = 1;; lev++) {
+      int ch = 0;
+      for (int u = threadIdx.x; u < n; u += CB) {
+        if (S.h[u] != HMAX) continue;
+        int ic[3] = {u % si[0], (u / si[0]) % si[1], u / (si[0] * si[1])};
+        bool hit = false;
+        for (int d = 0; d < 3 && !hit; d++) {
+          if (ic[d] > 0) {
+            int v = u - ist[d];
+            if (S.rb[3 * v + d] > 0.0 && S.h[v] == lev) hit = true;
+          }
+          if (!hit && (S.fl[u] & (1 << d))) {
+            int v = u + ist[d];
+            if (S.rf[3 * u + d] > 0.0 && S.h[v] == lev) hit = true;
+          }
+        }
+        if (hit) { S.h[u] = lev + 1; ch = 1; }
+      }
+      if (!__syncthreads_or(ch)) break;
+    }
+    int act = 0;
+    for (int u = threadIdx.x; u < n; u += CB) act |= (S.e[u] > 0.0 && S.h[u] < HMAX);
+    if (!__syncthreads_or(act)) break;
+    if (total > IQ_PR_LIMIT) {
+      if (threadIdx.x == 0) atomicCAS(err, 0, IQ_KERR_ITER);
+      break;
+    }
+    for (int it = 0; it < IQ_GR_FREQ; it++) {
+      // relabel phase (also clears the push records of the previous round)
+      for (int u = threadIdx.x; u < n; u += CB) {
+        S.pr[3 * u] = 0.0; S.pr[3 * u + 1] = 0.0; S.pr[3 * u + 2] = 0.0;
+        int hu = S.h[u];
+        int nh = hu;
+        if (S.e[u] > 0.0 && hu < HMAX) {
+          int ic[3] = {u % si[0], (u / si[0]) % si[1], u / (si[0] * si[1])};
+          bool adm = false;
+          int mh = HMAX;
+          if ((S.fl[u] & 8) && S.rt[u] > 0.0) { if (hu == 1) adm = true; mh = 0; }
+          for (int d = 0; d < 3; d++) {
+            if (ic[d] > 0) {
+              int v = u - ist[d];
+              if (S.rb[3 * v + d] > 0.0) { int hv = S.h[v]; if (hu == hv + 1) adm = true; if (hv < mh) mh = hv; }
+            }
+            if (S.fl[u] & (1 << d)) {
+              int v = u + ist[d];
+              if (S.rf[3 * u + d] > 0.0) { int hv = S.h[v]; if (hu == hv + 1) adm = true; if (hv < mh) mh = hv; }
+            }
+          }
+          if (!adm) nh = (mh + 1 < HMAX) ? mh + 1 : HMAX;
+        }
+        S.hn[u] = nh;
+      }
+      __syncthreads();
+      for (int u = threadIdx.x; u < n; u += CB) S.h[u] = S.hn[u];
+      __syncthreads();
+      // push phase
+      for (int u = threadIdx.x; u < n; u += CB) {
+        double ex = S.e[u];
+        int hu = S.h[u];
+        if (!(ex > 0.0 && hu < HMAX)) continue;
+        int ic[3] = {u % si[0], (u / si[0]) % si[1], u / (si[0] * si[1])};
+        if ((S.fl[u] & 8) && hu == 1 && S.rt[u] > 0.0) {
+          double r = S.rt[u];
+          double dl = ex < r ? ex : r;
+          ex -= dl;
+          S.rt[u] = r - dl;
+        }
+        for (int d = 0; d < 3 && ex > 0.0; d++) {
+          if (ic[d] > 0) {
+            int v = u - ist[d];
+            if (hu == S.h[v] + 1) {
+              double r = S.rb[3 * v + d];
+              if (r > 0.0) {
+                double dl = ex < r ? ex : r;
+                ex -= dl;
+                S.rb[3 * v + d] = r - dl;
+                S.rf[3 * v + d] += dl;
+                S.pr[3 * v + d] = -dl;
+              }
+            }
+          }
+          if (ex > 0.0 && (S.fl[u] & (1 << d))) {
+            int v = u + ist[d];
+            if (hu == S.h[v] + 1) {
+              double r = S.rf[3 * u + d];
+              if (r > 0.0) {
+                double dl = ex < r ? ex : r;
+                ex -= dl;
+                S.rf[3 * u + d] = r - dl;
+                S.rb[3 * u + d] += dl;
+                S.pr[3 * u + d] = dl;
+              }
+            }
+          }
+        }
+        S.e[u] = ex;
+      }
+      __syncthreads();
+      // gather phase
+      int a2 = 0;
+      for (int u = threadIdx.x; u < n; u += CB) {
+        int ic[3] = {u % si[0], (u / si[0]) % si[1], u / (si[0] * si[1])};
+        double ex = S.e[u];
+        for (int d = 0; d < 3; d++) {
+          if (ic[d] > 0) {
+            double q = S.pr[3 * (u - ist[d]) + d];
+            if (q > 0.0) ex += q;
+          }
+          if (S.fl[u] & (1 << d)) {
+            double q = S.pr[3 * u + d];
+            if (q < 0.0) ex += -q;
+          }
+        }
+        S.e[u] = ex;
+        a2 |= (ex > 0.0 && S.h[u] < HMAX);
+      }
+      total++;
+      if (!__syncthreads_or(a2)) break;
+    }
+  }
+  for (int u = threadIdx.x; u < n; u += CB) {
+    int ic[3] = {u % si[0], (u / si[0]) % si[1], u / (si[0] * si[1])};
+    int pc[3] = {ic[0], ic[1], ic[2]};
+    pc[dim] += 1;
+    uint8_t m = (S.h[u] < HMAX) ? 0 : 1;
+    out[pc[0] + s[0] * (pc[1] + s[1] * pc[2])] = m ^ side;
+  }
+  if (threadIdx.x == 0 && rounds) atomicAdd((unsigned long long*)rounds, (unsigned long long)total);
+}
+
+template __global__ void k_graphcut<float>(const float*, const float*, IqGeo, const IqStep*, const int64_t*, uint8_t*, char*, int64_t, int32_t*, int64_t*);
+template __global__ void k_graphcut<double>(const double*, const double*, IqGeo, const IqStep*, const int64_t*, uint8_t*, char*, int64_t, int32_t*, int64_t*);
+
+// Paste (iqsim.jl:278): voxels whose OR-ed keep flag is 0 take the chosen
+// training-image window; the keep flags are the debug cut grid (iqsim.jl:281).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_paste(T* __restrict__ sim, const T* __restrict__ ti, IqGeo g, const IqStep* __restrict__ step,
+        const int64_t* __restrict__ pick, const uint8_t* __restrict__ keep, uint8_t* __restrict__ cut) {
+  const int64_t pk = *pick;
+  if (pk < 0) return;
+  const int rc[3] = {(int)(pk % g.dist[0]), (int)((pk / g.dist[0]) % g.dist[1]),
+                     (int)(pk / ((int64_t)g.dist[0] * g.dist[1]))};
+  const int ns = step->nslab;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < g.tilevox; j += gridDim.x * blockDim.x) {
+    int x = j % g.tile[0], y = (j / g.tile[0]) % g.tile[1], z = j / (g.tile[0] * g.tile[1]);
+    uint8_t k = 0;
+    for (int s = 0; s < ns; s++) {
+      const IqSlab& sl = step->slab[s];
+      int a = x - sl.lo[0], b = y - sl.lo[1], c = z - sl.lo[2];
+      if (a >= 0 && b >= 0 && c >= 0 && a < sl.sz[0] && b < sl.sz[1] && c < sl.sz[2])
+        k |= keep[(int64_t)s * g.tilevox + a + sl.sz[0] * (b + sl.sz[1] * c)];
+    }
+    int64_t gi = iq_lin3(g.pad, step->origin[0] + x, step->origin[1] + y, step->origin[2] + z);
+    if (!k) sim[gi] = ti[iq_lin3(g.ti, rc[0] + x, rc[1] + y, rc[2] + z)];
+    if (cut) cut[gi] = k;
+  }
+}
+
+template __global__ void k_paste<float>(float*, const float*, IqGeo, const IqStep*, const int64_t*, const uint8_t*, uint8_t*);
+template __global__ void k_paste<double>(double*, const double*, IqGeo, const IqStep*, const int64_t*, const uint8_t*, uint8_t*);
+
+// Fill helpers (no memset nodes: the run may be captured in a graph)
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+template __global__ void k_fill<float>(float*, int64_t, float);
+template __global__ void k_fill<double>(double*, int64_t, double);
+template __global__ void k_fill<uint8_t>(uint8_t*, int64_t, uint8_t);
+template __global__ void k_fill<int64_t>(int64_t*, int64_t, int64_t);
+template __global__ void k_fill<int32_t>(int32_t*, int64_t, int32_t);
+
+// Disabled candidates -> +inf in an exact map (iqsim.jl:207,217,226).
+extern "C" __global__ void k_zero_map(double* __restrict__ d, const uint8_t* __restrict__ disabled, int64_t P) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = disabled[i] ? __builtin_inf() : 0.0;
+}
+
+// Pick index from host (replay of a forced pick for the standalone cut tests).
+extern "C" __global__ void k_iota32(int32_t* __restrict__ v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = (int32_t)i;
+}

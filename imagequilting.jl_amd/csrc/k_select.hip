@@ -1,0 +1,505 @@
+// k_select.hip -- candidate selection of one tile step (iqsim.jl:237-243):
+//   * tol-band pick  findall(D .<= (1 + tol) * minimum(D))   (iqsim.jl:237)
+//   * relaxation     soft/hard pattern database              (relaxation.jl:5-48)
+//   * taumodel       dense ranks -> tau-model weights         (taumodel.jl:5-45)
+//   * weighted draw  with a caller-supplied uniform          (iqsim.jl:243)
+// Each kernel is one 1024-wide workgroup per query.  Every floating-point
+// step uses the reference's operator order (the library is compiled with
+// -ffp-contract=off), so results are bit-identical to the FP64 oracle.
+#include "iq_internal.h"
+
+#define FB 1024
+#define NW (FB / 64)
+#define S2MAX 2048
+
+struct SelScratch {
+  int64_t i64[NW + 4];
+  double f64[NW + 4];
+  int64_t result;
+  int64_t n;
+  int32_t flag;
+};
+
+// ---------------------------------------------------------------------------
+// block primitives (blockDim.x == FB)
+// ---------------------------------------------------------------------------
+__device__ int64_t blk_excl_scan(int64_t v, int64_t* total, SelScratch& sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int64_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh.i64[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t run = 0;
+    for (int i = 0; i < NW; i++) { int64_t t = sh.i64[i]; sh.i64[i] = run; run += t; }
+    sh.i64[NW] = run;
+  }
+  __syncthreads();
+  int64_t r = sh.i64[w] + x - v;
+  if (total) *total = sh.i64[NW];
+  __syncthreads();
+  return r;
+}
+
+__device__ int64_t blk_sum_i64(int64_t v, SelScratch& sh) {
+  int64_t t;
+  blk_excl_scan(v, &t, sh);
+  return t;
+}
+
+__device__ double blk_max_f64(double v, SelScratch& sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) sh.f64[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sh.f64[0];
+    for (int i = 1; i < NW; i++) m = fmax(m, sh.f64[i]);
+    sh.f64[NW] = m;
+  }
+  __syncthreads();
+  double r = sh.f64[NW];
+  __syncthreads();
+  return r;
+}
+
+__device__ double blk_min_f64(double v, SelScratch& sh) { return -blk_max_f64(-v, sh); }
+
+// Ascending bitonic sort of sv[0..m) in LDS, m a power of two (padded with +inf).
+__device__ void blk_bitonic(double* sv, int m) {
+  for (int k = 2; k <= m; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < m; i += FB) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          double a = sv[i], b = sv[ixj];
+          bool up = (i & k) == 0;
+          if ((a > b) == up) { sv[i] = b; sv[ixj] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Dense ranks (ties share a rank, taumodel.jl:21-31) of vals[0..n) written to
+// rank[0..n).  Returns false when n values with more than one distinct value
+// exceed the LDS sorter.
+__device__ bool blk_dense_rank(const double* vals, int64_t n, int32_t* rank, double* sv, int32_t* dp,
+                               SelScratch& sh) {
+  double lmin = __builtin_inf(), lmax = -__builtin_inf();
+  for (int64_t i = threadIdx.x; i < n; i += FB) { lmin = fmin(lmin, vals[i]); lmax = fmax(lmax, vals[i]); }
+  double mn = blk_min_f64(lmin, sh), mx = blk_max_f64(lmax, sh);
+  if (mn == mx) {
+    for (int64_t i = threadIdx.x; i < n; i += FB) rank[i] = 1;
+    __syncthreads();
+    return true;
+  }
+  if (n > IQ_SORT_MAX) return false;
+  int m = 1;
+  while (m < n) m <<= 1;
+  for (int i = threadIdx.x; i < m; i += FB) sv[i] = i < n ? vals[i] : __builtin_inf();
+  __syncthreads();
+  blk_bitonic(sv, m);
+  // inclusive count of distinct values: 4 consecutive entries per work-item
+  int64_t local = 0;
+  const int per = (m + FB - 1) / FB;
+  const int i0 = threadIdx.x * per;
+  for (int i = i0; i < i0 + per && i < m; i++) local += (i == 0 || sv[i] != sv[i - 1]);
+  int64_t ex = blk_excl_scan(local, nullptr, sh);
+  for (int i = i0; i < i0 + per && i < m; i++) {
+    ex += (i == 0 || sv[i] != sv[i - 1]);
+    dp[i] = (int32_t)ex;
+  }
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < n; i += FB) {
+    double v = vals[i];
+    int lo = 0, hi = m;  // first position with sv >= v
+    while (lo < hi) { int mid = (lo + hi) >> 1; if (sv[mid] < v) lo = mid + 1; else hi = mid; }
+    rank[i] = dp[lo];
+  }
+  __syncthreads();
+  return true;
+}
+
+// Three-level blocked cumulative draw: the same summation order as the
+// oracle (blocks of IQ_SB at two levels, a sequential top level).
+__device__ int64_t blk_sample(const double* p, int64_t n, double u, double* s1, double* s2, SelScratch& sh) {
+  const int64_t n1 = (n + IQ_SB - 1) / IQ_SB, n2 = (n1 + IQ_SB - 1) / IQ_SB;
+  for (int64_t k = threadIdx.x; k < n1; k += FB) {
+    double s = 0.0;
+    int64_t e = (k + 1) * IQ_SB < n ? (k + 1) * IQ_SB : n;
+    for (int64_t i = k * IQ_SB; i < e; i++) s += p[i];
+    s1[k] = s;
+  }
+  __syncthreads();
+  for (int64_t m = threadIdx.x; m < n2; m += FB) {
+    double s = 0.0;
+    int64_t e = (m + 1) * IQ_SB < n1 ? (m + 1) * IQ_SB : n1;
+    for (int64_t k = m * IQ_SB; k < e; k++) s += s1[k];
+    s2[m] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double W = 0.0;
+    for (int64_t m = 0; m < n2; m++) W += s2[m];
+    double t = u * W;
+    double acc = 0.0;
+    int64_t m = 0;
+    for (;; m++) {
+      double nx = acc + s2[m];
+      if (nx >= t || m == n2 - 1) break;
+      acc = nx;
+    }
+    int64_t k = m * IQ_SB, kend = (m + 1) * IQ_SB < n1 ? (m + 1) * IQ_SB : n1;
+    for (;; k++) {
+      double nx = acc + s1[k];
+      if (nx >= t || k == kend - 1) break;
+      acc = nx;
+    }
+    int64_t i = k * IQ_SB, iend = (k + 1) * IQ_SB < n ? (k + 1) * IQ_SB : n;
+    for (;; i++) {
+      double nx = acc + p[i];
+      if (nx >= t || i == iend - 1) break;
+      acc = nx;
+    }
+    sh.result = i;
+  }
+  __syncthreads();
+  int64_t r = sh.result;
+  __syncthreads();
+  return r;
+}
+
+// Position of `key` in the ascending list idx[0..n) (or -1).
+__device__ int64_t blk_find(const int32_t* idx, int64_t n, int64_t key, SelScratch& sh) {
+  if (threadIdx.x == 0) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) { int64_t mid = (lo + hi) >> 1; if (idx[mid] < key) lo = mid + 1; else hi = mid; }
+    sh.result = (lo < n && idx[lo] == key) ? lo : -1;
+  }
+  __syncthreads();
+  int64_t r = sh.result;
+  __syncthreads();
+  return r;
+}
+
+__device__ void set_err(int32_t* err, int code) {
+  atomicCAS(err, 0, code);
+}
+
+// Tau-model weights (taumodel.jl:34-44) for one more source with dense
+// ranks `rank`; p holds the running product prod(X / x0) (starts at 1.0).
+__device__ void tau_accumulate(int64_t n, const int32_t* rank, double* p, SelScratch& sh) {
+  int64_t ls = 0;
+  for (int64_t i = threadIdx.x; i < n; i += FB) ls += (n - rank[i]) + 1;
+  const double S = (double)blk_sum_i64(ls, sh);  // exact integer sum
+  const double nd = (double)n;
+  const double x0 = (1.0 - 1.0 / nd) / (1.0 / nd);
+  for (int64_t i = threadIdx.x; i < n; i += FB) {
+    double P = ((nd - (double)rank[i]) + 1.0) / S;
+    double X = (1.0 - P) / P;
+    p[i] = p[i] * (X / x0);
+  }
+  __syncthreads();
+}
+
+__device__ void tau_finish(int64_t n, double* p) {
+  const double nd = (double)n;
+  const double x0 = (1.0 - 1.0 / nd) / (1.0 / nd);
+  for (int64_t i = threadIdx.x; i < n; i += FB) {
+    double x = x0 * p[i];
+    p[i] = 1.0 / (1.0 + x);
+  }
+  __syncthreads();
+}
+
+struct SelOut {
+  int64_t* pick;        // current pick of this query (consumed by the cut / paste)
+  int64_t* log;         // pick log entry of this path position
+  int32_t* err;         // run error word
+  int64_t* nband;       // size of the pattern database (diagnostics)
+};
+
+// Draw (or replay `forced`) from db[0..n) with weights p; publish the pick.
+__device__ void publish_pick(const int32_t* db, int64_t n, const double* p, double u, int64_t forced,
+                             double* s1, double* s2, SelOut o, SelScratch& sh) {
+  int64_t i;
+  if (forced >= 0) {
+    i = blk_find(db, n, forced, sh);
+    if (i < 0) {
+      if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_NOTINDB); *o.pick = -1; *o.log = forced; }
+      return;
+    }
+  } else {
+    i = blk_sample(p, n, u, s1, s2, sh);
+  }
+  if (threadIdx.x == 0) {
+    *o.pick = db[i];
+    *o.log = db[i];
+    if (o.nband) *o.nband = n;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tol-band finalize after k_band_superset: exact min -> threshold -> ordered
+// compaction of the band -> dense ranks -> weights -> draw.
+// ---------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(FB)
+k_band_finalize(int64_t P, const unsigned long long* __restrict__ bitmap,
+                const double* __restrict__ dexact, const unsigned long long* __restrict__ gmin64,
+                double tol, double u, int64_t forced, int32_t* __restrict__ band,
+                double* __restrict__ bandd, int32_t* __restrict__ rank, double* __restrict__ p,
+                double* __restrict__ s1, SelOut o) {
+  __shared__ SelScratch sh;
+  __shared__ double sv[IQ_SORT_MAX];
+  __shared__ int32_t dp[IQ_SORT_MAX];
+  __shared__ double s2[S2MAX];
+  if (*o.pick == -2) return;  // an earlier step failed: leave the grid untouched
+  const double m = __longlong_as_double((long long)*gmin64);
+  if (__builtin_isinf(m)) {
+    if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_NOCAND); *o.pick = -2; }
+    return;
+  }
+  const double thr = (1.0 + tol) * m;  // iqsim.jl:237
+  const int64_t nw = (P + 63) / 64;
+  const int64_t per = (nw + FB - 1) / FB;
+  const int64_t w0 = threadIdx.x * per, w1 = (w0 + per < nw) ? w0 + per : nw;
+  int64_t cnt = 0;
+  for (int64_t w = w0; w < w1; w++) {
+    unsigned long long b = bitmap[w];
+    while (b) {
+      int bit = __builtin_ctzll(b);
+      b &= b - 1;
+      cnt += dexact[w * 64 + bit] <= thr;
+    }
+  }
+  int64_t nb;
+  int64_t off = blk_excl_scan(cnt, &nb, sh);
+  for (int64_t w = w0; w < w1; w++) {
+    unsigned long long b = bitmap[w];
+    while (b) {
+      int bit = __builtin_ctzll(b);
+      b &= b - 1;
+      int64_t c = w * 64 + bit;
+      double d = dexact[c];
+      if (d <= thr) { band[off] = (int32_t)c; bandd[off] = d; off++; }
+    }
+  }
+  __syncthreads();
+  if (nb == 1) {
+    if (threadIdx.x == 0) p[0] = 1.0;  // taumodel.jl:9
+    __syncthreads();
+  } else {
+    if (!blk_dense_rank(bandd, nb, rank, sv, dp, sh)) {
+      if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_SORTMAX); *o.pick = -2; }
+      return;
+    }
+    for (int64_t i = threadIdx.x; i < nb; i += FB) p[i] = 1.0;
+    __syncthreads();
+    tau_accumulate(nb, rank, p, sh);
+    tau_finish(nb, p);
+  }
+  publish_pick(band, nb, p, u, forced, s1, s2, o, sh);
+}
+
+// Steps with nothing to compare (no pasted neighbour, no hard or soft data):
+// D == 0 on every enabled candidate, so the band is the enabled list and every
+// rank is 1.
+extern "C" __global__ void __launch_bounds__(FB)
+k_band_enabled(const int32_t* __restrict__ enabled, int64_t nen, double u, int64_t forced,
+               double* __restrict__ p, double* __restrict__ s1, SelOut o) {
+  __shared__ SelScratch sh;
+  __shared__ double s2[S2MAX];
+  if (*o.pick == -2) return;
+  if (nen == 0) {
+    if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_NOCAND); *o.pick = -2; }
+    return;
+  }
+  if (nen == 1) {
+    if (threadIdx.x == 0) p[0] = 1.0;
+  } else {
+    const double nd = (double)nen;
+    const double S = (double)(nen * nen);  // sum of (n - 1 + 1) over n events
+    const double x0 = (1.0 - 1.0 / nd) / (1.0 / nd);
+    const double P = ((nd - 1.0) + 1.0) / S;
+    const double X = (1.0 - P) / P;
+    const double pp = 1.0 * (X / x0);
+    const double x = x0 * pp;
+    const double w = 1.0 / (1.0 + x);
+    for (int64_t i = threadIdx.x; i < nen; i += FB) p[i] = w;
+  }
+  __syncthreads();
+  publish_pick(enabled, nen, p, u, forced, s1, s2, o, sh);
+}
+
+// ---------------------------------------------------------------------------
+// Relaxation finalize (relaxation.jl:5-39 + taumodel.jl with several sources).
+// maps[0] is the primary map D, maps[1..nsrc) the auxiliary maps Ds, each an
+// exact FP64 map over P candidates; sorted[j] is the stable ascending order of
+// maps[j] (ties by index, Julia's Perm ordering).  top-k sets are prefixes of
+// those orders; the running intersection is kept as bitmaps.
+// ---------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(FB)
+k_relax_finalize(int64_t P, int nsrc, const double* const* __restrict__ maps,
+                 const int32_t* const* __restrict__ sorted, double tol, double u,
+                 int64_t forced, uint32_t* __restrict__ bits, int32_t* __restrict__ ev,
+                 double* __restrict__ vals, int32_t* __restrict__ rank, double* __restrict__ p,
+                 double* __restrict__ s1, SelOut o) {
+  // o.pick == nullptr: stop after the pattern database (written to ev, size to *o.nband)
+  __shared__ SelScratch sh;
+  __shared__ double sv[IQ_SORT_MAX];
+  __shared__ int32_t dp[IQ_SORT_MAX];
+  __shared__ double s2[S2MAX];
+  __shared__ int64_t lenj[16];
+  const bool select_only = o.pick == nullptr;
+  if (!select_only && *o.pick == -2) return;
+  const int64_t nwd = (P + 31) / 32;
+  const double* D = maps[0];
+  // enabled patterns = finite entries = prefix of the ascending order (+inf sorts last)
+  if (threadIdx.x == 0) {
+    int64_t lo = 0, hi = P;
+    while (lo < hi) { int64_t mid = (lo + hi) >> 1; if (!__builtin_isinf(D[sorted[0][mid]])) lo = mid + 1; else hi = mid; }
+    sh.n = lo;
+  }
+  __syncthreads();
+  const int64_t nen = sh.n;
+  __syncthreads();
+  if (nen == 0) {  // relaxation.jl would never terminate
+    if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_NOCAND); if (!select_only) *o.pick = -2; }
+    return;
+  }
+  const bool allzero = D[sorted[0][nen - 1]] == 0.0;  // largest finite value is 0
+  const int64_t dbsize = allzero ? nen : (int64_t)ceil(tol * (double)nen);
+  for (int64_t i = threadIdx.x; i < nwd * nsrc; i += FB) bits[i] = 0u;
+  if (threadIdx.x < 16) lenj[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < dbsize; i += FB) {
+    int32_t c = sorted[0][i];
+    atomicOr(&bits[c >> 5], 1u << (c & 31));
+  }
+  __syncthreads();
+  double frac = 0.1 * ((double)dbsize / (double)nen);
+  int64_t cnt = dbsize;
+  for (;;) {
+    const int64_t softdbsize = (int64_t)ceil(frac * (double)nen);
+    cnt = dbsize;
+    for (int j = 1; j < nsrc; j++) {
+      const int64_t l0 = lenj[j];
+      for (int64_t i = l0 + threadIdx.x; i < softdbsize; i += FB) {
+        int32_t c = sorted[j][i];
+        atomicOr(&bits[j * nwd + (c >> 5)], 1u << (c & 31));
+      }
+      __syncthreads();
+      if (threadIdx.x == 0 && softdbsize > l0) lenj[j] = softdbsize;
+      int64_t lc = 0;
+      for (int64_t w = threadIdx.x; w < nwd; w += FB) {
+        uint32_t x = bits[w];
+        for (int q = 1; q <= j; q++) x &= bits[q * nwd + w];
+        lc += __builtin_popcount(x);
+      }
+      cnt = blk_sum_i64(lc, sh);
+      if (cnt == 0) break;
+    }
+    if (cnt != 0) break;
+    frac = fmin(frac + 0.1, 1.0);
+  }
+  // ordered compaction of the pattern database
+  const int64_t per = (nwd + FB - 1) / FB;
+  const int64_t w0 = threadIdx.x * per, w1 = (w0 + per < nwd) ? w0 + per : nwd;
+  int64_t lc = 0;
+  for (int64_t w = w0; w < w1; w++) {
+    uint32_t x = bits[w];
+    for (int q = 1; q < nsrc; q++) x &= bits[q * nwd + w];
+    lc += __builtin_popcount(x);
+  }
+  int64_t nev;
+  int64_t off = blk_excl_scan(lc, &nev, sh);
+  for (int64_t w = w0; w < w1; w++) {
+    uint32_t x = bits[w];
+    for (int q = 1; q < nsrc; q++) x &= bits[q * nwd + w];
+    while (x) {
+      int b = __builtin_ctz(x);
+      x &= x - 1;
+      ev[off++] = (int32_t)(w * 32 + b);
+    }
+  }
+  __syncthreads();
+  if (select_only) {
+    if (threadIdx.x == 0) *o.nband = nev;
+    return;
+  }
+  if (nev == 1) {
+    if (threadIdx.x == 0) p[0] = 1.0;
+    __syncthreads();
+  } else {
+    for (int64_t i = threadIdx.x; i < nev; i += FB) p[i] = 1.0;
+    __syncthreads();
+    for (int j = 0; j < nsrc; j++) {
+      for (int64_t i = threadIdx.x; i < nev; i += FB) vals[i] = maps[j][ev[i]];
+      __syncthreads();
+      if (!blk_dense_rank(vals, nev, rank, sv, dp, sh)) {
+        if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_SORTMAX); *o.pick = -2; }
+        return;
+      }
+      tau_accumulate(nev, rank, p, sh);
+    }
+    tau_finish(nev, p);
+  }
+  publish_pick(ev, nev, p, u, forced, s1, s2, o, sh);
+}
+
+// ---------------------------------------------------------------------------
+// Standalone entry points (iq_band_pick / iq_taumodel / iq_sample)
+// ---------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(FB)
+k_band_pick_host(const double* __restrict__ D, int64_t n, double tol, int32_t* __restrict__ out,
+                 int64_t* __restrict__ count) {
+  __shared__ SelScratch sh;
+  double lm = __builtin_inf();
+  for (int64_t i = threadIdx.x; i < n; i += FB) lm = fmin(lm, D[i]);
+  const double m = blk_min_f64(lm, sh);
+  const double thr = (1.0 + tol) * m;
+  const int64_t per = (n + FB - 1) / FB;
+  const int64_t i0 = threadIdx.x * per, i1 = (i0 + per < n) ? i0 + per : n;
+  int64_t c = 0;
+  for (int64_t i = i0; i < i1; i++) c += D[i] <= thr;
+  int64_t tot;
+  int64_t off = blk_excl_scan(c, &tot, sh);
+  for (int64_t i = i0; i < i1; i++)
+    if (D[i] <= thr) out[off++] = (int32_t)i;
+  if (threadIdx.x == 0) *count = tot;
+}
+
+extern "C" __global__ void __launch_bounds__(FB)
+k_taumodel_host(const int32_t* __restrict__ ev, int64_t nev, int nsrc, const double* const* __restrict__ maps,
+                double* __restrict__ vals, int32_t* __restrict__ rank, double* __restrict__ p,
+                int32_t* __restrict__ err) {
+  __shared__ SelScratch sh;
+  __shared__ double sv[IQ_SORT_MAX];
+  __shared__ int32_t dp[IQ_SORT_MAX];
+  if (nev == 1) { if (threadIdx.x == 0) p[0] = 1.0; return; }
+  for (int64_t i = threadIdx.x; i < nev; i += FB) p[i] = 1.0;
+  __syncthreads();
+  for (int j = 0; j < nsrc; j++) {
+    for (int64_t i = threadIdx.x; i < nev; i += FB) vals[i] = maps[j][ev[i]];
+    __syncthreads();
+    if (!blk_dense_rank(vals, nev, rank, sv, dp, sh)) {
+      if (threadIdx.x == 0) set_err(err, IQ_KERR_SORTMAX);
+      return;
+    }
+    tau_accumulate(nev, rank, p, sh);
+  }
+  tau_finish(nev, p);
+}
+
+extern "C" __global__ void __launch_bounds__(FB)
+k_sample_host(const double* __restrict__ p, int64_t n, double u, double* __restrict__ s1,
+              int64_t* __restrict__ out) {
+  __shared__ SelScratch sh;
+  __shared__ double s2[S2MAX];
+  int64_t i = blk_sample(p, n, u, s1, s2, sh);
+  if (threadIdx.x == 0) *out = i;
+}

@@ -1,0 +1,115 @@
+"""ctypes binding of libiqhip.so (the C ABI declared in include/iqhip.h).
+
+There is no fallback: if the HIP library is missing or no GPU is visible,
+every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libiqhip.so")
+
+IQ_OK = 0
+IQ_F32 = 0
+IQ_F64 = 1
+
+STATUS = {
+    -1: "IQ_ERR_ARG",
+    -2: "IQ_ERR_HIP",
+    -3: "IQ_ERR_NOCAND",
+    -4: "IQ_ERR_NOTINDB",
+    -5: "IQ_ERR_ITER",
+    -6: "IQ_ERR_UNSUPPORTED",
+    -7: "IQ_ERR_NOMEM",
+}
+
+# every symbol include/iqhip.h declares
+EXPORTS = (
+    "iq_device_count", "iq_ctx_create", "iq_ctx_destroy", "iq_last_error", "iq_ctx_synchronize",
+    "iq_imfilter", "iq_fastdistance", "iq_band_pick", "iq_relaxation", "iq_taumodel", "iq_sample",
+    "iq_graphcut", "iq_plan_create", "iq_plan_destroy", "iq_plan_run", "iq_plan_results",
+    "iq_plan_last_time", "iq_simulate",
+)
+
+
+class IQError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("ti", ctypes.c_void_p),
+        ("ti_dtype", ctypes.c_int),
+        ("tisize", ctypes.c_int64 * 3),
+        ("tilesize", ctypes.c_int64 * 3),
+        ("ovlsize", ctypes.c_int64 * 3),
+        ("spacing", ctypes.c_int64 * 3),
+        ("ntiles", ctypes.c_int64 * 3),
+        ("padsize", ctypes.c_int64 * 3),
+        ("disabled", ctypes.c_void_p),
+        ("path", ctypes.c_void_p),
+        ("npath", ctypes.c_int64),
+        ("skipped", ctypes.c_void_p),
+        ("hardgrid", ctypes.c_void_p),
+        ("naux", ctypes.c_int),
+        ("aux", ctypes.c_void_p),
+        ("auxti", ctypes.c_void_p),
+        ("tol", ctypes.c_double),
+        ("nreal", ctypes.c_int64),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path: str = LIB_PATH):
+    """Load libiqhip.so (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise IQError(-2, f"{path} not built: run __graft_entry__.build() or make -C imagequilting.jl_amd/csrc")
+        L = ctypes.CDLL(path)
+        vp, i64, f64, c_int = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int
+        P = ctypes.POINTER
+        sig = {
+            "iq_device_count": [P(c_int)],
+            "iq_ctx_create": [c_int, P(vp)],
+            "iq_ctx_destroy": [vp],
+            "iq_ctx_synchronize": [vp],
+            "iq_imfilter": [vp, vp, vp, vp, vp, vp],
+            "iq_fastdistance": [vp, vp, c_int, vp, vp, vp, vp, vp],
+            "iq_band_pick": [vp, vp, i64, f64, vp, vp],
+            "iq_relaxation": [vp, vp, vp, c_int, i64, f64, vp, vp],
+            "iq_taumodel": [vp, vp, i64, vp, vp, c_int, i64, vp],
+            "iq_sample": [vp, vp, i64, f64, vp],
+            "iq_graphcut": [vp, vp, vp, c_int, vp, c_int, vp],
+            "iq_plan_create": [vp, vp, P(vp)],
+            "iq_plan_destroy": [vp],
+            "iq_plan_run": [vp, vp, vp, vp],
+            "iq_plan_results": [vp, vp, vp, vp, vp],
+            "iq_plan_last_time": [vp, P(f64)],
+            "iq_simulate": [vp, vp, vp, vp, vp, vp, vp],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = c_int
+        L.iq_last_error.argtypes = [vp]
+        L.iq_last_error.restype = ctypes.c_char_p
+        _lib = L
+        return L
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != IQ_OK:
+        L = load()
+        msg = L.iq_last_error(ctx)
+        raise IQError(rc, msg.decode() if msg else "")

@@ -1,0 +1,371 @@
+"""Python mirror of ImageQuilting.jl's API over the HIP C ABI.
+
+``iqsim`` keeps the reference's keywords, validation, typing and return
+convention (iqsim.jl:50-315); every distance map, pattern pick, boundary cut
+and paste runs in libiqhip on the GPU.  Host-side work is limited to what the
+reference also does on the host before the tile loop (prep.py).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import warnings
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from . import prep
+
+_f64p = ctypes.c_void_p
+
+
+def _f(a, dtype) -> np.ndarray:
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def _d3(shape) -> np.ndarray:
+    return np.asarray(list(shape) + [1] * (3 - len(shape)), dtype=np.int64)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    """One GPU and one HIP stream (iq_ctx).  Not thread safe."""
+
+    def __init__(self, device: int = 0):
+        self._L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self._L.iq_ctx_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._L.iq_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int):
+        _lib.check(rc, self.handle)
+
+
+_DEFAULT: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = Context(0)
+    return _DEFAULT
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _lib.check(_lib.load().iq_device_count(ctypes.byref(n)))
+    return n.value
+
+
+# ---------------------------------------------------------------------------
+# per-function entry points (each replaces one reference function)
+# ---------------------------------------------------------------------------
+def imfilter(img, krn, ctx: Optional[Context] = None) -> np.ndarray:
+    """imfilter_cpu / imfilter_gpu (imfilter.jl:5-26): valid cross-correlation."""
+    ctx = ctx or default_context()
+    img = _f(img, np.float64)
+    krn = _f(krn, np.float64)
+    oshape = tuple(a - b + 1 for a, b in zip(img.shape, krn.shape))
+    if any(s <= 0 for s in oshape):
+        raise ValueError("kernel larger than image")
+    out = np.zeros(oshape, dtype=np.float64, order="F")
+    ctx.check(ctx._L.iq_imfilter(ctx.handle, _ptr(img), _ptr(_d3(img.shape)), _ptr(krn), _ptr(_d3(krn.shape)),
+                                 _ptr(out)))
+    return out
+
+
+def fastdistance(img, kern, weights=None, ctx: Optional[Context] = None) -> np.ndarray:
+    """fastdistance (utils.jl:5-13): |A2 - 2AB + B2| over the weighted voxels."""
+    ctx = ctx or default_context()
+    img = np.asarray(img)
+    dt = _lib.IQ_F32 if img.dtype == np.float32 else _lib.IQ_F64
+    imgf = _f(img, np.float32 if dt == _lib.IQ_F32 else np.float64)
+    kern = _f(kern, np.float64)
+    w = None if weights is None else _f(np.asarray(weights) != 0, np.uint8)
+    oshape = tuple(a - b + 1 for a, b in zip(imgf.shape, kern.shape))
+    out = np.zeros(oshape, dtype=np.float64, order="F")
+    ctx.check(ctx._L.iq_fastdistance(ctx.handle, _ptr(imgf), dt, _ptr(_d3(imgf.shape)), _ptr(kern), _ptr(w),
+                                     _ptr(_d3(kern.shape)), _ptr(out)))
+    return out
+
+
+def band_pick(D, tol: float, ctx: Optional[Context] = None) -> np.ndarray:
+    """findall(vec(D .<= (1 + tol) * minimum(D))) (iqsim.jl:237), 0-based."""
+    ctx = ctx or default_context()
+    D = np.ascontiguousarray(np.ravel(np.asarray(D, dtype=np.float64), order="F"))
+    out = np.zeros(D.size, dtype=np.int64)
+    cnt = np.zeros(1, dtype=np.int64)
+    ctx.check(ctx._L.iq_band_pick(ctx.handle, _ptr(D), D.size, float(tol), _ptr(out), _ptr(cnt)))
+    return out[: cnt[0]].copy()
+
+
+def _map_ptrs(Ds):
+    arrs = [np.ascontiguousarray(np.ravel(np.asarray(d, dtype=np.float64), order="F")) for d in Ds]
+    ptrs = (ctypes.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+    return arrs, ptrs
+
+
+def relaxation(D, Ds, tol: float, ctx: Optional[Context] = None) -> np.ndarray:
+    """relaxation (relaxation.jl:5-48), 0-based ascending pattern database."""
+    ctx = ctx or default_context()
+    D = np.ascontiguousarray(np.ravel(np.asarray(D, dtype=np.float64), order="F"))
+    arrs, ptrs = _map_ptrs(Ds)
+    out = np.zeros(D.size, dtype=np.int64)
+    cnt = np.zeros(1, dtype=np.int64)
+    ctx.check(ctx._L.iq_relaxation(ctx.handle, _ptr(D), ctypes.cast(ptrs, ctypes.c_void_p), len(arrs), D.size,
+                                   float(tol), _ptr(out), _ptr(cnt)))
+    return out[: cnt[0]].copy()
+
+
+def taumodel(events, D, Ds, ctx: Optional[Context] = None) -> np.ndarray:
+    """taumodel (taumodel.jl:5-45): unnormalised weights of `events`."""
+    ctx = ctx or default_context()
+    ev = np.ascontiguousarray(events, dtype=np.int64)
+    D = np.ascontiguousarray(np.ravel(np.asarray(D, dtype=np.float64), order="F"))
+    arrs, ptrs = _map_ptrs(Ds)
+    p = np.zeros(ev.size, dtype=np.float64)
+    ctx.check(ctx._L.iq_taumodel(ctx.handle, _ptr(ev), ev.size, _ptr(D), ctypes.cast(ptrs, ctypes.c_void_p),
+                                 len(arrs), D.size, _ptr(p)))
+    return p
+
+
+def sample(p, u: float, ctx: Optional[Context] = None) -> int:
+    """Weighted draw of one index (iqsim.jl:243) for a given uniform u."""
+    ctx = ctx or default_context()
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.zeros(1, dtype=np.int64)
+    ctx.check(ctx._L.iq_sample(ctx.handle, _ptr(p), p.size, float(u), _ptr(out)))
+    return int(out[0])
+
+
+def graphcut(A, B, dim: int, ctx: Optional[Context] = None) -> np.ndarray:
+    """graphcut (graphcut.jl:5-84) along 0-based axis `dim`: True = keep A."""
+    ctx = ctx or default_context()
+    A = np.asarray(A)
+    if A.shape != np.asarray(B).shape:
+        raise ValueError("arrays must have the same size for cut")
+    dt = _lib.IQ_F32 if A.dtype == np.float32 else _lib.IQ_F64
+    npdt = np.float32 if dt == _lib.IQ_F32 else np.float64
+    Af, Bf = _f(A, npdt), _f(B, npdt)
+    m = np.zeros(A.shape, dtype=np.uint8, order="F")
+    ctx.check(ctx._L.iq_graphcut(ctx.handle, _ptr(Af), _ptr(Bf), dt, _ptr(_d3(A.shape)), int(dim), _ptr(m)))
+    return m.astype(bool)
+
+
+# ---------------------------------------------------------------------------
+# iqsim (iqsim.jl:50-315)
+# ---------------------------------------------------------------------------
+class Plan:
+    """Device-resident simulation inputs (iq_plan): TI, geometry, path, data."""
+
+    def __init__(self, ctx: Context, desc: _lib.PlanDesc, keep: list, npath: int, nreal: int, padsize, dtype):
+        self.ctx = ctx
+        self._keep = keep
+        self.npath, self.nreal, self.padsize, self.dtype = npath, nreal, tuple(padsize), dtype
+        h = ctypes.c_void_p()
+        ctx.check(ctx._L.iq_plan_create(ctx.handle, ctypes.byref(desc), ctypes.byref(h)))
+        self.handle = h
+
+    def run(self, uniforms: Optional[np.ndarray] = None, forced: Optional[np.ndarray] = None) -> None:
+        u = None if uniforms is None else np.ascontiguousarray(uniforms, dtype=np.float64).reshape(self.nreal, self.npath)
+        f = None if forced is None else np.ascontiguousarray(forced, dtype=np.int64).reshape(self.nreal, self.npath)
+        self.ctx.check(self.ctx._L.iq_plan_run(self.ctx.handle, self.handle, _ptr(u), _ptr(f)))
+
+    def last_time(self) -> float:
+        t = ctypes.c_double(0.0)
+        self.ctx.check(self.ctx._L.iq_plan_last_time(self.handle, ctypes.byref(t)))
+        return t.value
+
+    def results(self, grids=True, chosen=True, cuts=False):
+        padvox = int(np.prod(self.padsize))
+        out = np.zeros(self.nreal * padvox, dtype=self.dtype) if grids else None
+        ch = np.zeros(self.nreal * max(self.npath, 1), dtype=np.int64) if chosen else None
+        cg = np.zeros(self.nreal * padvox, dtype=np.float64) if cuts else None
+        self.ctx.check(self.ctx._L.iq_plan_results(self.ctx.handle, self.handle, _ptr(out), _ptr(ch), _ptr(cg)))
+        res = {}
+        if grids:
+            res["grids"] = [out[r * padvox:(r + 1) * padvox].reshape(self.padsize, order="F") for r in range(self.nreal)]
+        if chosen:
+            res["chosen"] = ch.reshape(self.nreal, max(self.npath, 1))[:, : self.npath]
+        if cuts:
+            res["cuts"] = [cg[r * padvox:(r + 1) * padvox].reshape(self.padsize, order="F") for r in range(self.nreal)]
+        return res
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx._L.iq_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _validate(trainimg, tilesize, simsize, overlap, soft, hard, tol, path, nreal):
+    """The @asserts of iqsim.jl:69-89."""
+    tis = np.shape(trainimg)
+    N = len(tis)
+    if len(tilesize) != N or len(simsize) != N or len(overlap) != N:
+        raise ValueError("tilesize, simsize and overlap need one entry per axis")
+    if not all(0 < t <= s for t, s in zip(tilesize, tis)):
+        raise ValueError("invalid tile size")
+    if not all(s >= t for s, t in zip(simsize, tilesize)):
+        raise ValueError("invalid grid size")
+    if not all(0 < o < 1 for o in overlap):
+        raise ValueError("overlaps must be in range (0,1)")
+    if not (0 < tol <= 1):
+        raise ValueError("tolerance must be in range (0,1]")
+    if path not in ("raster", "dilation", "random"):
+        raise ValueError("invalid simulation path")
+    if nreal <= 0:
+        raise ValueError("invalid number of realizations")
+    for aux, auxti in soft:
+        if not all(a >= s for a, s in zip(np.shape(aux), simsize)):
+            raise ValueError("soft data size < grid size")
+        if np.shape(auxti) != tis:
+            raise ValueError("auxiliary TI must have the same size as TI")
+    if hard:
+        coords = np.asarray([list(k) for k in hard.keys()])
+        if (coords.max(axis=0) >= np.asarray(simsize)).any():
+            raise ValueError("hard data coordinates outside of grid")
+        if (coords.min(axis=0) < 0).any():
+            raise ValueError("hard data coordinates must be non-negative indices")
+
+
+def make_plan(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None, tol=0.1, path="raster",
+              nreal=1, rng=None, ctx: Optional[Context] = None):
+    """Host preparation of iqsim.jl:65-139 and the upload of everything the
+    tile loop reads.  Returns (plan, info)."""
+    ctx = ctx or default_context()
+    T = trainimg
+    tis = np.shape(T)
+    N = len(tis)
+    simsize = tuple(tis) if simsize is None else tuple(int(s) for s in simsize)
+    overlap = tuple(1 / 6 for _ in range(N)) if overlap is None else tuple(overlap)
+    hard = dict(hard or {})
+    soft = list(soft)
+    rng = np.random.default_rng() if rng is None else rng
+    tilesize = tuple(int(t) for t in tilesize)
+    _validate(T, tilesize, simsize, overlap, soft, hard, tol, path, nreal)
+    geo = prep.geometry(tis, tilesize, simsize, overlap)
+    if any(t > 1 and o == 1 for t, o in zip(tilesize, geo["ovlsize"])):
+        warnings.warn("Overlaps with only 1 voxel, check tilesize/overlap configuration")  # iqsim.jl:95-97
+    TI = prep.prepare(T)
+    dtype = np.float32 if TI.dtype == np.float32 else np.float64
+    TI = _f(TI, dtype)
+    SOFT = [(_f(prep.prepare(prep.pad_symmetric(a, geo["padsize"])), np.float64), _f(prep.prepare(b), np.float64))
+            for a, b in soft]
+    disabled = _f(prep.finddisabled(T, geo), np.uint8)
+    skipped, datainds = prep.findskipped(hard, geo)
+    simpath = prep.genpath(rng, geo["ntiles"], path, list(datainds))
+    nt = int(np.prod(geo["ntiles"]))
+    skp = np.zeros(nt, dtype=np.uint8)
+    for s_ in skipped:
+        skp[s_] = 1
+    pth = np.ascontiguousarray(simpath, dtype=np.int64)
+    hardgrid = None
+    if hard:
+        hardgrid = np.full(geo["padsize"], np.nan, dtype=np.float64, order="F")
+        for k, v in hard.items():
+            hardgrid[tuple(k)] = v
+    d = _lib.PlanDesc()
+    d.ti = TI.ctypes.data
+    d.ti_dtype = _lib.IQ_F32 if dtype == np.float32 else _lib.IQ_F64
+    d.tisize[:] = list(_d3(tis))
+    for name in ("tilesize", "ovlsize", "spacing", "ntiles", "padsize"):
+        getattr(d, name)[:] = list(_d3(geo[name]))
+    d.disabled = disabled.ctypes.data
+    d.path = pth.ctypes.data
+    d.npath = len(simpath)
+    d.skipped = skp.ctypes.data
+    d.hardgrid = None if hardgrid is None else hardgrid.ctypes.data
+    d.naux = len(SOFT)
+    aux_p = (ctypes.c_void_p * max(1, len(SOFT)))(*[a.ctypes.data for a, _ in SOFT])
+    auxti_p = (ctypes.c_void_p * max(1, len(SOFT)))(*[b.ctypes.data for _, b in SOFT])
+    d.aux = ctypes.cast(aux_p, ctypes.c_void_p)
+    d.auxti = ctypes.cast(auxti_p, ctypes.c_void_p)
+    d.tol = float(tol)
+    d.nreal = int(nreal)
+    keep = [TI, SOFT, disabled, pth, skp, hardgrid, aux_p, auxti_p]
+    plan = Plan(ctx, d, keep, len(simpath), int(nreal), geo["padsize"], dtype)
+    info = dict(geo=geo, path=simpath, skipped=skipped, datainds=datainds, hard=hard, T=prep.element_type(T),
+                rng=rng)
+    return plan, info
+
+
+def finish(info, grids, cuts=None):
+    """Hard-data overwrite, crop and output typing (iqsim.jl:287-308)."""
+    geo, hard, T = info["geo"], info["hard"], info["T"]
+    reals, outcuts, voxs = [], [], []
+    crop = tuple(slice(0, s) for s in geo["simsize"])
+    for r, g in enumerate(grids):
+        g = g.copy()
+        cg = None
+        if cuts is not None:
+            cg = cuts[r].copy()
+            voxs.append(float(cg.sum()) / geo["ovlvol"] if geo["ovlvol"] > 0 else float("nan"))  # iqsim.jl:288
+        for k, v in hard.items():  # iqsim.jl:291-294
+            g[tuple(k)] = v
+            if cg is not None and isinstance(v, float) and math.isnan(v):
+                cg[tuple(k)] = v
+        reals.append(prep.unprepare(T, g[crop]))
+        if cg is not None:
+            outcuts.append(cg[crop])
+    return reals, outcuts, voxs
+
+
+def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard: Optional[Dict] = None, tol=0.1,
+          path="raster", nreal=1, debug=False, showprogress=False, rng=None, uniforms=None,
+          ctx: Optional[Context] = None, return_trace=False):
+    """Image quilting simulation (iqsim.jl:50-315).
+
+    ``hard`` maps 0-based coordinate tuples to values (NaN marks inactive
+    grid voxels); ``soft`` is a list of (data, dataTI) pairs.  Draws come from
+    ``rng`` (a numpy Generator) unless ``uniforms`` (nreal x len(path)) is
+    given.  Returns ``reals`` or, with ``debug``, ``(reals, cuts, voxs)``.
+    """
+    plan, info = make_plan(trainimg, tilesize, simsize, overlap=overlap, soft=soft, hard=hard, tol=tol, path=path,
+                           nreal=nreal, rng=rng, ctx=ctx)
+    try:
+        if uniforms is None:
+            uniforms = info["rng"].random((int(nreal), plan.npath))
+        plan.run(uniforms)
+        res = plan.results(grids=True, chosen=return_trace, cuts=debug)
+    finally:
+        plan.close()
+    reals, cuts, voxs = finish(info, res["grids"], res.get("cuts"))
+    out = (reals, cuts, voxs) if debug else reals
+    if return_trace:
+        return out, dict(chosen=res["chosen"], path=info["path"], geo=info["geo"], uniforms=uniforms)
+    return out
+
+
+def voxelreuse(trainimg, tilesize, *, overlap=None, nreal=10, **kwargs):
+    """voxelreuse (voxelreuse.jl:18-41): mean and std of the voxel reuse."""
+    N = len(tilesize)
+    overlap = tuple(1 / 6 for _ in range(N)) if overlap is None else tuple(overlap)
+    ovlsize = [int(math.ceil(o * t)) for o, t in zip(overlap, tilesize)]
+    ntiles = [2 if o > 1 else 1 for o in ovlsize]
+    simsize = tuple(n * (t - o) + o for n, t, o in zip(ntiles, tilesize, ovlsize))
+    _, _, voxs = iqsim(trainimg, tilesize, simsize, overlap=overlap, nreal=nreal, debug=True, **kwargs)
+    mu = float(np.mean(voxs))
+    sd = float(np.std(voxs, ddof=1)) if len(voxs) > 1 else float("nan")
+    return mu, sd

@@ -1,0 +1,150 @@
+/*
+ * iqhip.h -- C ABI of the MI355X (gfx950) image-quilting hot path.
+ *
+ * Drop-in boundary for ImageQuilting.jl's `iqsim` tile loop.  The reference
+ * has no formal plugin API; its backend seam is three load-time function
+ * constants (imfilter.jl:28 `imfilter_kernel`, utils.jl:61 `array_kernel`,
+ * utils.jl:67 `view_kernel`) called from `fastdistance` (utils.jl:5-13) and
+ * the tile loop (iqsim.jl:163-312).  Each entry point below replaces one
+ * reference function, so each can be checked on its own, and
+ * iq_plan_* / iq_simulate replace the whole loop.
+ *
+ * Conventions
+ *   * Arrays are column-major (first axis fastest), exactly Julia's memory
+ *     layout, and always described by three extents (pass 1 for missing
+ *     trailing axes).
+ *   * Indices are 0-based (Julia wrapper adds 1).
+ *   * Host buffers are borrowed for the duration of a call only.
+ *   * Every function returns an iq_status; iq_last_error() gives the text.
+ *     No C++ exception crosses the ABI and no call waits forever.
+ *   * One iq_ctx = one device + one HIP stream; calls on one context are not
+ *     thread safe, distinct contexts are independent.
+ */
+#ifndef IQHIP_H
+#define IQHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum iq_status {
+  IQ_OK = 0,
+  IQ_ERR_ARG = -1,          /* invalid argument (mirrors the @asserts of iqsim.jl:69-89) */
+  IQ_ERR_HIP = -2,          /* HIP runtime error */
+  IQ_ERR_NOCAND = -3,       /* every candidate pattern is disabled (iqsim.jl:237 / relaxation.jl:20-36) */
+  IQ_ERR_NOTINDB = -4,      /* a forced pick is not in the pattern database */
+  IQ_ERR_ITER = -5,         /* min-cut iteration guard reached */
+  IQ_ERR_UNSUPPORTED = -6,  /* size beyond what this build handles (message names the limit) */
+  IQ_ERR_NOMEM = -7
+} iq_status;
+
+typedef enum iq_dtype { IQ_F32 = 0, IQ_F64 = 1 } iq_dtype;
+
+typedef struct iq_ctx iq_ctx;
+typedef struct iq_plan iq_plan;
+
+/* ---- context --------------------------------------------------------- */
+/* Replaces the load-time backend choice `CUDA.functional()`
+ * (imfilter.jl:28, utils.jl:61, utils.jl:67) with a runtime handle. */
+int iq_device_count(int* count);
+int iq_ctx_create(int device, iq_ctx** out);
+int iq_ctx_destroy(iq_ctx* ctx);
+const char* iq_last_error(const iq_ctx* ctx); /* ctx may be NULL: last error of any context on this thread */
+int iq_ctx_synchronize(iq_ctx* ctx);
+
+/* ---- per-function entry points (host buffers in, host buffers out) ---- */
+
+/* imfilter_cpu / imfilter_gpu (imfilter.jl:5-26): valid cross-correlation
+ * out[c] = sum_j img[c+j] * krn[j], out extents = idims - kdims + 1. */
+int iq_imfilter(iq_ctx* ctx, const double* img, const int64_t idims[3], const double* krn,
+                const int64_t kdims[3], double* out);
+
+/* fastdistance (utils.jl:5-13): D[c] = |A2 - 2 AB + B2| over the voxels
+ * where weights != 0 (weights may be NULL = all ones).  img is float or
+ * double per `img_dtype`; D is FP64, exact to the project's FP64 definition
+ * (fused-multiply-add chains in ascending column-major tile order). */
+int iq_fastdistance(iq_ctx* ctx, const void* img, int img_dtype, const int64_t idims[3],
+                    const double* kern, const uint8_t* weights, const int64_t kdims[3], double* out);
+
+/* Tol-band pick (iqsim.jl:237, empty-auxiliary branch):
+ * ascending indices i with D[i] <= (1 + tol) * minimum(D). */
+int iq_band_pick(iq_ctx* ctx, const double* D, int64_t n, double tol, int64_t* out_idx,
+                 int64_t* out_count);
+
+/* relaxation (relaxation.jl:5-48): ascending pattern database for primary
+ * distances D and `naux` auxiliary maps Ds[0..naux). */
+int iq_relaxation(iq_ctx* ctx, const double* D, const double* const* Ds, int naux, int64_t n,
+                  double tol, int64_t* out_idx, int64_t* out_count);
+
+/* taumodel (taumodel.jl:5-45): unnormalised probabilities of `events`
+ * (indices into the maps) from the primary map D and auxiliary maps Ds. */
+int iq_taumodel(iq_ctx* ctx, const int64_t* events, int64_t nev, const double* D,
+                const double* const* Ds, int naux, int64_t n, double* out_p);
+
+/* Weighted draw (iqsim.jl:243, StatsBase direct method) with a caller-drawn
+ * uniform u in [0,1): the first i whose cumulative weight reaches u * sum(p),
+ * cumulative sums taken in fixed 64-element blocks over three levels. */
+int iq_sample(iq_ctx* ctx, const double* p, int64_t n, double u, int64_t* out_index);
+
+/* graphcut (graphcut.jl:5-84): mask over the slab A/B (extents dims, type
+ * dtype) cut along 0-based axis `dim`: 1 where the voxel cannot reach the
+ * sink in the residual graph of a maximum flow (keep A), 0 elsewhere. */
+int iq_graphcut(iq_ctx* ctx, const void* A, const void* B, int dtype, const int64_t dims[3], int dim,
+                uint8_t* out_mask);
+
+/* ---- whole simulation (iqsim.jl:163-300) ------------------------------ */
+/* Everything the tile loop reads, prepared on the host exactly as
+ * imagepreproc / finddisabled / findskipped / genpath do (utils.jl:69-204);
+ * geometry as iqsim.jl:92-115. */
+typedef struct iq_plan_desc {
+  const void* ti;               /* prepared training image (NaN -> 0), tisize */
+  int ti_dtype;                 /* IQ_F32 or IQ_F64 (element type of the prepared TI) */
+  int64_t tisize[3];
+  int64_t tilesize[3];
+  int64_t ovlsize[3];
+  int64_t spacing[3];
+  int64_t ntiles[3];
+  int64_t padsize[3];
+  const uint8_t* disabled;      /* distsize = tisize - tilesize + 1 (finddisabled, utils.jl:115) */
+  const int64_t* path;          /* npath tile linear indices (genpath, utils.jl:158) */
+  int64_t npath;
+  const uint8_t* skipped;       /* prod(ntiles) flags (findskipped, utils.jl:131) */
+  const double* hardgrid;       /* padsize, NaN where no hard datum; NULL = no hard data */
+  int naux;                     /* soft pairs (iqsim.jl:222-227) */
+  const double* const* aux;     /* naux grids of padsize (padded + prepared) */
+  const double* const* auxti;   /* naux training images of tisize (prepared) */
+  double tol;
+  int64_t nreal;
+} iq_plan_desc;
+
+/* Upload everything once (the TI stays resident in HBM). */
+int iq_plan_create(iq_ctx* ctx, const iq_plan_desc* desc, iq_plan** out);
+int iq_plan_destroy(iq_plan* plan);
+
+/* Run all realizations.  uniforms: nreal x npath (row r holds realization
+ * r's draws, indexed by path position).  forced: NULL, or nreal x npath
+ * pattern indices to replay (-1 = draw).  Results stay on the device until
+ * iq_plan_results; the call returns once the work is queued and checked. */
+int iq_plan_run(iq_ctx* ctx, iq_plan* plan, const double* uniforms, const int64_t* forced);
+
+/* Copy results: out = nreal x padsize in the TI element type (before the
+ * hard-data overwrite and crop of iqsim.jl:291-303), chosen = nreal x npath
+ * picked pattern indices (-1 for skipped tiles), cutgrid = nreal x padsize
+ * boundary-cut values 0/1 (iqsim.jl:281; NULL to skip).  Any may be NULL. */
+int iq_plan_results(iq_ctx* ctx, iq_plan* plan, void* out, int64_t* chosen, double* cutgrid);
+
+/* Seconds of device time of the last iq_plan_run (HIP events on the
+ * context's stream). */
+int iq_plan_last_time(iq_plan* plan, double* seconds);
+
+/* Convenience: create + run + results + destroy. */
+int iq_simulate(iq_ctx* ctx, const iq_plan_desc* desc, const double* uniforms, const int64_t* forced,
+                void* out, int64_t* chosen, double* cutgrid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IQHIP_H */

@@ -1,0 +1,91 @@
+"""End-to-end parity of the device tile loop (iq_plan_run) with the oracle.
+
+Both sides receive the same uniforms, so every pattern pick, every cut and
+every pasted voxel must agree exactly.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import iqhip
+import iqref
+
+pytestmark = pytest.mark.gpu
+
+
+def _smooth(rng, shape, width=5, dtype=np.float32):
+    x = rng.standard_normal(shape)
+    k = np.ones(width) / width
+    for ax in range(len(shape)):
+        if shape[ax] > width:
+            x = np.apply_along_axis(lambda v: np.convolve(v, k, mode="same"), ax, x)
+    return x.astype(dtype)
+
+
+def _compare(T, tile, sim, seed=0, **kw):
+    (reals, cuts, voxs), tr = iqhip.iqsim(T, tile, sim, rng=np.random.default_rng(seed), debug=True,
+                                          return_trace=True, **kw)
+    (rr, rc, rv), rtr = iqref.iqsim(T, tile, sim, uniforms=tr["uniforms"], debug=True, return_trace=True, **kw)
+    assert list(tr["path"]) == list(rtr["path"])
+    assert np.array_equal(tr["chosen"], rtr["chosen"])
+    for a, b in zip(reals, rr):
+        a = np.asarray(a, dtype=object if a.dtype == object else a.dtype)
+        if a.dtype == object:
+            assert all((x is None and y is None) or x == y for x, y in zip(a.ravel(), np.asarray(b).ravel()))
+        else:
+            assert np.array_equal(a, b, equal_nan=True)
+    for a, b in zip(cuts, rc):
+        assert np.array_equal(a, b, equal_nan=True)
+    assert voxs == rv
+    return reals, tr
+
+
+def test_sim_2d_binary_f64():
+    rng = np.random.default_rng(1)
+    x = _smooth(rng, (60, 60), 7, np.float64)
+    T = (x > np.quantile(x, 0.7)).astype(np.float64)
+    print("2d binary", flush=True)
+    _compare(T, (15, 15), (80, 80), tol=0.1)
+
+
+def test_sim_3d_continuous_f32():
+    rng = np.random.default_rng(2)
+    T = _smooth(rng, (30, 30, 12), 5, np.float32)
+    print("3d continuous f32 (z overlap of one voxel: no z cut)", flush=True)
+    with pytest.warns(UserWarning):
+        _compare(T, (10, 10, 5), (40, 40, 12), overlap=(1 / 5, 1 / 5, 1 / 5), tol=0.1)
+
+
+def test_sim_3d_full_overlap_f64_nreal():
+    rng = np.random.default_rng(3)
+    T = _smooth(rng, (20, 20, 16), 3, np.float64)
+    print("3d f64 nreal=2", flush=True)
+    _compare(T, (8, 8, 8), (20, 20, 20), overlap=(1 / 4, 1 / 4, 1 / 4), nreal=2)
+
+
+def test_sim_categorical_int():
+    rng = np.random.default_rng(4)
+    x = _smooth(rng, (30, 30, 8), 5, np.float64)
+    T = np.digitize(x, np.quantile(x, [0.65, 0.9])).astype(np.int64)
+    print("categorical", flush=True)
+    reals, _ = _compare(T, (10, 10, 4), (35, 35, 8), overlap=(1 / 4, 1 / 4, 1 / 4))
+    assert set(v for v in reals[0].ravel()) <= set(np.unique(T).tolist())
+
+
+def test_sim_soft_and_hard():
+    rng = np.random.default_rng(5)
+    T = _smooth(rng, (40, 40), 5, np.float32)
+    aux_ti = _smooth(rng, (40, 40), 3, np.float32)
+    aux = _smooth(rng, (50, 50), 3, np.float32)
+    hard = {(int(i), int(j)): float(T[i % 40, j % 40]) for i, j in zip(rng.integers(0, 50, 12), rng.integers(0, 50, 12))}
+    print("soft + hard", flush=True)
+    _compare(T, (10, 10), (50, 50), soft=[(aux, aux_ti)], hard=hard, tol=0.2)
+
+
+def test_sim_masked_ti_and_grid():
+    T = np.ones((20, 20, 20))
+    T[:, 4, :] = np.nan
+    shape = {(i, 4, k): float("nan") for i in range(20) for k in range(20)}
+    print("masked ti + grid", flush=True)
+    _compare(T, (10, 10, 10), (20, 20, 20), hard=shape)

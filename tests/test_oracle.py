@@ -1,0 +1,331 @@
+"""Pins the FP64 oracle (oracle/iqref.c + iqref.py) without the GPU.
+
+Each function is checked against something that does not share its code:
+golden vectors computed with scipy / exact rationals / brute force
+(tests/golden/make_golden.py), closed forms, networkx max-flow, and the
+behavioural known-answer tests of the reference (test/runtests.jl).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy.signal import correlate
+
+import iqref
+
+G = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
+
+
+# ------------------------------------------------------------------ A5 imfilter
+def test_imfilter_golden():
+    got = iqref.imfilter(G["imf_img"], G["imf_krn"])
+    assert np.allclose(got, G["imf_out"], rtol=1e-13, atol=0)
+
+
+def test_imfilter_vs_fft_restatement_of_gpu_path():
+    # imfilter.jl:18-25: real(ifft(fft(img) .* conj(fft(padkrn)))) cropped to the valid region
+    rng = np.random.default_rng(0)
+    img, krn = rng.random((200, 100)), rng.random((30, 10))
+    pad = np.zeros_like(img)
+    pad[: krn.shape[0], : krn.shape[1]] = krn
+    fft = np.real(np.fft.ifftn(np.fft.fftn(img) * np.conj(np.fft.fftn(pad))))[:171, :91]
+    got = iqref.imfilter(img, krn)
+    assert np.abs(got - fft).max() < 1e-2  # the tolerance of runtests.jl:152
+    assert np.abs(got - fft).max() < 1e-9  # and far tighter in FP64
+    assert np.allclose(got, correlate(img, krn, mode="valid"), rtol=1e-12)
+
+
+# -------------------------------------------------------------- A4 fastdistance
+def test_fastdistance_golden_exact():
+    got = iqref.fastdistance(G["fd_ti"], G["fd_kern"], G["fd_w"])
+    assert np.array_equal(got, G["fd_out"])  # dyadic inputs: every step is exact
+
+
+def test_fastdistance_closed_form():
+    rng = np.random.default_rng(1)
+    img = rng.standard_normal((14, 11, 6))
+    k = rng.standard_normal((5, 4, 3))
+    w = rng.random(k.shape) < 0.5
+    got = iqref.fastdistance(img, k, w)
+    ref = np.zeros(got.shape)
+    for c in np.ndindex(*got.shape):
+        win = img[c[0]:c[0] + 5, c[1]:c[1] + 4, c[2]:c[2] + 3]
+        ref[c] = (w * (win - k) ** 2).sum()
+    scale = (w * (img.max() ** 2 + k ** 2)).sum()
+    assert np.abs(got - ref).max() <= 1e-13 * scale
+    # a dropped B2 term or a wrong sign shows up far above that bound
+    assert np.abs(got - ref).max() < 1e-10
+
+
+def test_fastdistance_categorical_is_exact_integer():
+    rng = np.random.default_rng(2)
+    img = rng.integers(0, 3, (20, 20)).astype(float)
+    k = rng.integers(0, 3, (6, 6)).astype(float)
+    got = iqref.fastdistance(img, k)
+    ref = np.array([[((img[i:i + 6, j:j + 6] - k) ** 2).sum() for j in range(15)] for i in range(15)])
+    assert np.array_equal(got, ref)
+
+
+# ----------------------------------------------------------------- A10 band
+def test_band_golden():
+    assert np.array_equal(iqref.band(G["band_D"], 0.1), G["band_out"])
+
+
+def test_band_empty_mask_selects_all_enabled():
+    D = np.zeros(50)
+    D[[3, 7]] = np.inf
+    assert np.array_equal(iqref.band(D, 0.1), np.setdiff1d(np.arange(50), [3, 7]))
+
+
+# ----------------------------------------------------------- A11 relaxation
+def test_relaxation_golden():
+    got = iqref.relaxation(G["rel_D"], [G["rel_A"]], 0.1)
+    assert np.array_equal(got, G["rel_out"])
+
+
+def _relax_sets(D, Ds, tol):
+    """Set formulation: top-k prefixes of the stable (value, index) orders."""
+    en = int(np.isfinite(D).sum())
+    allzero = np.all(D[np.isfinite(D)] == 0)
+    dbsize = en if allzero else int(math.ceil(tol * en))
+    top = lambda v, k: set(np.lexsort((np.arange(v.size), v))[:k].tolist())
+    over = top(D, dbsize)
+    frac = 0.1 * (dbsize / en)
+    while True:
+        k = int(math.ceil(frac * en))
+        res = set(over)
+        for d in Ds:
+            res &= top(d, k)
+            if not res:
+                break
+        if res:
+            return np.asarray(sorted(res))
+        frac = min(frac + 0.1, 1.0)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_relaxation_literal_equals_set_formulation(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(20, 600))
+    D = np.round(rng.random(n) * 12)
+    Ds = [np.round(rng.random(n) * 8) for _ in range(int(rng.integers(1, 4)))]
+    dis = rng.random(n) < 0.15
+    D[dis] = np.inf
+    for d in Ds:
+        d[dis] = np.inf
+    tol = [0.05, 0.1, 0.3][seed % 3]
+    got = iqref.relaxation(D, Ds, tol)
+    assert np.array_equal(got, _relax_sets(D, Ds, tol))
+    assert len(got) > 0 and np.all(np.diff(got) > 0)
+
+
+# ------------------------------------------------------------- A12 taumodel
+def test_taumodel_golden():
+    got = iqref.taumodel(G["tau_ev"], G["tau_D1"], [G["tau_D2"]])
+    # the reference rounds each operation; the golden values round once
+    assert np.allclose(got, G["tau_p"], rtol=1e-13)
+
+
+def test_taumodel_single_event():
+    assert iqref.taumodel([4], np.arange(10.0), [])[0] == 1.0
+
+
+def test_taumodel_single_source_is_rank_weight():
+    rng = np.random.default_rng(3)
+    D = np.round(rng.random(100) * 7)
+    ev = np.arange(100)
+    from scipy.stats import rankdata
+    r = rankdata(D, method="dense")
+    P = (100 - r + 1) / (100 - r + 1).sum()
+    assert np.allclose(iqref.taumodel(ev, D, []), P, rtol=1e-13)
+
+
+def test_taumodel_tied_source_is_neutral_bitwise():
+    rng = np.random.default_rng(4)
+    D = np.round(rng.random(80) * 7)
+    ev = np.sort(rng.choice(80, 40, replace=False))
+    a = iqref.taumodel(ev, D, [])
+    b = iqref.taumodel(ev, D, [np.full(80, 3.0)])
+    assert np.array_equal(a, b)
+
+
+# --------------------------------------------------------------- A13 sample
+def test_sample_small_is_first_crossing():
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 5, 64):
+        p = rng.random(n)
+        for u in rng.random(20):
+            t = u * p.sum()
+            cum = np.cumsum(p)
+            i = iqref.sample(p, u)
+            expect = int(np.searchsorted(cum, t, side="left"))
+            assert i == min(expect, n - 1)
+
+
+def test_sample_blocked_matches_exact_rationals_within_rounding():
+    from fractions import Fraction
+    rng = np.random.default_rng(6)
+    p = rng.random(5000)
+    tot = sum(Fraction(x) for x in p)
+    for u in rng.random(30):
+        i = iqref.sample(p, float(u))
+        t = Fraction(float(u)) * tot
+        before = sum(Fraction(x) for x in p[:i])
+        slack = tot * Fraction(1, 10 ** 12)
+        assert before <= t + slack and before + Fraction(p[i]) >= t - slack
+
+
+# ------------------------------------------------------------- A16 graphcut
+def test_graphcut_reference_convention():
+    # runtests.jl:112-119
+    C = iqref.graphcut(np.ones((20, 20)), np.ones((20, 20)), 0)
+    assert C[:-1, :].all() and not C[-1, :].any()
+    C = iqref.graphcut(np.ones((20, 20)), np.ones((20, 20)), 1)
+    assert C[:, :-1].all() and not C[:, -1].any()
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_graphcut_golden_brute_force(k):
+    A, B, dim = G[f"gc{k}_A"], G[f"gc{k}_B"], int(G[f"gc{k}_dim"])
+    assert np.array_equal(iqref.graphcut(A, B, dim), G[f"gc{k}_mask"])
+    assert np.array_equal(iqref.graphcut_ek(A, B, dim), G[f"gc{k}_mask"])
+
+
+def _nx_mask(A, B, dim):
+    import networkx as nx
+    from networkx.algorithms.flow import boykov_kolmogorov
+    shape = A.shape
+    Gr = nx.DiGraph()
+    idx = list(np.ndindex(*shape))
+    for d in range(A.ndim):
+        for c in idx:
+            if c[d] + 1 >= shape[d]:
+                continue
+            v = tuple(c[q] + (q == d) for q in range(A.ndim))
+            w = tuple(c[q] + 2 * (q == d) for q in range(A.ndim))
+            ok = w[d] < shape[d]
+            Du, Dv = abs(A[c] - B[c]), abs(A[v] - B[v])
+            gAu, gBu = abs(A[v] - A[c]), abs(B[v] - B[c])
+            gAv = abs(A[w] - A[v]) if ok else gAu
+            gBv = abs(B[w] - B[v]) if ok else gBu
+            cap = (Du + Dv) / (gAu + gAv + gBu + gBv + 2.0 ** -52)
+            Gr.add_edge(c, v, capacity=cap)
+            Gr.add_edge(v, c, capacity=cap)
+    for c in idx:
+        if c[dim] == 0:
+            Gr.add_edge("s", c)  # no capacity attribute = infinite
+        if c[dim] == shape[dim] - 1:
+            Gr.add_edge(c, "t")
+    R = boykov_kolmogorov(Gr, "s", "t")
+    co = {"t"}
+    stack = ["t"]
+    while stack:
+        x = stack.pop()
+        for y in R.predecessors(x):
+            e = R[y][x]
+            if y not in co and e["capacity"] - e["flow"] > 1e-12 * max(1.0, e["capacity"] if np.isfinite(e["capacity"]) else 1.0):
+                co.add(y)
+                stack.append(y)
+    m = np.ones(shape, dtype=bool)
+    for c in idx:
+        if c in co:
+            m[c] = False
+    return m
+
+
+@pytest.mark.parametrize("shape,dim", [((5, 30), 0), ((4, 20, 5), 0), ((20, 4, 5), 1), ((6, 12, 4), 0), ((8, 16), 0)])
+def test_graphcut_vs_networkx_bk(shape, dim):
+    rng = np.random.default_rng(sum(shape) + dim)
+    for _ in range(3):
+        A = rng.standard_normal(shape)
+        B = rng.standard_normal(shape)
+        assert np.array_equal(iqref.graphcut(A, B, dim), _nx_mask(A, B, dim))
+
+
+def test_graphcut_float32_arithmetic_differs_from_float64_only_by_rounding():
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((6, 20)).astype(np.float32)
+    B = rng.standard_normal((6, 20)).astype(np.float32)
+    m32 = iqref.graphcut(A, B, 0, is_f32=True)
+    assert np.array_equal(m32, iqref.graphcut_ek(A, B, 0, is_f32=True))
+
+
+# -------------------------------------------------------------- A2 geometry
+@pytest.mark.parametrize("ti,tile,sim,ov,exp", [
+    ((200, 200), (30, 30), (400, 400), (1 / 6, 1 / 6), dict(ovlsize=(5, 5), spacing=(25, 25), ntiles=(16, 16), padsize=(405, 405), distsize=(171, 171))),
+    ((100, 100, 50), (20, 20, 5), (200, 200, 50), (0.2, 0.2, 0.2), dict(ovlsize=(4, 4, 1), spacing=(16, 16, 4), ntiles=(13, 13, 13), padsize=(212, 212, 53), distsize=(81, 81, 46))),
+    ((150, 150, 30), (24, 24, 6), (300, 300, 30), (0.25, 0.25, 0.25), dict(ovlsize=(6, 6, 2), spacing=(18, 18, 4), ntiles=(17, 17, 8), padsize=(312, 312, 34), distsize=(127, 127, 25))),
+    ((512, 512), (48, 48), (1024, 1024), (1 / 6, 1 / 6), dict(ovlsize=(8, 8), spacing=(40, 40), ntiles=(26, 26), padsize=(1048, 1048), distsize=(465, 465))),
+    ((256, 256, 64), (32, 32, 8), (384, 384, 64), (0.25, 0.25, 0.25), dict(ovlsize=(8, 8, 2), spacing=(24, 24, 6), ntiles=(16, 16, 11), padsize=(392, 392, 68), distsize=(225, 225, 57))),
+])
+def test_geometry_table(ti, tile, sim, ov, exp):
+    g = iqref.geometry(ti, tile, sim, ov)
+    for k, v in exp.items():
+        assert g[k] == v, k
+
+
+# ------------------------------------------- known-answer tests (runtests.jl)
+def test_homogeneous_ti_gives_homogeneous_output():
+    TI = np.ones((20, 20, 20))
+    r = iqref.iqsim(TI, (10, 10, 10))  # runtests.jl:11-14
+    assert r[0].dtype == np.float64 and np.array_equal(r[0], TI)
+
+
+def test_float32_preserved():
+    TI = np.random.default_rng(0).random((20, 20, 20)).astype(np.float32)
+    assert iqref.iqsim(TI, (10, 10, 10))[0].dtype == np.float32  # runtests.jl:17-19
+
+
+def test_categories_from_training_image():
+    TI = np.random.default_rng(1).integers(1, 4, (20, 20, 20))
+    r = iqref.iqsim(TI, (10, 10, 10))  # runtests.jl:23-26
+    assert set(r[0].ravel().tolist()) <= set(np.unique(TI).tolist())
+
+
+def test_soft_trend_ordering():
+    TI = np.concatenate([np.zeros((10, 20, 1)), np.ones((10, 20, 1))], axis=0)
+    trend = np.concatenate([np.zeros((20, 10, 1)), np.ones((20, 10, 1))], axis=1)
+    r = iqref.iqsim(TI, (10, 10, 1), soft=[(trend, TI)], tol=1)  # runtests.jl:31-34
+    assert r[0][:, :10, :].mean() <= r[0][:, 10:, :].mean()
+
+
+def test_hard_data_honoured():
+    TI = np.ones((20, 20, 20))
+    obs = np.zeros((20, 20, 20))
+    data = {c: 0.0 for c in np.ndindex(20, 20, 20)}
+    assert np.array_equal(iqref.iqsim(TI, (10, 10, 10), hard=data)[0], obs)  # runtests.jl:52-56
+    r = iqref.iqsim(TI, (10, 10, 10), hard={(19, 19, 19): 10}, nreal=3)  # runtests.jl:59-64
+    assert all(x[19, 19, 19] == 10 for x in r)
+
+
+def test_masked_sphere():
+    TI = np.ones((20, 20, 20))
+    shape, active = {}, np.ones((20, 20, 20), dtype=bool)
+    for c in np.ndindex(20, 20, 20):
+        if (c[0] - 9) ** 2 + (c[1] - 9) ** 2 + (c[2] - 9) ** 2 < 25:
+            shape[c] = float("nan")
+            active[c] = False
+    r = iqref.iqsim(TI, (10, 10, 10), hard=shape)[0]  # runtests.jl:69-80
+    assert np.isnan(r[~active]).all() and not np.isnan(r[active]).any()
+
+
+def test_masked_training_image():
+    TI = np.ones((20, 20, 20))
+    TI[:, 4, :] = np.nan
+    assert np.array_equal(iqref.iqsim(TI, (10, 10, 10))[0], np.ones((20, 20, 20)))  # runtests.jl:83-86
+    TI[0, 4, :] = 0
+    assert np.array_equal(iqref.iqsim(TI, (10, 10, 10))[0], np.ones((20, 20, 20)))  # runtests.jl:87-89
+
+
+def test_voxel_reuse_range():
+    TI = np.ones((20, 20, 20))
+    _, _, voxs = iqref.iqsim(TI, (10, 10, 10), overlap=(1 / 3, 1 / 3, 1 / 3), debug=True)  # runtests.jl:109-110
+    assert 0 <= voxs[0] <= 1
+
+
+def test_genpath_lengths_and_data_first():
+    for kind in ("raster", "dilation", "random"):
+        assert len(iqref.genpath(np.random.default_rng(123), (10, 10, 10), kind, [])) == 1000  # runtests.jl:124-128
+    p = iqref.genpath(np.random.default_rng(123), (10, 10, 10), "raster", [0, 999])  # runtests.jl:131-133
+    assert p[:2] in ([0, 999], [999, 0]) and sorted(p) == list(range(1000))
