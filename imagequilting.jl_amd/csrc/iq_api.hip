@@ -325,6 +325,12 @@ struct iq_plan {
   float last_ms = 0.0f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   size_t cut_lds_max = 0;
+  // scan profiling
+  bool prof = false;
+  std::vector<hipEvent_t> prof_ev;
+  int64_t prof_used = 0;
+  double prof_flop = 0.0, prof_sec = 0.0;
+  int64_t prof_launches = 0;
 };
 
 namespace {
@@ -660,8 +666,23 @@ void run_step(iq_plan* pl, int64_t r, int64_t i, double u, int64_t forced) {
                          pl->gmin64);
       HCL();
       if (pl->fast) {
+        if (pl->prof) {
+          if (pl->prof_used + 2 > (int64_t)pl->prof_ev.size()) {
+            hipEvent_t a, b;
+            HC(hipEventCreate(&a));
+            HC(hipEventCreate(&b));
+            pl->prof_ev.push_back(a);
+            pl->prof_ev.push_back(b);
+          }
+          HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));
+        }
         launch_scan(pl->mask_cfg[st.mask], g, pl->mask_lds[st.mask], s, pl->ti32, dm, pl->kv32, pl->b2, a2,
                     pl->disabled, pl->lo32, pl->gmin_hi);
+        if (pl->prof) {
+          HC(hipEventRecord(pl->prof_ev[pl->prof_used + 1], s));
+          pl->prof_used += 2;
+          pl->prof_flop += 2.0 * (double)P * (double)m.V;
+        }
         hipLaunchKernelGGL((k_band_superset<T>), dim3(nblk(P, 256)), dim3(256), 0, s, ti, g, (const float*)pl->lo32,
                            (const uint32_t*)pl->gmin_hi, pl->tol, pl->dtoff + m.off, (const double*)pl->kv, m.V,
                            (const double*)pl->b2, a2, pl->dexact, pl->bitmap, pl->gmin64);
@@ -747,6 +768,10 @@ template <typename T>
 void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   hipStream_t s = pl->ctx->stream;
   const IqGeo& g = pl->geo;
+  pl->prof_used = 0;
+  pl->prof_flop = 0.0;
+  pl->prof_sec = 0.0;
+  pl->prof_launches = 0;
   HC(hipEventRecord(pl->e0, s));
   hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(64), 0, s, pl->err, (int64_t)1, 0);
   hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, pl->rounds, (int64_t)1, (int64_t)0);
@@ -767,6 +792,12 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   HC(hipEventRecord(pl->e1, s));
   HC(hipStreamSynchronize(s));
   HC(hipEventElapsedTime(&pl->last_ms, pl->e0, pl->e1));
+  for (int64_t k = 0; k + 1 < pl->prof_used; k += 2) {
+    float ms = 0.0f;
+    HC(hipEventElapsedTime(&ms, pl->prof_ev[k], pl->prof_ev[k + 1]));
+    pl->prof_sec += ms * 1e-3;
+    pl->prof_launches++;
+  }
   int32_t code = 0;
   HC(hipMemcpy(&code, pl->err, sizeof(int32_t), hipMemcpyDeviceToHost));
   check_kernel_error(code);
@@ -1092,6 +1123,7 @@ int iq_plan_create(iq_ctx* ctx, const iq_plan_desc* desc, iq_plan** out) {
 int iq_plan_destroy(iq_plan* plan) {
   if (!plan) return IQ_OK;
   if (plan->ctx) (void)hipSetDevice(plan->ctx->device);
+  for (hipEvent_t e : plan->prof_ev) (void)hipEventDestroy(e);
   if (plan->e0) (void)hipEventDestroy(plan->e0);
   if (plan->e1) (void)hipEventDestroy(plan->e1);
   plan->mem.release();
@@ -1133,6 +1165,20 @@ int iq_plan_results(iq_ctx* ctx, iq_plan* plan, void* out, int64_t* chosen, doub
 int iq_plan_last_time(iq_plan* plan, double* seconds) {
   if (!plan || !seconds) return fail(nullptr, IQ_ERR_ARG, "null argument");
   *seconds = plan->last_ms * 1e-3;
+  return IQ_OK;
+}
+
+int iq_plan_profile(iq_plan* plan, int enable) {
+  if (!plan) return fail(nullptr, IQ_ERR_ARG, "null plan");
+  plan->prof = enable != 0;
+  return IQ_OK;
+}
+
+int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double* flop) {
+  if (!plan) return fail(nullptr, IQ_ERR_ARG, "null plan");
+  if (launches) *launches = plan->prof_launches;
+  if (seconds) *seconds = plan->prof_sec;
+  if (flop) *flop = plan->prof_flop;
   return IQ_OK;
 }
 

@@ -24,8 +24,10 @@ def _f(a, dtype) -> np.ndarray:
     return np.asfortranarray(np.asarray(a, dtype=dtype))
 
 
-def _d3(shape) -> np.ndarray:
-    return np.asarray(list(shape) + [1] * (3 - len(shape)), dtype=np.int64)
+def _d3(shape, fill: int = 1) -> np.ndarray:
+    """Extents padded to three axes (missing trailing axes have extent `fill`).
+    Keep the returned array referenced for as long as C reads its address."""
+    return np.asarray(list(shape) + [fill] * (3 - len(shape)), dtype=np.int64)
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -85,8 +87,8 @@ def imfilter(img, krn, ctx: Optional[Context] = None) -> np.ndarray:
     if any(s <= 0 for s in oshape):
         raise ValueError("kernel larger than image")
     out = np.zeros(oshape, dtype=np.float64, order="F")
-    ctx.check(ctx._L.iq_imfilter(ctx.handle, _ptr(img), _ptr(_d3(img.shape)), _ptr(krn), _ptr(_d3(krn.shape)),
-                                 _ptr(out)))
+    idims, kdims = _d3(img.shape), _d3(krn.shape)
+    ctx.check(ctx._L.iq_imfilter(ctx.handle, _ptr(img), _ptr(idims), _ptr(krn), _ptr(kdims), _ptr(out)))
     return out
 
 
@@ -100,8 +102,9 @@ def fastdistance(img, kern, weights=None, ctx: Optional[Context] = None) -> np.n
     w = None if weights is None else _f(np.asarray(weights) != 0, np.uint8)
     oshape = tuple(a - b + 1 for a, b in zip(imgf.shape, kern.shape))
     out = np.zeros(oshape, dtype=np.float64, order="F")
-    ctx.check(ctx._L.iq_fastdistance(ctx.handle, _ptr(imgf), dt, _ptr(_d3(imgf.shape)), _ptr(kern), _ptr(w),
-                                     _ptr(_d3(kern.shape)), _ptr(out)))
+    idims, kdims = _d3(imgf.shape), _d3(kern.shape)
+    ctx.check(ctx._L.iq_fastdistance(ctx.handle, _ptr(imgf), dt, _ptr(idims), _ptr(kern), _ptr(w), _ptr(kdims),
+                                     _ptr(out)))
     return out
 
 
@@ -164,7 +167,8 @@ def graphcut(A, B, dim: int, ctx: Optional[Context] = None) -> np.ndarray:
     npdt = np.float32 if dt == _lib.IQ_F32 else np.float64
     Af, Bf = _f(A, npdt), _f(B, npdt)
     m = np.zeros(A.shape, dtype=np.uint8, order="F")
-    ctx.check(ctx._L.iq_graphcut(ctx.handle, _ptr(Af), _ptr(Bf), dt, _ptr(_d3(A.shape)), int(dim), _ptr(m)))
+    dims = _d3(A.shape)
+    ctx.check(ctx._L.iq_graphcut(ctx.handle, _ptr(Af), _ptr(Bf), dt, _ptr(dims), int(dim), _ptr(m)))
     return m.astype(bool)
 
 
@@ -191,6 +195,16 @@ class Plan:
         t = ctypes.c_double(0.0)
         self.ctx.check(self.ctx._L.iq_plan_last_time(self.handle, ctypes.byref(t)))
         return t.value
+
+    def profile(self, enable: bool = True) -> None:
+        self.ctx.check(self.ctx._L.iq_plan_profile(self.handle, int(bool(enable))))
+
+    def scan_stats(self):
+        """(launches, seconds, flop) of the FP32 scan during the last run."""
+        n = np.zeros(1, dtype=np.int64)
+        t = np.zeros(2, dtype=np.float64)
+        self.ctx.check(self.ctx._L.iq_plan_scan_stats(self.handle, _ptr(n), t[0:].ctypes.data, t[1:].ctypes.data))
+        return int(n[0]), float(t[0]), float(t[1])
 
     def results(self, grids=True, chosen=True, cuts=False):
         padvox = int(np.prod(self.padsize))
@@ -291,7 +305,8 @@ def make_plan(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=N
     d.ti_dtype = _lib.IQ_F32 if dtype == np.float32 else _lib.IQ_F64
     d.tisize[:] = list(_d3(tis))
     for name in ("tilesize", "ovlsize", "spacing", "ntiles", "padsize"):
-        getattr(d, name)[:] = list(_d3(geo[name]))
+        # a missing trailing axis is one voxel thick with no overlap
+        getattr(d, name)[:] = list(_d3(geo[name], 0 if name == "ovlsize" else 1))
     d.disabled = disabled.ctypes.data
     d.path = pth.ctypes.data
     d.npath = len(simpath)

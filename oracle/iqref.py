@@ -285,8 +285,6 @@ def genpath(rng: np.random.Generator, extent, kind: str, datainds: List[int]) ->
             return [int(i) for i in rng.permutation(n)]
         if kind == "dilation":
             pivot = int(rng.integers(n))
-            grid = np.zeros(extent, dtype=bool)
-            grid.ravel(order="F")  # layout check only
             flat = np.zeros(n, dtype=bool)
             flat[pivot] = True
             path = [pivot]
@@ -325,7 +323,7 @@ def pad_symmetric(aux, padsize):
 # ---------------------------------------------------------------------------
 def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None, tol=0.1,
           path="raster", nreal=1, debug=False, uniforms=None, forced=None, seed=0,
-          gr_freq=GR_FREQ, return_trace=False):
+          gr_freq=GR_FREQ, return_trace=False, max_steps=None):
     T = np.asarray(trainimg)
     N = T.ndim
     if simsize is None:
@@ -343,6 +341,8 @@ def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None,
     skipped, datainds = findskipped(hard, geo)
     rng = np.random.default_rng(seed)
     simpath = genpath(rng, geo["ntiles"], path, list(datainds))
+    if max_steps is not None:  # bounded sample: the first tile steps only
+        simpath = simpath[:max_steps]
     npath = len(simpath)
     if uniforms is None:
         uniforms = rng.random((nreal, npath))
