@@ -186,43 +186,6 @@ size_t scan_lds_bytes(const ScanCfg& c, const std::vector<IqBox>& boxes) {
   return m;
 }
 
-template <int R, int LX, int LY, int LZ>
-void launch_scan_cfg(dim3 grid, size_t lds, hipStream_t s, const float* ti, IqGeo g, const IqMask* mask,
-                     const float* kv32, const double* b2, const double* a2, const uint8_t* dis, float* lo,
-                     uint32_t* gmin) {
-  hipLaunchKernelGGL((k_scan_f32<R, LX, LY, LZ>), grid, dim3(LX * LY * LZ), lds, s, ti, g, mask, kv32, b2, a2,
-                     dis, lo, gmin);
-}
-
-void launch_scan(int cfg, const IqGeo& g, size_t lds, hipStream_t s, const float* ti, const IqMask* mask,
-                 const float* kv32, const double* b2, const double* a2, const uint8_t* dis, float* lo,
-                 uint32_t* gmin) {
-  const ScanCfg& c = kScanCfg[cfg];
-  dim3 grid(nblk(g.dist[0], c.R * c.LX), nblk(g.dist[1], c.LY), nblk(g.dist[2], c.LZ));
-  switch (cfg) {
-    case 0: launch_scan_cfg<4, 8, 8, 4>(grid, lds, s, ti, g, mask, kv32, b2, a2, dis, lo, gmin); break;
-    case 1: launch_scan_cfg<4, 16, 4, 4>(grid, lds, s, ti, g, mask, kv32, b2, a2, dis, lo, gmin); break;
-    case 2: launch_scan_cfg<2, 16, 8, 2>(grid, lds, s, ti, g, mask, kv32, b2, a2, dis, lo, gmin); break;
-    default: launch_scan_cfg<4, 8, 16, 2>(grid, lds, s, ti, g, mask, kv32, b2, a2, dis, lo, gmin); break;
-  }
-  HCL();
-}
-
-int pick_scan_cfg(const IqGeo& g, const std::vector<IqBox>& boxes, size_t* lds) {
-  int best = -1;
-  double bestcost = 1e300;
-  for (int i = 0; i < kNumScanCfg; i++) {
-    const ScanCfg& c = kScanCfg[i];
-    size_t l = scan_lds_bytes(c, boxes);
-    if (l > 64 * 1024) continue;
-    double padded = (double)nblk(g.dist[0], c.R * c.LX) * c.R * c.LX * nblk(g.dist[1], c.LY) * c.LY *
-                    nblk(g.dist[2], c.LZ) * c.LZ;
-    double cost = padded * (1.0 + 0.02 * (double)l / 16384.0);
-    if (cost < bestcost) { bestcost = cost; best = i; *lds = l; }
-  }
-  return best;
-}
-
 bool fits_f32(const double* v, int64_t n) {
   for (int64_t i = 0; i < n; i++) {
     double x = v[i];
@@ -263,81 +226,73 @@ struct iq_plan {
   int naux = 0;
   bool fast = false;
   bool hard = false;
+  bool any_multi = false;
   std::vector<IqStep> steps;
   std::vector<IqMask> masks;
-  std::vector<int> mask_cfg;
-  std::vector<size_t> mask_lds;
+  std::vector<std::vector<int64_t>> waves;  // path positions per wave (dependency order)
+  int scan_cfg = -1;
+  size_t scan_lds = 0;
+  int nbz = 1;
+  int64_t nslots = 0;
   DMem mem;
-  void* ti = nullptr;
-  float* ti32 = nullptr;
-  uint8_t* disabled = nullptr;
-  int32_t* enabled = nullptr;
-  IqStep* dsteps = nullptr;
-  IqMask* dmasks = nullptr;
-  int32_t *dtvox = nullptr, *dtoff = nullptr, *dhtvox = nullptr, *dhtoff = nullptr;
-  int32_t full_off = 0;
-  std::vector<double*> a2maps;
-  double* hardgrid = nullptr;
-  std::vector<double*> aux, auxti, a2soft;
-  // per-query scratch
-  double* kv = nullptr;
-  float* kv32 = nullptr;
-  double* b2 = nullptr;
-  uint32_t* gmin_hi = nullptr;
-  unsigned long long* gmin64 = nullptr;
-  float* lo32 = nullptr;
-  double* dexact = nullptr;
-  unsigned long long* bitmap = nullptr;
-  int32_t* band = nullptr;
-  double* bandd = nullptr;
-  int32_t* rank = nullptr;
-  double* p = nullptr;
-  double* s1 = nullptr;
-  int64_t* pick = nullptr;
-  int32_t* err = nullptr;
-  int64_t* nband = nullptr;
-  int64_t* rounds = nullptr;
-  uint8_t* keep = nullptr;
-  char* cutscratch = nullptr;
-  int64_t cutscratch_per = 0;
-  // multi-source selection
-  std::vector<double*> maps;      // 0 overlap, 1 hard, 2+s soft
-  std::vector<int32_t*> sorted;   // stable ascending orders of maps
-  double* keys_out = nullptr;
-  int32_t* iota = nullptr;
-  void* sort_tmp = nullptr;
-  size_t sort_tmp_bytes = 0;
-  const double** dmaps_hard = nullptr;
-  const double** dmaps_soft = nullptr;
-  const int32_t** dsorted_hard = nullptr;
-  const int32_t** dsorted_soft = nullptr;
-  uint32_t* bits = nullptr;
-  int32_t* ev = nullptr;
-  double* vals = nullptr;
-  double* kvh = nullptr;
-  double* b2h = nullptr;
-  double* kvs = nullptr;
-  double* b2s = nullptr;
-  // results
-  void* sims = nullptr;
-  uint8_t* cuts = nullptr;
-  int64_t* log = nullptr;
+  IqDev hd{};            // host copy of the device descriptor
+  IqDev* dd = nullptr;   // device descriptor
+  IqQuery* dq = nullptr; // query table (all batches of a run)
+  int64_t dq_cap = 0;
+  size_t cut_lds_max = 0;
   float last_ms = 0.0f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  size_t cut_lds_max = 0;
   // scan profiling
   bool prof = false;
   std::vector<hipEvent_t> prof_ev;
   int64_t prof_used = 0;
   double prof_flop = 0.0, prof_sec = 0.0;
   int64_t prof_launches = 0;
+  std::vector<double> batch_flop;
+  // stable sorts of the multi-source maps
+  void* sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  double* sort_keys = nullptr;
+  int32_t* sort_iota = nullptr;
 };
 
 namespace {
 
-template <typename T>
-T* sim_of(iq_plan* pl, int64_t r) {
-  return (T*)pl->sims + r * pl->geo.padvox;
+// Waves: a path position depends on every earlier path position whose tile is
+// within one tile along each axis (only those overlap when the overlap is at
+// most half a tile); it runs one wave after the latest of them.  Tiles of one
+// wave never overlap, every neighbour pasted before a tile in path order is
+// pasted in an earlier wave and none after it is, so masks, windows and cuts
+// are exactly those of the sequential loop.
+std::vector<std::vector<int64_t>> build_waves(const iq_plan* pl, const int64_t* path, bool sequential) {
+  std::vector<std::vector<int64_t>> waves;
+  const int64_t nt = pl->ntiles[0] * pl->ntiles[1] * pl->ntiles[2];
+  std::vector<int64_t> wave_of((size_t)nt, -1);
+  int64_t last = -1;
+  for (int64_t i = 0; i < pl->npath; i++) {
+    if (!pl->steps[i].active) continue;
+    int64_t w;
+    if (sequential) {
+      w = ++last;
+    } else {
+      const int64_t ind = path[i];
+      int64_t c[3] = {ind % pl->ntiles[0], (ind / pl->ntiles[0]) % pl->ntiles[1], ind / (pl->ntiles[0] * pl->ntiles[1])};
+      w = 0;
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++) {
+            int64_t nb[3] = {c[0] + dx, c[1] + dy, c[2] + dz};
+            if (nb[0] < 0 || nb[1] < 0 || nb[2] < 0 || nb[0] >= pl->ntiles[0] || nb[1] >= pl->ntiles[1] || nb[2] >= pl->ntiles[2])
+              continue;
+            int64_t wi = wave_of[nb[0] + pl->ntiles[0] * (nb[1] + pl->ntiles[1] * nb[2])];
+            if (wi >= 0) w = std::max(w, wi + 1);
+          }
+      wave_of[ind] = w;
+    }
+    if ((int64_t)waves.size() <= w) waves.resize(w + 1);
+    waves[w].push_back(i);
+  }
+  return waves;
 }
 
 void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
@@ -345,7 +300,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
   if (d->ti_dtype != IQ_F32 && d->ti_dtype != IQ_F64) bad(IQ_ERR_ARG, "ti_dtype must be IQ_F32 or IQ_F64");
   if (d->nreal <= 0) bad(IQ_ERR_ARG, "invalid number of realizations");
   if (!(d->tol > 0.0 && d->tol <= 1.0)) bad(IQ_ERR_ARG, "tolerance must be in range (0,1]");
-  if (d->naux < 0 || d->naux > 8) bad(IQ_ERR_ARG, "naux must be in 0..8");
+  if (d->naux < 0 || d->naux > IQ_MAX_AUX) bad(IQ_ERR_ARG, "naux must be in 0..8");
   hipStream_t s = ctx->stream;
   pl->ctx = ctx;
   pl->dtype = d->ti_dtype;
@@ -353,6 +308,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
   pl->nreal = d->nreal;
   pl->tol = d->tol;
   pl->naux = d->naux;
+  bool half = true;
   for (int k = 0; k < 3; k++) {
     if (d->tilesize[k] <= 0 || d->tilesize[k] > d->tisize[k]) bad(IQ_ERR_ARG, "invalid tile size");
     if (d->ovlsize[k] < 0 || d->ovlsize[k] > d->tilesize[k]) bad(IQ_ERR_ARG, "invalid overlap size");
@@ -362,6 +318,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
     pl->ovl[k] = d->ovlsize[k];
     pl->spacing[k] = d->spacing[k];
     pl->ntiles[k] = d->ntiles[k];
+    if (2 * d->ovlsize[k] > d->tilesize[k] || d->spacing[k] < 1) half = false;
   }
   geo_init(pl->geo, d->tisize, d->tilesize, d->padsize);
   IqGeo& g = pl->geo;
@@ -378,11 +335,13 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
   std::map<std::vector<uint8_t>, int> mask_index;
   std::vector<HostMask> hmasks;
   std::vector<int32_t> htvox;
+  std::vector<int64_t> mask_uses;
   pl->hard = d->hardgrid != nullptr;
   for (int64_t i = 0; i < d->npath; i++) {
     IqStep st{};
     int64_t ind = d->path[i];
-    if (d->skipped[ind]) { st.active = 0; st.mask = -1; pl->steps.push_back(st); continue; }
+    if (pasted[ind]) bad(IQ_ERR_ARG, "path visits a tile twice");
+    if (d->skipped[ind]) { st.active = 0; st.mask = -1; pasted[ind] = 0; pl->steps.push_back(st); continue; }
     st.active = 1;
     int64_t ti3[3] = {ind % pl->ntiles[0], (ind / pl->ntiles[0]) % pl->ntiles[1], ind / (pl->ntiles[0] * pl->ntiles[1])};
     for (int k = 0; k < 3; k++) st.origin[k] = (int32_t)(ti3[k] * pl->spacing[k]);
@@ -430,9 +389,11 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
         mask_index[bits] = (int)hmasks.size();
         st.mask = (int)hmasks.size();
         hmasks.push_back(hm);
+        mask_uses.push_back(0);
       } else {
         st.mask = it->second;
       }
+      mask_uses[st.mask]++;
     }
     // hard data in the tile (iqsim.jl:210-219, indicator!/event! utils.jl:18-36)
     if (d->hardgrid) {
@@ -446,9 +407,11 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
       st.hcnt = (int32_t)htvox.size() - st.hoff;
       st.hard = st.hcnt > 0;
     }
+    if (st.hard || pl->naux > 0) pl->any_multi = true;
     pasted[ind] = 1;
     pl->steps.push_back(st);
   }
+  pl->waves = build_waves(pl, d->path, !half);
 
   // ---- offset tables: per mask, tile-linear and TI-linear offsets (ascending column-major)
   std::vector<int32_t> tvox, toff;
@@ -476,64 +439,114 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
     m.a2 = -1;
     pl->masks.push_back(m);
   }
-  pl->full_off = (int32_t)tvox.size();
+  const int32_t full_off = (int32_t)tvox.size();
   for (int32_t t = 0; t < g.tilevox; t++) { tvox.push_back(t); toff.push_back(ti_off(t)); }
   std::vector<int32_t> htoff(htvox.size());
   for (size_t i = 0; i < htvox.size(); i++) htoff[i] = ti_off(htvox[i]);
 
   // ---- training image
+  IqDev& H = pl->hd;
+  H.g = g;
   const double* tid = (const double*)d->ti;
   const float* tif = (const float*)d->ti;
   if (d->ti_dtype == IQ_F32) {
-    pl->ti = pl->mem.upload(tif, tivox, s);
+    H.ti = pl->mem.upload(tif, tivox, s);
     pl->fast = fits_range(tif, tivox);
-    if (pl->fast) pl->ti32 = (float*)pl->ti;
+    if (pl->fast) H.ti32 = (const float*)H.ti;
   } else {
-    pl->ti = pl->mem.upload(tid, tivox, s);
+    H.ti = pl->mem.upload(tid, tivox, s);
     pl->fast = fits_f32(tid, tivox);
     if (pl->fast) {
       std::vector<float> t32((size_t)tivox);
       for (int64_t i = 0; i < tivox; i++) t32[i] = (float)tid[i];
-      pl->ti32 = pl->mem.upload(t32.data(), tivox, s);
+      H.ti32 = pl->mem.upload(t32.data(), tivox, s);
       HC(hipStreamSynchronize(s));
     }
   }
-  pl->disabled = pl->mem.upload(d->disabled, (size_t)g.P, s);
+  H.ti_f32 = d->ti_dtype == IQ_F32;
+  H.naux = pl->naux;
+  H.tol = pl->tol;
+  H.disabled = pl->mem.upload(d->disabled, (size_t)g.P, s);
   std::vector<int32_t> en;
   en.reserve((size_t)g.P);
   for (int64_t c = 0; c < g.P; c++)
     if (!d->disabled[c]) en.push_back((int32_t)c);
   pl->nen = (int64_t)en.size();
-  pl->enabled = pl->mem.upload(en.data(), en.size(), s);
-  pl->dtvox = pl->mem.upload(tvox.data(), tvox.size(), s);
-  pl->dtoff = pl->mem.upload(toff.data(), toff.size(), s);
-  pl->dhtvox = pl->mem.upload(htvox.data(), htvox.size(), s);
-  pl->dhtoff = pl->mem.upload(htoff.data(), htoff.size(), s);
-  pl->dsteps = pl->mem.upload(pl->steps.data(), pl->steps.size(), s);
+  H.enabled = pl->mem.upload(en.data(), en.size(), s);
+  H.nen = pl->nen;
+  H.steps = pl->mem.upload(pl->steps.data(), pl->steps.size(), s);
+  H.tvox = pl->mem.upload(tvox.data(), tvox.size(), s);
+  H.toff = pl->mem.upload(toff.data(), toff.size(), s);
+  H.htvox = pl->mem.upload(htvox.data(), htvox.size(), s);
+  H.htoff = pl->mem.upload(htoff.data(), htoff.size(), s);
+  H.full_off = full_off;
 
-  // ---- per-query scratch
+  // ---- exact A2 maps of the overlap masks (A2 chains over the TI)
+  std::vector<const double*> a2ptr;
+  for (size_t mi = 0; mi < pl->masks.size(); mi++) {
+    IqMask& m = pl->masks[mi];
+    double* a2 = pl->mem.alloc<double>(g.P);
+    if (pl->dtype == IQ_F32)
+      hipLaunchKernelGGL((k_exact_map<float, 2>), dim3(nblk(g.P, 256)), dim3(256), 0, s, (const float*)H.ti, g,
+                         H.toff + m.off, (const double*)nullptr, m.V, (const double*)nullptr, a2);
+    else
+      hipLaunchKernelGGL((k_exact_map<double, 2>), dim3(nblk(g.P, 256)), dim3(256), 0, s, (const double*)H.ti, g,
+                         H.toff + m.off, (const double*)nullptr, m.V, (const double*)nullptr, a2);
+    HCL();
+    m.a2 = (int32_t)a2ptr.size();
+    a2ptr.push_back(a2);
+  }
+  H.masks = pl->mem.upload(pl->masks.data(), pl->masks.size(), s);
+  H.a2maps = (const double* const*)pl->mem.upload(a2ptr.data(), a2ptr.size(), s);
+  // one scan configuration for the plan: every mask must fit its LDS, the
+  // most used mask decides the tiling
+  if (pl->fast && !hmasks.empty()) {
+    size_t best_mi = std::max_element(mask_uses.begin(), mask_uses.end()) - mask_uses.begin();
+    double bestcost = 1e300;
+    for (int c = 0; c < kNumScanCfg; c++) {
+      size_t lmax = 0;
+      for (const HostMask& hm : hmasks) lmax = std::max(lmax, scan_lds_bytes(kScanCfg[c], hm.boxes));
+      if (lmax > 64 * 1024) continue;
+      size_t l = 0;
+      const ScanCfg& sc = kScanCfg[c];
+      double padded = (double)nblk(g.dist[0], sc.R * sc.LX) * sc.R * sc.LX * nblk(g.dist[1], sc.LY) * sc.LY *
+                      nblk(g.dist[2], sc.LZ) * sc.LZ;
+      l = scan_lds_bytes(sc, hmasks[best_mi].boxes);
+      double cost = padded * (1.0 + 0.02 * (double)l / 16384.0);
+      if (cost < bestcost) { bestcost = cost; pl->scan_cfg = c; pl->scan_lds = lmax; }
+    }
+    if (pl->scan_cfg < 0) pl->fast = false;
+    else pl->nbz = (int)nblk(g.dist[2], kScanCfg[pl->scan_cfg].LZ);
+  }
+  if (!pl->fast) H.ti32 = nullptr;
+
+  // ---- hard and soft data
+  if (pl->hard) H.hardgrid = pl->mem.upload(d->hardgrid, (size_t)g.padvox, s);
+  std::vector<const double*> auxp, auxtip, a2sp;
+  if (pl->naux > 0) {
+    if (!d->aux || !d->auxti) bad(IQ_ERR_ARG, "null soft data");
+    for (int a = 0; a < pl->naux; a++) {
+      if (!d->aux[a] || !d->auxti[a]) bad(IQ_ERR_ARG, "null soft data");
+      auxp.push_back(pl->mem.upload(d->aux[a], (size_t)g.padvox, s));
+      double* at = pl->mem.upload(d->auxti[a], (size_t)tivox, s);
+      auxtip.push_back(at);
+      double* a2 = pl->mem.alloc<double>(g.P);
+      hipLaunchKernelGGL((k_exact_map<double, 2>), dim3(nblk(g.P, 256)), dim3(256), 0, s, (const double*)at, g,
+                         H.toff + full_off, (const double*)nullptr, g.tilevox, (const double*)nullptr, a2);
+      HCL();
+      a2sp.push_back(a2);
+    }
+    H.aux = (const double* const*)pl->mem.upload(auxp.data(), auxp.size(), s);
+    H.auxti = (const double* const*)pl->mem.upload(auxtip.data(), auxtip.size(), s);
+    H.a2soft = (const double* const*)pl->mem.upload(a2sp.data(), a2sp.size(), s);
+  }
+
+  // ---- per-slot scratch: as many slots as the widest batch, within a memory budget
   const int64_t P = g.P;
-  const int64_t nw64 = (P + 63) / 64;
+  const int64_t nw64 = (P + 63) / 64, nw32 = (P + 31) / 32;
+  const int nmap = 2 + pl->naux;
   int kmax = 4;
   for (const IqMask& m : pl->masks) kmax = std::max(kmax, m.kbox);
-  pl->kv = pl->mem.alloc<double>(g.tilevox);
-  pl->kv32 = pl->mem.alloc<float>(kmax);
-  pl->b2 = pl->mem.alloc<double>(1);
-  pl->gmin_hi = pl->mem.alloc<uint32_t>(1);
-  pl->gmin64 = pl->mem.alloc<unsigned long long>(1);
-  pl->lo32 = pl->mem.alloc<float>(P);
-  pl->dexact = pl->mem.alloc<double>(P);
-  pl->bitmap = pl->mem.alloc<unsigned long long>(nw64);
-  pl->band = pl->mem.alloc<int32_t>(P);
-  pl->bandd = pl->mem.alloc<double>(P);
-  pl->rank = pl->mem.alloc<int32_t>(P);
-  pl->p = pl->mem.alloc<double>(P);
-  pl->s1 = pl->mem.alloc<double>(nw64 + 1);
-  pl->pick = pl->mem.alloc<int64_t>(1);
-  pl->err = pl->mem.alloc<int32_t>(1);
-  pl->nband = pl->mem.alloc<int64_t>(1);
-  pl->rounds = pl->mem.alloc<int64_t>(1);
-  pl->keep = pl->mem.alloc<uint8_t>((size_t)IQ_MAX_SLAB * g.tilevox);
   int64_t maxn = 0;
   for (const IqStep& st : pl->steps)
     for (int q = 0; q < st.nslab; q++) {
@@ -542,95 +555,69 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
       if (n <= IQ_PR_NLDS) pl->cut_lds_max = std::max(pl->cut_lds_max, (size_t)n * CUT_NODE_BYTES);
       maxn = std::max(maxn, n);
     }
-  if (maxn > IQ_PR_NLDS) {
-    pl->cutscratch_per = maxn * CUT_NODE_BYTES;
-    pl->cutscratch = pl->mem.alloc<char>((size_t)pl->cutscratch_per * IQ_MAX_SLAB);
+  const int64_t cutper = maxn > IQ_PR_NLDS ? maxn * CUT_NODE_BYTES : 0;
+  size_t per_slot = (size_t)nmap * g.tilevox * 8 + (size_t)kmax * 4 + 64 + (size_t)P * (4 + 8 + 4 + 4) + nw64 * 8 +
+                    (size_t)IQ_SORT_MAX * 12 + (size_t)(nw64 + 1) * 8 + (size_t)IQ_MAX_SLAB * g.tilevox +
+                    (size_t)IQ_MAX_SLAB * cutper;
+  if (pl->any_multi) per_slot += (size_t)nmap * P * 12 + (size_t)nmap * nw32 * 4 + (size_t)P * 12;
+  int64_t widest = 1;
+  for (auto& w : pl->waves) widest = std::max<int64_t>(widest, (int64_t)w.size() * pl->nreal);
+  const double budget = 24.0 * (1ull << 30);
+  int64_t cap = std::max<int64_t>(1, (int64_t)(budget / (double)per_slot));
+  cap = std::min<int64_t>(cap, 65535 / std::max(1, pl->nbz));
+  cap = std::min<int64_t>(cap, 4096);
+  pl->nslots = std::min(widest, cap);
+  const int64_t NS = pl->nslots;
+  H.kv = pl->mem.alloc<double>((size_t)NS * nmap * g.tilevox);
+  H.kv_stride = (int64_t)nmap * g.tilevox;
+  H.kv32 = pl->mem.alloc<float>((size_t)NS * kmax);
+  H.kv32_stride = kmax;
+  H.b2 = pl->mem.alloc<double>((size_t)NS * nmap);
+  H.gmin_hi = pl->mem.alloc<uint32_t>(NS);
+  H.gmin64 = pl->mem.alloc<unsigned long long>(NS);
+  H.lo32 = pl->mem.alloc<float>((size_t)NS * P);
+  H.dexact = pl->mem.alloc<double>((size_t)NS * P);
+  H.bitmap = pl->mem.alloc<unsigned long long>((size_t)NS * nw64);
+  H.nw64 = nw64;
+  H.band = pl->mem.alloc<int32_t>((size_t)NS * P);
+  H.sup = pl->mem.alloc<int32_t>((size_t)NS * P);
+  H.nsup = pl->mem.alloc<int32_t>(NS);
+  H.sd = pl->mem.alloc<double>((size_t)NS * IQ_SORT_MAX);
+  H.si = pl->mem.alloc<int32_t>((size_t)NS * IQ_SORT_MAX);
+  H.s1 = pl->mem.alloc<double>((size_t)NS * (nw64 + 1));
+  H.s1_stride = nw64 + 1;
+  H.pick = pl->mem.alloc<int64_t>(NS);
+  H.nband = pl->mem.alloc<int64_t>(NS);
+  H.keep = pl->mem.alloc<uint8_t>((size_t)NS * IQ_MAX_SLAB * g.tilevox);
+  if (cutper) {
+    H.cutscratch = pl->mem.alloc<char>((size_t)NS * IQ_MAX_SLAB * cutper);
+    H.cutscratch_per = cutper;
   }
-
-  // ---- exact A2 maps of the overlap masks (MODE 2 chains)
-  IqMask* dm = pl->mem.upload(pl->masks.data(), pl->masks.size(), s);
-  pl->dmasks = dm;
-  for (size_t mi = 0; mi < pl->masks.size(); mi++) {
-    IqMask& m = pl->masks[mi];
-    double* a2 = pl->mem.alloc<double>(P);
-    if (pl->dtype == IQ_F32)
-      hipLaunchKernelGGL((k_exact_map<float, 2>), dim3(nblk(P, 256)), dim3(256), 0, s, (const float*)pl->ti, g,
-                         pl->dtoff + m.off, (const double*)nullptr, m.V, (const double*)nullptr,
-                         (const double*)nullptr, (const uint8_t*)nullptr, a2);
-    else
-      hipLaunchKernelGGL((k_exact_map<double, 2>), dim3(nblk(P, 256)), dim3(256), 0, s, (const double*)pl->ti, g,
-                         pl->dtoff + m.off, (const double*)nullptr, m.V, (const double*)nullptr,
-                         (const double*)nullptr, (const uint8_t*)nullptr, a2);
-    HCL();
-    m.a2 = (int32_t)pl->a2maps.size();
-    pl->a2maps.push_back(a2);
-    size_t lds = 0;
-    int cfg = pl->fast ? pick_scan_cfg(g, hmasks[mi].boxes, &lds) : -1;
-    if (cfg < 0) pl->fast = false;
-    pl->mask_cfg.push_back(cfg);
-    pl->mask_lds.push_back(lds);
-  }
-  HC(hipMemcpyAsync(dm, pl->masks.data(), pl->masks.size() * sizeof(IqMask), hipMemcpyHostToDevice, s));
-
-  // ---- hard and soft data, multi-source selection buffers
-  const bool multi = pl->hard || pl->naux > 0;
-  if (pl->hard) {
-    pl->hardgrid = pl->mem.upload(d->hardgrid, (size_t)g.padvox, s);
-    pl->kvh = pl->mem.alloc<double>(g.tilevox);
-    pl->b2h = pl->mem.alloc<double>(1);
-  }
-  if (pl->naux > 0) {
-    if (!d->aux || !d->auxti) bad(IQ_ERR_ARG, "null soft data");
-    pl->kvs = pl->mem.alloc<double>((size_t)g.tilevox * pl->naux);
-    pl->b2s = pl->mem.alloc<double>(pl->naux);
-    for (int a = 0; a < pl->naux; a++) {
-      if (!d->aux[a] || !d->auxti[a]) bad(IQ_ERR_ARG, "null soft data");
-      pl->aux.push_back(pl->mem.upload(d->aux[a], (size_t)g.padvox, s));
-      double* at = pl->mem.upload(d->auxti[a], (size_t)tivox, s);
-      pl->auxti.push_back(at);
-      double* a2 = pl->mem.alloc<double>(P);
-      hipLaunchKernelGGL((k_exact_map<double, 2>), dim3(nblk(P, 256)), dim3(256), 0, s, (const double*)at, g,
-                         pl->dtoff + pl->full_off, (const double*)nullptr, g.tilevox, (const double*)nullptr,
-                         (const double*)nullptr, (const uint8_t*)nullptr, a2);
-      HCL();
-      pl->a2soft.push_back(a2);
-    }
-  }
-  if (multi) {
-    int nm = 2 + pl->naux;
-    for (int k = 0; k < nm; k++) {
-      pl->maps.push_back(pl->mem.alloc<double>(P));
-      pl->sorted.push_back(pl->mem.alloc<int32_t>(P));
-    }
-    pl->keys_out = pl->mem.alloc<double>(P);
-    pl->iota = pl->mem.alloc<int32_t>(P);
-    hipLaunchKernelGGL(k_iota32, dim3(nblk(P, 256)), dim3(256), 0, s, pl->iota, P);
+  H.nmap = nmap;
+  H.nw32 = nw32;
+  if (pl->any_multi) {
+    H.maps = pl->mem.alloc<double>((size_t)NS * nmap * P);
+    H.sorted = pl->mem.alloc<int32_t>((size_t)NS * nmap * P);
+    H.bits = pl->mem.alloc<uint32_t>((size_t)NS * nmap * nw32);
+    H.ev = pl->mem.alloc<int32_t>((size_t)NS * P);
+    H.vals = pl->mem.alloc<double>((size_t)NS * P);
+    pl->sort_keys = pl->mem.alloc<double>(P);
+    pl->sort_iota = pl->mem.alloc<int32_t>(P);
+    hipLaunchKernelGGL(k_iota32, dim3(nblk(P, 256)), dim3(256), 0, s, pl->sort_iota, P);
     HCL();
     pl->sort_tmp_bytes = iq_sort_temp_bytes((int)P);
     pl->sort_tmp = pl->mem.alloc<char>(pl->sort_tmp_bytes);
-    std::vector<const double*> mh, ms;
-    std::vector<const int32_t*> sh_, ss;
-    mh.push_back(pl->maps[1]); sh_.push_back(pl->sorted[1]);
-    mh.push_back(pl->maps[0]); sh_.push_back(pl->sorted[0]);
-    ms.push_back(pl->maps[0]); ss.push_back(pl->sorted[0]);
-    for (int a = 0; a < pl->naux; a++) {
-      mh.push_back(pl->maps[2 + a]); sh_.push_back(pl->sorted[2 + a]);
-      ms.push_back(pl->maps[2 + a]); ss.push_back(pl->sorted[2 + a]);
-    }
-    pl->dmaps_hard = (const double**)pl->mem.upload(mh.data(), mh.size(), s);
-    pl->dmaps_soft = (const double**)pl->mem.upload(ms.data(), ms.size(), s);
-    pl->dsorted_hard = (const int32_t**)pl->mem.upload(sh_.data(), sh_.size(), s);
-    pl->dsorted_soft = (const int32_t**)pl->mem.upload(ss.data(), ss.size(), s);
-    pl->bits = pl->mem.alloc<uint32_t>((size_t)((P + 31) / 32) * nm);
-    pl->ev = pl->mem.alloc<int32_t>(P);
-    pl->vals = pl->mem.alloc<double>(P);
   }
+  H.err = pl->mem.alloc<int32_t>(1);
+  H.rounds = pl->mem.alloc<int64_t>(1);
 
   // ---- results
   const size_t esz = pl->dtype == IQ_F32 ? 4 : 8;
-  pl->sims = pl->mem.alloc<char>((size_t)pl->nreal * g.padvox * esz);
-  pl->cuts = pl->mem.alloc<uint8_t>((size_t)pl->nreal * g.padvox);
-  pl->log = pl->mem.alloc<int64_t>((size_t)pl->nreal * std::max<int64_t>(pl->npath, 1));
+  H.sims = pl->mem.alloc<char>((size_t)pl->nreal * g.padvox * esz);
+  H.cuts = pl->mem.alloc<uint8_t>((size_t)pl->nreal * g.padvox);
+  H.log = pl->mem.alloc<int64_t>((size_t)pl->nreal * std::max<int64_t>(pl->npath, 1));
+  H.npath = pl->npath;
+  pl->dd = pl->mem.upload(&pl->hd, 1, s);
   HC(hipEventCreate(&pl->e0));
   HC(hipEventCreate(&pl->e1));
   if (pl->cut_lds_max > 64 * 1024) {
@@ -640,127 +627,85 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
   HC(hipStreamSynchronize(s));
 }
 
+void launch_scan_batch(iq_plan* pl, const IqQuery* q, int nq, hipStream_t s) {
+  const IqGeo& g = pl->geo;
+  const ScanCfg& c = kScanCfg[pl->scan_cfg];
+  dim3 grid(nblk(g.dist[0], c.R * c.LX), nblk(g.dist[1], c.LY), (unsigned)(pl->nbz * nq));
+  const size_t lds = pl->scan_lds;
+  switch (pl->scan_cfg) {
+    case 0: hipLaunchKernelGGL((k_scan_f32<4, 8, 8, 4>), grid, dim3(256), lds, s, (const IqDev*)pl->dd, q, pl->nbz); break;
+    case 1: hipLaunchKernelGGL((k_scan_f32<4, 16, 4, 4>), grid, dim3(256), lds, s, (const IqDev*)pl->dd, q, pl->nbz); break;
+    case 2: hipLaunchKernelGGL((k_scan_f32<2, 16, 8, 2>), grid, dim3(256), lds, s, (const IqDev*)pl->dd, q, pl->nbz); break;
+    default: hipLaunchKernelGGL((k_scan_f32<4, 8, 16, 2>), grid, dim3(256), lds, s, (const IqDev*)pl->dd, q, pl->nbz); break;
+  }
+  HCL();
+}
+
+// One batch of independent queries: distances, selection, cut and paste.
 template <typename T>
-void run_step(iq_plan* pl, int64_t r, int64_t i, double u, int64_t forced) {
+void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool has_plain, bool has_multi) {
   const IqGeo& g = pl->geo;
   hipStream_t s = pl->ctx->stream;
-  const IqStep& st = pl->steps[i];
-  const IqStep* dst = pl->dsteps + i;
-  T* sim = sim_of<T>(pl, r);
-  const T* ti = (const T*)pl->ti;
+  const IqDev* D = pl->dd;
   const int64_t P = g.P;
-  SelOut o{pl->pick, pl->log + r * pl->npath + i, pl->err, pl->nband};
-  const bool multi = st.hard || pl->naux > 0;
-  const int32_t* origin = &dst->origin[0];
-  if (!multi) {
-    if (st.mask < 0) {
-      hipLaunchKernelGGL(k_band_enabled, dim3(1), dim3(FB), 0, s, (const int32_t*)pl->enabled, pl->nen, u, forced,
-                         pl->p, pl->s1, o);
-      HCL();
-    } else {
-      const IqMask& m = pl->masks[st.mask];
-      const IqMask* dm = pl->dmasks + st.mask;
-      const double* a2 = pl->a2maps[m.a2];
-      hipLaunchKernelGGL((k_query_prep<T>), dim3(1), dim3(256), 0, s, (const T*)sim, g, origin, pl->dtvox + m.off,
-                         m.V, pl->fast ? dm : (const IqMask*)nullptr, pl->kv, pl->kv32, pl->b2, pl->gmin_hi,
-                         pl->gmin64);
-      HCL();
-      if (pl->fast) {
-        if (pl->prof) {
-          if (pl->prof_used + 2 > (int64_t)pl->prof_ev.size()) {
-            hipEvent_t a, b;
-            HC(hipEventCreate(&a));
-            HC(hipEventCreate(&b));
-            pl->prof_ev.push_back(a);
-            pl->prof_ev.push_back(b);
-          }
-          HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));
+  const int nsrc_prep = has_multi ? 2 + pl->naux : 1;
+  hipLaunchKernelGGL((k_query_prep<T>), dim3(nq, nsrc_prep), dim3(256), 0, s, D, q);
+  HCL();
+  if (has_plain) {
+    if (pl->fast) {
+      if (pl->prof) {
+        if (pl->prof_used + 2 > (int64_t)pl->prof_ev.size()) {
+          hipEvent_t a, b;
+          HC(hipEventCreate(&a));
+          HC(hipEventCreate(&b));
+          pl->prof_ev.push_back(a);
+          pl->prof_ev.push_back(b);
         }
-        launch_scan(pl->mask_cfg[st.mask], g, pl->mask_lds[st.mask], s, pl->ti32, dm, pl->kv32, pl->b2, a2,
-                    pl->disabled, pl->lo32, pl->gmin_hi);
-        if (pl->prof) {
-          HC(hipEventRecord(pl->prof_ev[pl->prof_used + 1], s));
-          pl->prof_used += 2;
-          pl->prof_flop += 2.0 * (double)P * (double)m.V;
-        }
-        hipLaunchKernelGGL((k_band_superset<T>), dim3(nblk(P, 256)), dim3(256), 0, s, ti, g, (const float*)pl->lo32,
-                           (const uint32_t*)pl->gmin_hi, pl->tol, pl->dtoff + m.off, (const double*)pl->kv, m.V,
-                           (const double*)pl->b2, a2, pl->dexact, pl->bitmap, pl->gmin64);
-        HCL();
-      } else {
-        hipLaunchKernelGGL((k_exact_map<T, 1>), dim3(nblk(P, 256)), dim3(256), 0, s, ti, g, pl->dtoff + m.off,
-                           (const double*)pl->kv, m.V, (const double*)pl->b2, a2, (const uint8_t*)pl->disabled,
-                           pl->dexact);
-        HCL();
-        hipLaunchKernelGGL(k_band_all, dim3(nblk(P, 256)), dim3(256), 0, s, (const double*)pl->dexact, g, pl->bitmap,
-                           pl->gmin64);
-        HCL();
+        HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));
       }
-      hipLaunchKernelGGL(k_band_finalize, dim3(1), dim3(FB), 0, s, P, (const unsigned long long*)pl->bitmap,
-                         (const double*)pl->dexact, (const unsigned long long*)pl->gmin64, pl->tol, u, forced,
-                         pl->band, pl->bandd, pl->rank, pl->p, pl->s1, o);
+      launch_scan_batch(pl, q, nq, s);
+      if (pl->prof) {
+        HC(hipEventRecord(pl->prof_ev[pl->prof_used + 1], s));
+        pl->prof_used += 2;
+        double f = 0.0;
+        for (int i = 0; i < nq; i++) {
+          const IqStep& st = pl->steps[hq[i].step];
+          if (!hq[i].multi && st.mask >= 0) f += 2.0 * (double)P * (double)pl->masks[st.mask].V;
+        }
+        pl->batch_flop.push_back(f);
+        pl->prof_flop += f;
+      }
+      hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q);
       HCL();
-    }
-  } else {
-    // overlap map (iqsim.jl:206-207)
-    if (st.mask >= 0) {
-      const IqMask& m = pl->masks[st.mask];
-      hipLaunchKernelGGL((k_query_prep<T>), dim3(1), dim3(256), 0, s, (const T*)sim, g, origin, pl->dtvox + m.off,
-                         m.V, (const IqMask*)nullptr, pl->kv, (float*)nullptr, pl->b2, (uint32_t*)nullptr,
-                         (unsigned long long*)nullptr);
-      HCL();
-      hipLaunchKernelGGL((k_exact_map<T, 1>), dim3(nblk(P, 256)), dim3(256), 0, s, ti, g, pl->dtoff + m.off,
-                         (const double*)pl->kv, m.V, (const double*)pl->b2, (const double*)pl->a2maps[m.a2],
-                         (const uint8_t*)pl->disabled, pl->maps[0]);
+      hipLaunchKernelGGL(k_band_recheck, dim3(64, nq), dim3(256), 0, s, D, q);
       HCL();
     } else {
-      hipLaunchKernelGGL(k_zero_map, dim3(nblk(P, 256)), dim3(256), 0, s, pl->maps[0], (const uint8_t*)pl->disabled, P);
+      hipLaunchKernelGGL(k_band_exact_all, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q);
       HCL();
     }
-    // hard map (iqsim.jl:210-219)
-    if (st.hard) {
-      hipLaunchKernelGGL((k_query_prep<double>), dim3(1), dim3(256), 0, s, (const double*)pl->hardgrid, g, origin,
-                         pl->dhtvox + st.hoff, st.hcnt, (const IqMask*)nullptr, pl->kvh, (float*)nullptr, pl->b2h,
-                         (uint32_t*)nullptr, (unsigned long long*)nullptr);
-      HCL();
-      hipLaunchKernelGGL((k_exact_map<T, 0>), dim3(nblk(P, 256)), dim3(256), 0, s, ti, g, pl->dhtoff + st.hoff,
-                         (const double*)pl->kvh, st.hcnt, (const double*)pl->b2h, (const double*)nullptr,
-                         (const uint8_t*)pl->disabled, pl->maps[1]);
-      HCL();
-    }
-    // soft maps (iqsim.jl:222-227)
-    for (int a = 0; a < pl->naux; a++) {
-      hipLaunchKernelGGL((k_query_prep<double>), dim3(1), dim3(256), 0, s, (const double*)pl->aux[a], g, origin,
-                         pl->dtvox + pl->full_off, g.tilevox, (const IqMask*)nullptr, pl->kvs + (int64_t)a * g.tilevox,
-                         (float*)nullptr, pl->b2s + a, (uint32_t*)nullptr, (unsigned long long*)nullptr);
-      HCL();
-      hipLaunchKernelGGL((k_exact_map<double, 1>), dim3(nblk(P, 256)), dim3(256), 0, s, (const double*)pl->auxti[a],
-                         g, pl->dtoff + pl->full_off, (const double*)(pl->kvs + (int64_t)a * g.tilevox), g.tilevox,
-                         (const double*)(pl->b2s + a), (const double*)pl->a2soft[a], (const uint8_t*)pl->disabled,
-                         pl->maps[2 + a]);
-      HCL();
-    }
-    // stable orders (partialsortperm, relaxation.jl:12,27)
-    std::vector<int> srcs;
-    if (st.hard) srcs.push_back(1);
-    srcs.push_back(0);
-    for (int a = 0; a < pl->naux; a++) srcs.push_back(2 + a);
-    for (int k : srcs)
-      HC(iq_sort_pairs(pl->sort_tmp, pl->sort_tmp_bytes, pl->maps[k], pl->keys_out, pl->iota, pl->sorted[k], (int)P, s));
-    const int nsrc = (int)srcs.size();
-    hipLaunchKernelGGL(k_relax_finalize, dim3(1), dim3(FB), 0, s, P, nsrc,
-                       st.hard ? pl->dmaps_hard : pl->dmaps_soft, st.hard ? pl->dsorted_hard : pl->dsorted_soft,
-                       pl->tol, u, forced, pl->bits, pl->ev, pl->vals, pl->rank, pl->p, pl->s1, o);
+    hipLaunchKernelGGL(k_band_finalize, dim3(nq), dim3(FB), 0, s, D, q);
     HCL();
   }
-  // boundary cut (iqsim.jl:252-275) and paste (iqsim.jl:278-281)
-  if (st.nslab > 0) {
-    hipLaunchKernelGGL((k_graphcut<T>), dim3(st.nslab), dim3(CB), pl->cut_lds_max, s, (const T*)sim, ti, g, dst,
-                       (const int64_t*)pl->pick, pl->keep, pl->cutscratch, pl->cutscratch_per, pl->err, pl->rounds);
+  if (has_multi) {
+    hipLaunchKernelGGL(k_multi_maps, dim3(nblk(P, 256), nq, 2 + pl->naux), dim3(256), 0, s, D, q);
+    HCL();
+    const IqDev& H = pl->hd;
+    for (int i = 0; i < nq; i++) {
+      if (!hq[i].multi) continue;
+      const IqStep& st = pl->steps[hq[i].step];
+      for (int k = 0; k < H.nmap; k++) {
+        if (k == 1 && !st.hard) continue;
+        const int64_t m = (int64_t)hq[i].slot * H.nmap + k;
+        HC(iq_sort_pairs(pl->sort_tmp, pl->sort_tmp_bytes, H.maps + m * P, pl->sort_keys, pl->sort_iota,
+                         H.sorted + m * P, (int)P, s));
+      }
+    }
+    hipLaunchKernelGGL(k_relax_finalize, dim3(nq), dim3(FB), 0, s, D, q);
     HCL();
   }
-  hipLaunchKernelGGL((k_paste<T>), dim3(nblk(g.tilevox, 256)), dim3(256), 0, s, sim, ti, g, dst,
-                     (const int64_t*)pl->pick, (const uint8_t*)pl->keep, pl->cuts + r * g.padvox);
+  hipLaunchKernelGGL((k_graphcut<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB), pl->cut_lds_max, s, D, q);
+  HCL();
+  hipLaunchKernelGGL((k_paste<T>), dim3(nblk(g.tilevox, 256), nq), dim3(256), 0, s, D, q);
   HCL();
 }
 
@@ -768,27 +713,56 @@ template <typename T>
 void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   hipStream_t s = pl->ctx->stream;
   const IqGeo& g = pl->geo;
+  IqDev& H = pl->hd;
   pl->prof_used = 0;
   pl->prof_flop = 0.0;
   pl->prof_sec = 0.0;
   pl->prof_launches = 0;
-  HC(hipEventRecord(pl->e0, s));
-  hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(64), 0, s, pl->err, (int64_t)1, 0);
-  hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, pl->rounds, (int64_t)1, (int64_t)0);
-  hipLaunchKernelGGL((k_fill<int64_t>), dim3(256), dim3(256), 0, s, pl->log, pl->nreal * pl->npath, (int64_t)-1);
-  HCL();
-  for (int64_t r = 0; r < pl->nreal; r++) {
-    hipLaunchKernelGGL((k_fill<T>), dim3(1024), dim3(256), 0, s, sim_of<T>(pl, r), g.padvox, (T)0);
-    hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s, pl->cuts + r * g.padvox, g.padvox, (uint8_t)0);
-    hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, pl->pick, (int64_t)1, (int64_t)-1);
-    HCL();
-    for (int64_t i = 0; i < pl->npath; i++) {
-      if (!pl->steps[i].active) continue;
-      const double u = uniforms ? uniforms[r * pl->npath + i] : 0.5;
-      const int64_t f = forced ? forced[r * pl->npath + i] : -1;
-      run_step<T>(pl, r, i, u, f);
+  pl->batch_flop.clear();
+  // query table of every batch
+  std::vector<IqQuery> qs;
+  struct Batch { int64_t off; int n; bool plain, multi; };
+  std::vector<Batch> batches;
+  for (const auto& wave : pl->waves) {
+    std::vector<IqQuery> wq;
+    for (int64_t r = 0; r < pl->nreal; r++)
+      for (int64_t i : wave) {
+        IqQuery q{};
+        q.step = (int32_t)i;
+        q.real = (int32_t)r;
+        q.multi = (pl->steps[i].hard || pl->naux > 0) ? 1 : 0;
+        q.u = uniforms ? uniforms[r * pl->npath + i] : 0.5;
+        q.forced = forced ? forced[r * pl->npath + i] : -1;
+        wq.push_back(q);
+      }
+    for (size_t b0 = 0; b0 < wq.size(); b0 += (size_t)pl->nslots) {
+      Batch bt{(int64_t)qs.size(), 0, false, false};
+      for (size_t k = b0; k < wq.size() && k < b0 + (size_t)pl->nslots; k++) {
+        IqQuery q = wq[k];
+        q.slot = (int32_t)(k - b0);
+        bt.plain |= !q.multi;
+        bt.multi |= q.multi != 0;
+        qs.push_back(q);
+        bt.n++;
+      }
+      batches.push_back(bt);
     }
   }
+  if ((int64_t)qs.size() > pl->dq_cap) {
+    pl->dq = pl->mem.alloc<IqQuery>(qs.size());
+    pl->dq_cap = (int64_t)qs.size();
+  }
+  if (!qs.empty()) HC(hipMemcpyAsync(pl->dq, qs.data(), qs.size() * sizeof(IqQuery), hipMemcpyHostToDevice, s));
+  HC(hipEventRecord(pl->e0, s));
+  hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(64), 0, s, H.err, (int64_t)1, 0);
+  hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, H.rounds, (int64_t)1, (int64_t)0);
+  hipLaunchKernelGGL((k_fill<int64_t>), dim3(256), dim3(256), 0, s, H.log, pl->nreal * pl->npath, (int64_t)-1);
+  hipLaunchKernelGGL((k_fill<T>), dim3(1024), dim3(256), 0, s, (T*)H.sims, pl->nreal * g.padvox, (T)0);
+  hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s, H.cuts, pl->nreal * g.padvox, (uint8_t)0);
+  hipLaunchKernelGGL((k_fill<int64_t>), dim3(16), dim3(256), 0, s, H.pick, pl->nslots, (int64_t)-1);
+  HCL();
+  for (const Batch& b : batches)
+    run_batch<T>(pl, qs.data() + b.off, pl->dq + b.off, b.n, b.plain, b.multi);
   HC(hipEventRecord(pl->e1, s));
   HC(hipStreamSynchronize(s));
   HC(hipEventElapsedTime(&pl->last_ms, pl->e0, pl->e1));
@@ -799,7 +773,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
     pl->prof_launches++;
   }
   int32_t code = 0;
-  HC(hipMemcpy(&code, pl->err, sizeof(int32_t), hipMemcpyDeviceToHost));
+  HC(hipMemcpy(&code, H.err, sizeof(int32_t), hipMemcpyDeviceToHost));
   check_kernel_error(code);
 }
 
@@ -910,20 +884,14 @@ int iq_fastdistance(iq_ctx* ctx, const void* img, int img_dtype, const int64_t i
     double* kv = m.alloc<double>(std::max(V, 1));
     double* b2 = m.alloc<double>(1);
     double* dout = m.alloc<double>(g.P);
-    int32_t zero3[3] = {0, 0, 0};
-    int32_t* dorg = m.upload(zero3, 3, s);
-    hipLaunchKernelGGL((k_query_prep<double>), dim3(1), dim3(256), 0, s, (const double*)dk, g, (const int32_t*)dorg,
-                       (const int32_t*)dtv, V, (const IqMask*)nullptr, kv, (float*)nullptr, b2, (uint32_t*)nullptr,
-                       (unsigned long long*)nullptr);
+    hipLaunchKernelGGL(k_gather_b2, dim3(1), dim3(256), 0, s, (const double*)dk, (const int32_t*)dtv, V, kv, b2);
     HCL();
     if (img_dtype == IQ_F32)
       hipLaunchKernelGGL((k_exact_map<float, 0>), dim3(nblk(g.P, 256)), dim3(256), 0, s, (const float*)di, g,
-                         (const int32_t*)dto, (const double*)kv, V, (const double*)b2, (const double*)nullptr,
-                         (const uint8_t*)nullptr, dout);
+                         (const int32_t*)dto, (const double*)kv, V, (const double*)b2, dout);
     else
       hipLaunchKernelGGL((k_exact_map<double, 0>), dim3(nblk(g.P, 256)), dim3(256), 0, s, (const double*)di, g,
-                         (const int32_t*)dto, (const double*)kv, V, (const double*)b2, (const double*)nullptr,
-                         (const uint8_t*)nullptr, dout);
+                         (const int32_t*)dto, (const double*)kv, V, (const double*)b2, dout);
     HCL();
     HC(hipMemcpyAsync(out, dout, g.P * sizeof(double), hipMemcpyDeviceToHost, s));
     HC(hipStreamSynchronize(s));
@@ -956,7 +924,7 @@ int iq_relaxation(iq_ctx* ctx, const double* D, const double* const* Ds, int nau
                   int64_t* out_idx, int64_t* out_count) {
   if (!ctx) return fail(nullptr, IQ_ERR_ARG, "null context");
   return guard(ctx, [&] {
-    if (!D || !out_idx || !out_count || n <= 0 || naux < 1 || naux > 15 || !Ds) bad(IQ_ERR_ARG, "invalid argument");
+    if (!D || !out_idx || !out_count || n <= 0 || naux < 1 || naux > IQ_MAX_AUX + 1 || !Ds) bad(IQ_ERR_ARG, "invalid argument");
     if (n >= (1ll << 31)) bad(IQ_ERR_UNSUPPORTED, "more than 2^31 candidates");
     hipStream_t s = ctx->stream;
     DMem m;
@@ -985,9 +953,7 @@ int iq_relaxation(iq_ctx* ctx, const double* D, const double* const* Ds, int nau
     int64_t* cnt = m.alloc<int64_t>(1);
     int32_t* err = m.alloc<int32_t>(1);
     HC(hipMemsetAsync(err, 0, sizeof(int32_t), s));
-    SelOut o{nullptr, nullptr, err, cnt};
-    hipLaunchKernelGGL(k_relax_finalize, dim3(1), dim3(FB), 0, s, n, nsrc, dmaps, dsorted, tol, 0.0, (int64_t)-1,
-                       bits, ev, (double*)nullptr, (int32_t*)nullptr, (double*)nullptr, (double*)nullptr, o);
+    hipLaunchKernelGGL(k_relax_host, dim3(1), dim3(FB), 0, s, n, nsrc, dmaps, dsorted, tol, bits, ev, cnt, err);
     HCL();
     int32_t code = 0;
     int64_t c = 0;
@@ -1061,27 +1027,17 @@ int iq_graphcut(iq_ctx* ctx, const void* A, const void* B, int dtype, const int6
     if (dim < 0 || dim > 2 || dims[dim] < 2) bad(IQ_ERR_ARG, "cut axis needs at least two slices");
     for (int d = 0; d < 3; d++)
       if (dims[d] <= 0) bad(IQ_ERR_ARG, "invalid slab extents");
-    IqGeo g{};
-    geo_init(g, dims, dims, dims);  // A is the grid, B is the training image, one candidate at 0
-    IqStep st{};
-    st.active = 1;
-    st.nslab = 1;
-    st.slab[0].dim = dim;
-    st.slab[0].side = 0;
-    for (int d = 0; d < 3; d++) st.slab[0].sz[d] = (int32_t)dims[d];
+    const int64_t nvox = dims[0] * dims[1] * dims[2];
     int L = (int)dims[dim];
-    int64_t n = g.tilevox / L * std::max(0, L - 2);
+    int64_t n = nvox / L * std::max(0, L - 2);
     hipStream_t s = ctx->stream;
     DMem m;
     const size_t esz = dtype == IQ_F32 ? 4 : 8;
-    char* dA = m.alloc<char>(g.tilevox * esz);
-    char* dB = m.alloc<char>(g.tilevox * esz);
-    HC(hipMemcpyAsync(dA, A, g.tilevox * esz, hipMemcpyHostToDevice, s));
-    HC(hipMemcpyAsync(dB, B, g.tilevox * esz, hipMemcpyHostToDevice, s));
-    IqStep* dst = m.upload(&st, 1, s);
-    int64_t zero = 0;
-    int64_t* pick = m.upload(&zero, 1, s);
-    uint8_t* keep = m.alloc<uint8_t>(g.tilevox);
+    char* dA = m.alloc<char>(nvox * esz);
+    char* dB = m.alloc<char>(nvox * esz);
+    HC(hipMemcpyAsync(dA, A, nvox * esz, hipMemcpyHostToDevice, s));
+    HC(hipMemcpyAsync(dB, B, nvox * esz, hipMemcpyHostToDevice, s));
+    uint8_t* keep = m.alloc<uint8_t>(nvox);
     int32_t* err = m.alloc<int32_t>(1);
     HC(hipMemsetAsync(err, 0, sizeof(int32_t), s));
     char* scratch = nullptr;
@@ -1089,21 +1045,19 @@ int iq_graphcut(iq_ctx* ctx, const void* A, const void* B, int dtype, const int6
     if (n > IQ_PR_NLDS) scratch = m.alloc<char>((size_t)n * CUT_NODE_BYTES);
     else lds = (size_t)n * CUT_NODE_BYTES;
     if (lds > 64 * 1024) {
-      HC(hipFuncSetAttribute((const void*)k_graphcut<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      HC(hipFuncSetAttribute((const void*)k_graphcut<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      HC(hipFuncSetAttribute((const void*)k_graphcut_host<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      HC(hipFuncSetAttribute((const void*)k_graphcut_host<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     if (dtype == IQ_F32)
-      hipLaunchKernelGGL((k_graphcut<float>), dim3(1), dim3(CB), lds, s, (const float*)dA, (const float*)dB, g,
-                         (const IqStep*)dst, (const int64_t*)pick, keep, scratch, (int64_t)(n * CUT_NODE_BYTES), err,
-                         (int64_t*)nullptr);
+      hipLaunchKernelGGL((k_graphcut_host<float>), dim3(1), dim3(CB), lds, s, (const float*)dA, (const float*)dB,
+                         (int)dims[0], (int)dims[1], (int)dims[2], dim, keep, scratch, err);
     else
-      hipLaunchKernelGGL((k_graphcut<double>), dim3(1), dim3(CB), lds, s, (const double*)dA, (const double*)dB, g,
-                         (const IqStep*)dst, (const int64_t*)pick, keep, scratch, (int64_t)(n * CUT_NODE_BYTES), err,
-                         (int64_t*)nullptr);
+      hipLaunchKernelGGL((k_graphcut_host<double>), dim3(1), dim3(CB), lds, s, (const double*)dA, (const double*)dB,
+                         (int)dims[0], (int)dims[1], (int)dims[2], dim, keep, scratch, err);
     HCL();
     int32_t code = 0;
     HC(hipMemcpyAsync(&code, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HC(hipMemcpyAsync(out_mask, keep, g.tilevox, hipMemcpyDeviceToHost, s));
+    HC(hipMemcpyAsync(out_mask, keep, nvox, hipMemcpyDeviceToHost, s));
     HC(hipStreamSynchronize(s));
     check_kernel_error(code);
   });
@@ -1149,13 +1103,13 @@ int iq_plan_results(iq_ctx* ctx, iq_plan* plan, void* out, int64_t* chosen, doub
     hipStream_t s = ctx->stream;
     const IqGeo& g = plan->geo;
     const size_t esz = plan->dtype == IQ_F32 ? 4 : 8;
-    if (out) HC(hipMemcpyAsync(out, plan->sims, (size_t)plan->nreal * g.padvox * esz, hipMemcpyDeviceToHost, s));
+    if (out) HC(hipMemcpyAsync(out, plan->hd.sims, (size_t)plan->nreal * g.padvox * esz, hipMemcpyDeviceToHost, s));
     if (chosen && plan->npath)
-      HC(hipMemcpyAsync(chosen, plan->log, (size_t)plan->nreal * plan->npath * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      HC(hipMemcpyAsync(chosen, plan->hd.log, (size_t)plan->nreal * plan->npath * sizeof(int64_t), hipMemcpyDeviceToHost, s));
     std::vector<uint8_t> c;
     if (cutgrid) {
       c.resize((size_t)plan->nreal * g.padvox);
-      HC(hipMemcpyAsync(c.data(), plan->cuts, c.size(), hipMemcpyDeviceToHost, s));
+      HC(hipMemcpyAsync(c.data(), plan->hd.cuts, c.size(), hipMemcpyDeviceToHost, s));
     }
     HC(hipStreamSynchronize(s));
     for (size_t i = 0; i < c.size(); i++) cutgrid[i] = (double)c[i];

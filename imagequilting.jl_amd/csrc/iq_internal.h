@@ -6,6 +6,7 @@
 
 #define IQ_MAX_BOX 27       // disjoint boxes describing one overlap mask (3^3 cells at most)
 #define IQ_MAX_SLAB 6       // prev/next slab per axis (iqsim.jl:252-275)
+#define IQ_MAX_AUX 8        // soft-data pairs
 #define IQ_SB 64            // block of the three-level cumulative sums of the draw
 #define IQ_GR_FREQ 8        // push-relabel rounds between global relabels
 #define IQ_PR_LIMIT 200000  // push-relabel round guard (never reached on valid input)
@@ -54,6 +55,16 @@ struct IqStep {
   IqSlab slab[IQ_MAX_SLAB];
 };
 
+// One tile step of one realization inside a batch (a wave of independent tiles).
+struct IqQuery {
+  int32_t step;    // path position
+  int32_t real;    // realization
+  int32_t slot;    // scratch slot
+  int32_t multi;   // 1: relaxation / tau model over several sources
+  double u;        // uniform of the draw
+  int64_t forced;  // pick to replay, or -1
+};
+
 // Geometry shared by all kernels of a plan.
 struct IqGeo {
   int32_t ti[3];     // training-image extents
@@ -63,6 +74,63 @@ struct IqGeo {
   int64_t P;         // number of candidates
   int64_t padvox;
   int32_t tilevox;
+};
+
+// Everything a kernel needs, resident in device memory (read through the
+// scalar cache).  Per-slot scratch arrays are laid out slot-major.
+struct IqDev {
+  IqGeo g;
+  const void* ti;
+  const float* ti32;
+  int32_t ti_f32;
+  int32_t naux;
+  double tol;
+  const uint8_t* disabled;
+  const int32_t* enabled;
+  int64_t nen;
+  const IqStep* steps;
+  const IqMask* masks;
+  const int32_t* tvox;   // tile-linear offsets (ascending column-major) per mask, then the full tile
+  const int32_t* toff;   // matching training-image offsets
+  const int32_t* htvox;  // hard voxels per step
+  const int32_t* htoff;
+  int32_t full_off;
+  const double* const* a2maps;
+  const double* hardgrid;
+  const double* const* aux;
+  const double* const* auxti;
+  const double* const* a2soft;
+  void* sims;
+  uint8_t* cuts;
+  int64_t* log;
+  int64_t npath;
+  int32_t* err;
+  int64_t* rounds;
+  // per-slot scratch
+  double* kv;        int64_t kv_stride;    // (2 + naux) query vectors of tilevox
+  float* kv32;       int64_t kv32_stride;
+  double* b2;                              // (2 + naux) per slot
+  uint32_t* gmin_hi;
+  unsigned long long* gmin64;
+  float* lo32;                             // P per slot
+  double* dexact;                          // P per slot
+  unsigned long long* bitmap; int64_t nw64;
+  int32_t* band;                           // P per slot
+  int32_t* sup;                            // superset list, P per slot
+  int32_t* nsup;
+  double* sd;                              // IQ_SORT_MAX per slot (band distances, then weights)
+  int32_t* si;                             // IQ_SORT_MAX per slot (ranks)
+  double* s1;        int64_t s1_stride;
+  int64_t* pick;
+  int64_t* nband;
+  uint8_t* keep;                           // IQ_MAX_SLAB * tilevox per slot
+  char* cutscratch;  int64_t cutscratch_per;
+  // multi-source selection (relaxation.jl / taumodel.jl), per slot
+  double* maps;      int32_t nmap;         // nmap maps of P: 0 overlap, 1 hard, 2+a soft
+  int32_t* sorted;                         // stable ascending order of each map
+  uint32_t* bits;    int64_t nw32;         // nmap bitmaps of P bits
+  int32_t* ev;                             // pattern database (P)
+  double* vals;                            // P
 };
 
 __host__ __device__ inline int64_t iq_lin3(const int32_t d[3], int64_t x, int64_t y, int64_t z) {
@@ -80,4 +148,12 @@ __device__ inline float iq_f32_up(double x) {
   float f = (float)x;
   if ((double)f < x) f = __uint_as_float(__float_as_uint(f) + 1u);
   return f;
+}
+
+// broadcast of lane `l` (uniform) of a double held in VGPRs
+__device__ inline double iq_readlane_f64(double v, int l) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
 }

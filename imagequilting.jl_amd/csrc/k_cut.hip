@@ -87,29 +87,15 @@ __device__ inline void cut_bind(CutState& S, char* mem, int n) {
 // bytes of state per interior node
 #define CUT_NODE_BYTES (8 * 11 + 4 * 3)
 
+// Minimum cut of one slab (graphcut.jl:5-84) by the whole workgroup: writes
+// keep = mask XOR side for every slab voxel into out (slab column-major).
+// A and B are strided views (the simulation grid and the chosen training-image
+// window); S points at LDS or global scratch sized for the interior nodes.
 template <typename T>
-__global__ void __launch_bounds__(CB)
-k_graphcut(const T* __restrict__ sim, const T* __restrict__ ti, IqGeo g, const IqStep* __restrict__ step,
-           const int64_t* __restrict__ pick, uint8_t* __restrict__ keep, char* __restrict__ gscratch,
-           int64_t gscratch_per_slab, int32_t* __restrict__ err, int64_t* __restrict__ rounds) {
-  extern __shared__ char cut_lds[];
-  const int64_t pk = *pick;
-  if (pk < 0) return;
-  const IqSlab sl = step->slab[blockIdx.x];
-  const int dim = sl.dim;
-  const int s[3] = {sl.sz[0], sl.sz[1], sl.sz[2]};
+__device__ void cut_slab(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
+                         uint8_t* out, char* mem, int32_t* err, int64_t* rounds) {
   const int L = s[dim];
-  // slab views: A = current grid over the tile, B = chosen training-image window
-  const int rc[3] = {(int)(pk % g.dist[0]), (int)((pk / g.dist[0]) % g.dist[1]),
-                     (int)(pk / ((int64_t)g.dist[0] * g.dist[1]))};
-  SlabView<T> A, B;
-  A.base = sim + iq_lin3(g.pad, step->origin[0] + sl.lo[0], step->origin[1] + sl.lo[1], step->origin[2] + sl.lo[2]);
-  A.st[0] = 1; A.st[1] = g.pad[0]; A.st[2] = (int64_t)g.pad[0] * g.pad[1];
-  B.base = ti + iq_lin3(g.ti, rc[0] + sl.lo[0], rc[1] + sl.lo[1], rc[2] + sl.lo[2]);
-  B.st[0] = 1; B.st[1] = g.ti[0]; B.st[2] = (int64_t)g.ti[0] * g.ti[1];
-  uint8_t* out = keep + (int64_t)blockIdx.x * g.tilevox;
   const int nvox = s[0] * s[1] * s[2];
-  const uint8_t side = (uint8_t)sl.side;
   // terminal slices
   for (int p = threadIdx.x; p < nvox; p += CB) {
     int c[3] = {p % s[0], (p / s[0]) % s[1], p / (s[0] * s[1])};
@@ -123,8 +109,7 @@ k_graphcut(const T* __restrict__ sim, const T* __restrict__ ti, IqGeo g, const I
   const int HMAX = n + 1;
   const int ist[3] = {1, si[0], si[0] * si[1]};
   CutState S;
-  if (n <= IQ_PR_NLDS) cut_bind(S, cut_lds, n);
-  else cut_bind(S, gscratch + (int64_t)blockIdx.x * gscratch_per_slab, n);
+  cut_bind(S, mem, n);
 
   // build: capacities of graphcut.jl:22-54 on interior edges; saturated source edges
   for (int u = threadIdx.x; u < n; u += CB) {
@@ -287,37 +272,92 @@ k_graphcut(const T* __restrict__ sim, const T* __restrict__ ti, IqGeo g, const I
   if (threadIdx.x == 0 && rounds) atomicAdd((unsigned long long*)rounds, (unsigned long long)total);
 }
 
-template __global__ void k_graphcut<float>(const float*, const float*, IqGeo, const IqStep*, const int64_t*, uint8_t*, char*, int64_t, int32_t*, int64_t*);
-template __global__ void k_graphcut<double>(const double*, const double*, IqGeo, const IqStep*, const int64_t*, uint8_t*, char*, int64_t, int32_t*, int64_t*);
-
-// Paste (iqsim.jl:278): voxels whose OR-ed keep flag is 0 take the chosen
-// training-image window; the keep flags are the debug cut grid (iqsim.jl:281).
+// Batched cut: grid (slab, query).  A = the realization's grid over the
+// tile, B = the picked training-image window (iqsim.jl:244-273).
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_paste(T* __restrict__ sim, const T* __restrict__ ti, IqGeo g, const IqStep* __restrict__ step,
-        const int64_t* __restrict__ pick, const uint8_t* __restrict__ keep, uint8_t* __restrict__ cut) {
-  const int64_t pk = *pick;
+__global__ void __launch_bounds__(CB)
+k_graphcut(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  extern __shared__ char cut_lds[];
+  const IqQuery q = qs[blockIdx.y];
+  const IqStep& st = D->steps[q.step];
+  if ((int)blockIdx.x >= st.nslab) return;
+  const int64_t slot = q.slot;
+  const int64_t pk = D->pick[slot];
   if (pk < 0) return;
+  const IqGeo& g = D->g;
+  const IqSlab sl = st.slab[blockIdx.x];
+  const int s[3] = {sl.sz[0], sl.sz[1], sl.sz[2]};
   const int rc[3] = {(int)(pk % g.dist[0]), (int)((pk / g.dist[0]) % g.dist[1]),
                      (int)(pk / ((int64_t)g.dist[0] * g.dist[1]))};
-  const int ns = step->nslab;
+  SlabView<T> A, B;
+  A.base = (const T*)D->sims + (int64_t)q.real * g.padvox +
+           iq_lin3(g.pad, st.origin[0] + sl.lo[0], st.origin[1] + sl.lo[1], st.origin[2] + sl.lo[2]);
+  A.st[0] = 1; A.st[1] = g.pad[0]; A.st[2] = (int64_t)g.pad[0] * g.pad[1];
+  B.base = (const T*)D->ti + iq_lin3(g.ti, rc[0] + sl.lo[0], rc[1] + sl.lo[1], rc[2] + sl.lo[2]);
+  B.st[0] = 1; B.st[1] = g.ti[0]; B.st[2] = (int64_t)g.ti[0] * g.ti[1];
+  uint8_t* out = D->keep + (slot * IQ_MAX_SLAB + blockIdx.x) * g.tilevox;
+  const int n = s[0] * s[1] * s[2] / s[sl.dim] * (s[sl.dim] > 2 ? s[sl.dim] - 2 : 0);
+  char* mem = n <= IQ_PR_NLDS ? cut_lds
+                              : D->cutscratch + (slot * IQ_MAX_SLAB + blockIdx.x) * D->cutscratch_per;
+  cut_slab<T>(A, B, s, sl.dim, (uint8_t)sl.side, out, mem, D->err, D->rounds);
+}
+
+template __global__ void k_graphcut<float>(const IqDev*, const IqQuery*);
+template __global__ void k_graphcut<double>(const IqDev*, const IqQuery*);
+
+// Standalone cut of one slab pair (iq_graphcut): A and B both column-major of extents s.
+template <typename T>
+__global__ void __launch_bounds__(CB)
+k_graphcut_host(const T* __restrict__ a, const T* __restrict__ b, int s0, int s1, int s2, int dim,
+                uint8_t* __restrict__ out, char* __restrict__ gmem, int32_t* __restrict__ err) {
+  extern __shared__ char cut_lds[];
+  const int s[3] = {s0, s1, s2};
+  SlabView<T> A, B;
+  A.base = a; A.st[0] = 1; A.st[1] = s0; A.st[2] = (int64_t)s0 * s1;
+  B.base = b; B.st[0] = 1; B.st[1] = s0; B.st[2] = (int64_t)s0 * s1;
+  const int n = s0 * s1 * s2 / s[dim] * (s[dim] > 2 ? s[dim] - 2 : 0);
+  cut_slab<T>(A, B, s, dim, (uint8_t)0, out, n <= IQ_PR_NLDS ? cut_lds : gmem, err, nullptr);
+}
+
+template __global__ void k_graphcut_host<float>(const float*, const float*, int, int, int, int, uint8_t*, char*, int32_t*);
+template __global__ void k_graphcut_host<double>(const double*, const double*, int, int, int, int, uint8_t*, char*, int32_t*);
+
+// Paste (iqsim.jl:278): voxels whose OR-ed keep flag is 0 take the picked
+// training-image window; the keep flags are the debug cut grid (iqsim.jl:281).
+// Grid: (tile blocks, query).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_paste(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  const IqQuery q = qs[blockIdx.y];
+  const int64_t slot = q.slot;
+  const int64_t pk = D->pick[slot];
+  if (pk < 0) return;
+  const IqGeo& g = D->g;
+  const IqStep& st = D->steps[q.step];
+  const int rc[3] = {(int)(pk % g.dist[0]), (int)((pk / g.dist[0]) % g.dist[1]),
+                     (int)(pk / ((int64_t)g.dist[0] * g.dist[1]))};
+  const int ns = st.nslab;
+  T* sim = (T*)D->sims + (int64_t)q.real * g.padvox;
+  uint8_t* cut = D->cuts + (int64_t)q.real * g.padvox;
+  const T* ti = (const T*)D->ti;
+  const uint8_t* keep = D->keep + slot * IQ_MAX_SLAB * g.tilevox;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < g.tilevox; j += gridDim.x * blockDim.x) {
     int x = j % g.tile[0], y = (j / g.tile[0]) % g.tile[1], z = j / (g.tile[0] * g.tile[1]);
     uint8_t k = 0;
     for (int s = 0; s < ns; s++) {
-      const IqSlab& sl = step->slab[s];
+      const IqSlab& sl = st.slab[s];
       int a = x - sl.lo[0], b = y - sl.lo[1], c = z - sl.lo[2];
       if (a >= 0 && b >= 0 && c >= 0 && a < sl.sz[0] && b < sl.sz[1] && c < sl.sz[2])
         k |= keep[(int64_t)s * g.tilevox + a + sl.sz[0] * (b + sl.sz[1] * c)];
     }
-    int64_t gi = iq_lin3(g.pad, step->origin[0] + x, step->origin[1] + y, step->origin[2] + z);
+    int64_t gi = iq_lin3(g.pad, st.origin[0] + x, st.origin[1] + y, st.origin[2] + z);
     if (!k) sim[gi] = ti[iq_lin3(g.ti, rc[0] + x, rc[1] + y, rc[2] + z)];
-    if (cut) cut[gi] = k;
+    cut[gi] = k;
   }
 }
 
-template __global__ void k_paste<float>(float*, const float*, IqGeo, const IqStep*, const int64_t*, const uint8_t*, uint8_t*);
-template __global__ void k_paste<double>(double*, const double*, IqGeo, const IqStep*, const int64_t*, const uint8_t*, uint8_t*);
+template __global__ void k_paste<float>(const IqDev*, const IqQuery*);
+template __global__ void k_paste<double>(const IqDev*, const IqQuery*);
 
 // Fill helpers (no memset nodes: the run may be captured in a graph)
 template <typename T>
@@ -330,13 +370,7 @@ template __global__ void k_fill<uint8_t>(uint8_t*, int64_t, uint8_t);
 template __global__ void k_fill<int64_t>(int64_t*, int64_t, int64_t);
 template __global__ void k_fill<int32_t>(int32_t*, int64_t, int32_t);
 
-// Disabled candidates -> +inf in an exact map (iqsim.jl:207,217,226).
-extern "C" __global__ void k_zero_map(double* __restrict__ d, const uint8_t* __restrict__ disabled, int64_t P) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x)
-    d[i] = disabled[i] ? __builtin_inf() : 0.0;
-}
-
-// Pick index from host (replay of a forced pick for the standalone cut tests).
+// 0, 1, 2, ... (values of the stable sorts)
 extern "C" __global__ void k_iota32(int32_t* __restrict__ v, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     v[i] = (int32_t)i;

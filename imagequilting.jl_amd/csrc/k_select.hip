@@ -3,8 +3,8 @@
 //   * relaxation     soft/hard pattern database              (relaxation.jl:5-48)
 //   * taumodel       dense ranks -> tau-model weights         (taumodel.jl:5-45)
 //   * weighted draw  with a caller-supplied uniform          (iqsim.jl:243)
-// Each kernel is one 1024-wide workgroup per query.  Every floating-point
-// step uses the reference's operator order (the library is compiled with
+// One 1024-wide workgroup per query.  Every floating-point step uses the
+// reference's operator order (the library is compiled with
 // -ffp-contract=off), so results are bit-identical to the FP64 oracle.
 #include "iq_internal.h"
 
@@ -84,26 +84,23 @@ __device__ void blk_bitonic(double* sv, int m) {
   }
 }
 
-// Dense ranks (ties share a rank, taumodel.jl:21-31) of vals[0..n) written to
-// rank[0..n).  Returns false when n values with more than one distinct value
-// exceed the LDS sorter.
-__device__ bool blk_dense_rank(const double* vals, int64_t n, int32_t* rank, double* sv, int32_t* dp,
-                               SelScratch& sh) {
+// Dense ranks (ties share a rank, taumodel.jl:21-31) of vals[0..n).  When all
+// values are equal every rank is 1 and *constant is set (rank is not
+// written).  Returns false for more than IQ_SORT_MAX values with several
+// distinct ones.
+__device__ bool blk_dense_rank(const double* vals, int64_t n, int32_t* rank, bool* constant, double* sv,
+                               int32_t* dp, SelScratch& sh) {
   double lmin = __builtin_inf(), lmax = -__builtin_inf();
   for (int64_t i = threadIdx.x; i < n; i += FB) { lmin = fmin(lmin, vals[i]); lmax = fmax(lmax, vals[i]); }
   double mn = blk_min_f64(lmin, sh), mx = blk_max_f64(lmax, sh);
-  if (mn == mx) {
-    for (int64_t i = threadIdx.x; i < n; i += FB) rank[i] = 1;
-    __syncthreads();
-    return true;
-  }
+  *constant = mn == mx;
+  if (*constant) return true;
   if (n > IQ_SORT_MAX) return false;
   int m = 1;
   while (m < n) m <<= 1;
   for (int i = threadIdx.x; i < m; i += FB) sv[i] = i < n ? vals[i] : __builtin_inf();
   __syncthreads();
   blk_bitonic(sv, m);
-  // inclusive count of distinct values: 4 consecutive entries per work-item
   int64_t local = 0;
   const int per = (m + FB - 1) / FB;
   const int i0 = threadIdx.x * per;
@@ -124,22 +121,37 @@ __device__ bool blk_dense_rank(const double* vals, int64_t n, int32_t* rank, dou
   return true;
 }
 
-// Three-level blocked cumulative draw: the same summation order as the
-// oracle (blocks of IQ_SB at two levels, a sequential top level).
-__device__ int64_t blk_sample(const double* p, int64_t n, double u, double* s1, double* s2, SelScratch& sh) {
+// Sequential sum of n consecutive weights starting at i0 (p == nullptr:
+// every weight is w); loads are issued eight at a time, zero padding is exact.
+__device__ inline double seq_sum(const double* p, double w, int64_t i0, int64_t n) {
+  double s = 0.0;
+  if (!p) {
+    for (int64_t i = 0; i < n; i++) s += w;
+    return s;
+  }
+  int64_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    double v0 = p[i0 + i], v1 = p[i0 + i + 1], v2 = p[i0 + i + 2], v3 = p[i0 + i + 3];
+    double v4 = p[i0 + i + 4], v5 = p[i0 + i + 5], v6 = p[i0 + i + 6], v7 = p[i0 + i + 7];
+    s += v0; s += v1; s += v2; s += v3; s += v4; s += v5; s += v6; s += v7;
+  }
+  for (; i < n; i++) s += p[i0 + i];
+  return s;
+}
+
+// Three-level blocked cumulative draw (the oracle's summation order: blocks of
+// IQ_SB at two levels, a sequential top level).  p == nullptr: all weights w.
+__device__ int64_t blk_sample(const double* p, double w, int64_t n, double u, double* s1, double* s2,
+                              SelScratch& sh) {
   const int64_t n1 = (n + IQ_SB - 1) / IQ_SB, n2 = (n1 + IQ_SB - 1) / IQ_SB;
   for (int64_t k = threadIdx.x; k < n1; k += FB) {
-    double s = 0.0;
     int64_t e = (k + 1) * IQ_SB < n ? (k + 1) * IQ_SB : n;
-    for (int64_t i = k * IQ_SB; i < e; i++) s += p[i];
-    s1[k] = s;
+    s1[k] = seq_sum(p, w, k * IQ_SB, e - k * IQ_SB);
   }
   __syncthreads();
   for (int64_t m = threadIdx.x; m < n2; m += FB) {
-    double s = 0.0;
     int64_t e = (m + 1) * IQ_SB < n1 ? (m + 1) * IQ_SB : n1;
-    for (int64_t k = m * IQ_SB; k < e; k++) s += s1[k];
-    s2[m] = s;
+    s2[m] = seq_sum(s1, 0.0, m * IQ_SB, e - m * IQ_SB);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -161,7 +173,7 @@ __device__ int64_t blk_sample(const double* p, int64_t n, double u, double* s1, 
     }
     int64_t i = k * IQ_SB, iend = (k + 1) * IQ_SB < n ? (k + 1) * IQ_SB : n;
     for (;; i++) {
-      double nx = acc + p[i];
+      double nx = acc + (p ? p[i] : w);
       if (nx >= t || i == iend - 1) break;
       acc = nx;
     }
@@ -186,13 +198,13 @@ __device__ int64_t blk_find(const int32_t* idx, int64_t n, int64_t key, SelScrat
   return r;
 }
 
-__device__ void set_err(int32_t* err, int code) {
-  atomicCAS(err, 0, code);
-}
+__device__ void set_err(int32_t* err, int code) { atomicCAS(err, 0, code); }
 
-// Tau-model weights (taumodel.jl:34-44) for one more source with dense
-// ranks `rank`; p holds the running product prod(X / x0) (starts at 1.0).
-__device__ void tau_accumulate(int64_t n, const int32_t* rank, double* p, SelScratch& sh) {
+// Tau-model factor of one more source (taumodel.jl:34-44): p[i] *= X_i / x0
+// with P = (n - r + 1) / sum(n - r + 1), X = (1 - P) / P.  A constant
+// source (all ranks 1) gives X == x0 exactly, a neutral factor of 1.0.
+__device__ void tau_accumulate(int64_t n, const int32_t* rank, bool constant, double* p, SelScratch& sh) {
+  if (constant) return;  // p * 1.0 == p
   int64_t ls = 0;
   for (int64_t i = threadIdx.x; i < n; i += FB) ls += (n - rank[i]) + 1;
   const double S = (double)blk_sum_i64(ls, sh);  // exact integer sum
@@ -206,35 +218,47 @@ __device__ void tau_accumulate(int64_t n, const int32_t* rank, double* p, SelScr
   __syncthreads();
 }
 
-__device__ void tau_finish(int64_t n, double* p) {
+__device__ inline double tau_weight(int64_t n, double prodv) {
   const double nd = (double)n;
   const double x0 = (1.0 - 1.0 / nd) / (1.0 / nd);
-  for (int64_t i = threadIdx.x; i < n; i += FB) {
-    double x = x0 * p[i];
-    p[i] = 1.0 / (1.0 + x);
-  }
+  double x = x0 * prodv;
+  return 1.0 / (1.0 + x);
+}
+
+__device__ void tau_finish(int64_t n, double* p) {
+  for (int64_t i = threadIdx.x; i < n; i += FB) p[i] = tau_weight(n, p[i]);
   __syncthreads();
 }
 
+// Weight shared by every event when all sources are constant (all ranks 1).
+__device__ inline double tau_constant_weight(int64_t n) {
+  const double nd = (double)n;
+  const double S = nd * nd;  // exact for n < 2^26
+  const double x0 = (1.0 - 1.0 / nd) / (1.0 / nd);
+  const double P = ((nd - 1.0) + 1.0) / S;
+  const double X = (1.0 - P) / P;
+  return tau_weight(n, 1.0 * (X / x0));
+}
+
 struct SelOut {
-  int64_t* pick;        // current pick of this query (consumed by the cut / paste)
-  int64_t* log;         // pick log entry of this path position
-  int32_t* err;         // run error word
-  int64_t* nband;       // size of the pattern database (diagnostics)
+  int64_t* pick;  // current pick of this query (consumed by the cut / paste)
+  int64_t* log;   // pick log entry of this path position
+  int32_t* err;   // run error word
+  int64_t* nband; // size of the pattern database
 };
 
-// Draw (or replay `forced`) from db[0..n) with weights p; publish the pick.
-__device__ void publish_pick(const int32_t* db, int64_t n, const double* p, double u, int64_t forced,
+// Draw (or replay `forced`) from db[0..n) with weights p (or constant w).
+__device__ void publish_pick(const int32_t* db, int64_t n, const double* p, double w, double u, int64_t forced,
                              double* s1, double* s2, SelOut o, SelScratch& sh) {
   int64_t i;
   if (forced >= 0) {
     i = blk_find(db, n, forced, sh);
     if (i < 0) {
-      if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_NOTINDB); *o.pick = -1; *o.log = forced; }
+      if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_NOTINDB); *o.pick = -2; *o.log = forced; }
       return;
     }
   } else {
-    i = blk_sample(p, n, u, s1, s2, sh);
+    i = blk_sample(p, w, n, u, s1, s2, sh);
   }
   if (threadIdx.x == 0) {
     *o.pick = db[i];
@@ -243,41 +267,93 @@ __device__ void publish_pick(const int32_t* db, int64_t n, const double* p, doub
   }
 }
 
+// Tau weights of n events over nsrc sources (values of source j at event i are
+// src_vals(j, i), staged into vals) -> p (or constant).  False on overflow of
+// the LDS ranker.
+template <class F>
+__device__ bool tau_weights(int64_t n, int nsrc, F src_vals, double* vals, int32_t* rank, double* p, bool* allconst,
+                            double* sv, int32_t* dp, SelScratch& sh) {
+  *allconst = true;
+  if (n == 1) return true;
+  bool started = false;
+  for (int j = 0; j < nsrc; j++) {
+    for (int64_t i = threadIdx.x; i < n; i += FB) vals[i] = src_vals(j, i);
+    __syncthreads();
+    bool c;
+    if (!blk_dense_rank(vals, n, rank, &c, sv, dp, sh)) return false;
+    if (!c) {
+      if (n > IQ_SORT_MAX) return false;
+      if (!started) {
+        for (int64_t i = threadIdx.x; i < n; i += FB) p[i] = 1.0;
+        __syncthreads();
+        started = true;
+      }
+      tau_accumulate(n, rank, false, p, sh);
+      *allconst = false;
+    }
+  }
+  if (started) tau_finish(n, p);
+  return true;
+}
+
 // ---------------------------------------------------------------------------
-// Tol-band finalize after k_band_superset: exact min -> threshold -> ordered
-// compaction of the band -> dense ranks -> weights -> draw.
+// Tol-band finalize (plain queries): exact min -> threshold -> ordered
+// compaction of the band -> dense ranks -> weights -> draw.  Queries without
+// an overlap (first tile, ...) have D == 0 on every enabled candidate, so the
+// band is the enabled list and every rank is 1.
 // ---------------------------------------------------------------------------
 extern "C" __global__ void __launch_bounds__(FB)
-k_band_finalize(int64_t P, const unsigned long long* __restrict__ bitmap,
-                const double* __restrict__ dexact, const unsigned long long* __restrict__ gmin64,
-                double tol, double u, int64_t forced, int32_t* __restrict__ band,
-                double* __restrict__ bandd, int32_t* __restrict__ rank, double* __restrict__ p,
-                double* __restrict__ s1, SelOut o) {
+k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   __shared__ SelScratch sh;
   __shared__ double sv[IQ_SORT_MAX];
   __shared__ int32_t dp[IQ_SORT_MAX];
   __shared__ double s2[S2MAX];
-  if (*o.pick == -2) return;  // an earlier step failed: leave the grid untouched
-  const double m = __longlong_as_double((long long)*gmin64);
-  if (__builtin_isinf(m)) {
-    if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_NOCAND); *o.pick = -2; }
+  const IqQuery q = qs[blockIdx.x];
+  if (q.multi) return;
+  const IqGeo& g = D->g;
+  const IqStep& st = D->steps[q.step];
+  const int64_t slot = q.slot;
+  SelOut o{D->pick + slot, D->log + (int64_t)q.real * D->npath + q.step, D->err, D->nband + slot};
+  double* s1 = D->s1 + slot * D->s1_stride;
+  double* sd = D->sd + slot * IQ_SORT_MAX;
+  int32_t* si = D->si + slot * IQ_SORT_MAX;
+  if (st.mask < 0) {
+    const int64_t nen = D->nen;
+    if (nen == 0) {
+      if (threadIdx.x == 0) { set_err(D->err, IQ_KERR_NOCAND); *o.pick = -2; }
+      return;
+    }
+    const double w = nen == 1 ? 1.0 : tau_constant_weight(nen);
+    publish_pick(D->enabled, nen, nullptr, w, q.u, q.forced, s1, s2, o, sh);
     return;
   }
-  const double thr = (1.0 + tol) * m;  // iqsim.jl:237
-  const int64_t nw = (P + 63) / 64;
+  const double m = __longlong_as_double((long long)D->gmin64[slot]);
+  if (__builtin_isinf(m)) {
+    if (threadIdx.x == 0) { set_err(D->err, IQ_KERR_NOCAND); *o.pick = -2; }
+    return;
+  }
+  const double thr = (1.0 + D->tol) * m;  // iqsim.jl:237
+  const unsigned long long* bitmap = D->bitmap + slot * D->nw64;
+  const double* dexact = D->dexact + slot * g.P;
+  int32_t* band = D->band + slot * g.P;
+  const int64_t nw = D->nw64;
   const int64_t per = (nw + FB - 1) / FB;
   const int64_t w0 = threadIdx.x * per, w1 = (w0 + per < nw) ? w0 + per : nw;
   int64_t cnt = 0;
+  double lmax = -__builtin_inf();
   for (int64_t w = w0; w < w1; w++) {
     unsigned long long b = bitmap[w];
     while (b) {
       int bit = __builtin_ctzll(b);
       b &= b - 1;
-      cnt += dexact[w * 64 + bit] <= thr;
+      double d = dexact[w * 64 + bit];
+      if (d <= thr) { cnt++; lmax = fmax(lmax, d); }
     }
   }
   int64_t nb;
   int64_t off = blk_excl_scan(cnt, &nb, sh);
+  const double dmax = blk_max_f64(lmax, sh);
+  const bool small = nb <= IQ_SORT_MAX;
   for (int64_t w = w0; w < w1; w++) {
     unsigned long long b = bitmap[w];
     while (b) {
@@ -285,96 +361,63 @@ k_band_finalize(int64_t P, const unsigned long long* __restrict__ bitmap,
       b &= b - 1;
       int64_t c = w * 64 + bit;
       double d = dexact[c];
-      if (d <= thr) { band[off] = (int32_t)c; bandd[off] = d; off++; }
+      if (d <= thr) {
+        band[off] = (int32_t)c;
+        if (small) sd[off] = d;
+        off++;
+      }
     }
   }
   __syncthreads();
   if (nb == 1) {
-    if (threadIdx.x == 0) p[0] = 1.0;  // taumodel.jl:9
-    __syncthreads();
-  } else {
-    if (!blk_dense_rank(bandd, nb, rank, sv, dp, sh)) {
-      if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_SORTMAX); *o.pick = -2; }
-      return;
-    }
-    for (int64_t i = threadIdx.x; i < nb; i += FB) p[i] = 1.0;
-    __syncthreads();
-    tau_accumulate(nb, rank, p, sh);
-    tau_finish(nb, p);
-  }
-  publish_pick(band, nb, p, u, forced, s1, s2, o, sh);
-}
-
-// Steps with nothing to compare (no pasted neighbour, no hard or soft data):
-// D == 0 on every enabled candidate, so the band is the enabled list and every
-// rank is 1.
-extern "C" __global__ void __launch_bounds__(FB)
-k_band_enabled(const int32_t* __restrict__ enabled, int64_t nen, double u, int64_t forced,
-               double* __restrict__ p, double* __restrict__ s1, SelOut o) {
-  __shared__ SelScratch sh;
-  __shared__ double s2[S2MAX];
-  if (*o.pick == -2) return;
-  if (nen == 0) {
-    if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_NOCAND); *o.pick = -2; }
+    publish_pick(band, 1, nullptr, 1.0, q.u, q.forced, s1, s2, o, sh);  // taumodel.jl:9
     return;
   }
-  if (nen == 1) {
-    if (threadIdx.x == 0) p[0] = 1.0;
-  } else {
-    const double nd = (double)nen;
-    const double S = (double)(nen * nen);  // sum of (n - 1 + 1) over n events
-    const double x0 = (1.0 - 1.0 / nd) / (1.0 / nd);
-    const double P = ((nd - 1.0) + 1.0) / S;
-    const double X = (1.0 - P) / P;
-    const double pp = 1.0 * (X / x0);
-    const double x = x0 * pp;
-    const double w = 1.0 / (1.0 + x);
-    for (int64_t i = threadIdx.x; i < nen; i += FB) p[i] = w;
+  if (dmax == m) {  // every band distance equal: all ranks 1
+    publish_pick(band, nb, nullptr, tau_constant_weight(nb), q.u, q.forced, s1, s2, o, sh);
+    return;
   }
+  if (!small) {
+    if (threadIdx.x == 0) { set_err(D->err, IQ_KERR_SORTMAX); *o.pick = -2; }
+    return;
+  }
+  bool c;
+  blk_dense_rank(sd, nb, si, &c, sv, dp, sh);
+  for (int64_t i = threadIdx.x; i < nb; i += FB) sd[i] = 1.0;  // sd now holds the weights
   __syncthreads();
-  publish_pick(enabled, nen, p, u, forced, s1, s2, o, sh);
+  tau_accumulate(nb, si, false, sd, sh);
+  tau_finish(nb, sd);
+  publish_pick(band, nb, sd, 0.0, q.u, q.forced, s1, s2, o, sh);
 }
 
 // ---------------------------------------------------------------------------
-// Relaxation finalize (relaxation.jl:5-39 + taumodel.jl with several sources).
-// maps[0] is the primary map D, maps[1..nsrc) the auxiliary maps Ds, each an
-// exact FP64 map over P candidates; sorted[j] is the stable ascending order of
-// maps[j] (ties by index, Julia's Perm ordering).  top-k sets are prefixes of
-// those orders; the running intersection is kept as bitmaps.
+// Relaxation (relaxation.jl:5-39): pattern database from stable orders.
+// maps[0] is the primary map D, maps[1..nsrc) the auxiliary maps Ds;
+// sorted[j] the stable ascending order of maps[j] (ties by index, Julia's Perm
+// ordering).  top-k sets are prefixes of those orders; the running
+// intersection is kept as bitmaps (bits: nsrc maps of nwd words).
+// Returns the size of the database written to ev (ascending), -1 if every
+// candidate is disabled.
 // ---------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(FB)
-k_relax_finalize(int64_t P, int nsrc, const double* const* __restrict__ maps,
-                 const int32_t* const* __restrict__ sorted, double tol, double u,
-                 int64_t forced, uint32_t* __restrict__ bits, int32_t* __restrict__ ev,
-                 double* __restrict__ vals, int32_t* __restrict__ rank, double* __restrict__ p,
-                 double* __restrict__ s1, SelOut o) {
-  // o.pick == nullptr: stop after the pattern database (written to ev, size to *o.nband)
-  __shared__ SelScratch sh;
-  __shared__ double sv[IQ_SORT_MAX];
-  __shared__ int32_t dp[IQ_SORT_MAX];
-  __shared__ double s2[S2MAX];
-  __shared__ int64_t lenj[16];
-  const bool select_only = o.pick == nullptr;
-  if (!select_only && *o.pick == -2) return;
+__device__ int64_t relax_db(int64_t P, int nsrc, const double* const* maps, const int32_t* const* sorted, double tol,
+                            uint32_t* bits, int32_t* ev, SelScratch& sh) {
+  __shared__ int64_t lenj[IQ_MAX_AUX + 2];
   const int64_t nwd = (P + 31) / 32;
-  const double* D = maps[0];
+  const double* Dp = maps[0];
   // enabled patterns = finite entries = prefix of the ascending order (+inf sorts last)
   if (threadIdx.x == 0) {
     int64_t lo = 0, hi = P;
-    while (lo < hi) { int64_t mid = (lo + hi) >> 1; if (!__builtin_isinf(D[sorted[0][mid]])) lo = mid + 1; else hi = mid; }
+    while (lo < hi) { int64_t mid = (lo + hi) >> 1; if (!__builtin_isinf(Dp[sorted[0][mid]])) lo = mid + 1; else hi = mid; }
     sh.n = lo;
   }
   __syncthreads();
   const int64_t nen = sh.n;
   __syncthreads();
-  if (nen == 0) {  // relaxation.jl would never terminate
-    if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_NOCAND); if (!select_only) *o.pick = -2; }
-    return;
-  }
-  const bool allzero = D[sorted[0][nen - 1]] == 0.0;  // largest finite value is 0
+  if (nen == 0) return -1;  // relaxation.jl would never terminate
+  const bool allzero = Dp[sorted[0][nen - 1]] == 0.0;  // largest finite value is 0 (relaxation.jl:11)
   const int64_t dbsize = allzero ? nen : (int64_t)ceil(tol * (double)nen);
   for (int64_t i = threadIdx.x; i < nwd * nsrc; i += FB) bits[i] = 0u;
-  if (threadIdx.x < 16) lenj[threadIdx.x] = 0;
+  if (threadIdx.x < IQ_MAX_AUX + 2) lenj[threadIdx.x] = 0;
   __syncthreads();
   for (int64_t i = threadIdx.x; i < dbsize; i += FB) {
     int32_t c = sorted[0][i];
@@ -397,7 +440,7 @@ k_relax_finalize(int64_t P, int nsrc, const double* const* __restrict__ maps,
       int64_t lc = 0;
       for (int64_t w = threadIdx.x; w < nwd; w += FB) {
         uint32_t x = bits[w];
-        for (int q = 1; q <= j; q++) x &= bits[q * nwd + w];
+        for (int k = 1; k <= j; k++) x &= bits[k * nwd + w];
         lc += __builtin_popcount(x);
       }
       cnt = blk_sum_i64(lc, sh);
@@ -412,14 +455,14 @@ k_relax_finalize(int64_t P, int nsrc, const double* const* __restrict__ maps,
   int64_t lc = 0;
   for (int64_t w = w0; w < w1; w++) {
     uint32_t x = bits[w];
-    for (int q = 1; q < nsrc; q++) x &= bits[q * nwd + w];
+    for (int k = 1; k < nsrc; k++) x &= bits[k * nwd + w];
     lc += __builtin_popcount(x);
   }
   int64_t nev;
   int64_t off = blk_excl_scan(lc, &nev, sh);
   for (int64_t w = w0; w < w1; w++) {
     uint32_t x = bits[w];
-    for (int q = 1; q < nsrc; q++) x &= bits[q * nwd + w];
+    for (int k = 1; k < nsrc; k++) x &= bits[k * nwd + w];
     while (x) {
       int b = __builtin_ctz(x);
       x &= x - 1;
@@ -427,50 +470,93 @@ k_relax_finalize(int64_t P, int nsrc, const double* const* __restrict__ maps,
     }
   }
   __syncthreads();
-  if (select_only) {
-    if (threadIdx.x == 0) *o.nband = nev;
+  return nev;
+}
+
+// Multi-source finalize: relaxation + tau model + draw (iqsim.jl:230-243).
+// Source order: hard tile -> [hard, overlap, soft...], else [overlap, soft...].
+extern "C" __global__ void __launch_bounds__(FB)
+k_relax_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  __shared__ SelScratch sh;
+  __shared__ double sv[IQ_SORT_MAX];
+  __shared__ int32_t dp[IQ_SORT_MAX];
+  __shared__ double s2[S2MAX];
+  __shared__ const double* mp[IQ_MAX_AUX + 2];
+  __shared__ const int32_t* sp[IQ_MAX_AUX + 2];
+  const IqQuery q = qs[blockIdx.x];
+  if (!q.multi) return;
+  const IqGeo& g = D->g;
+  const IqStep& st = D->steps[q.step];
+  const int64_t slot = q.slot;
+  SelOut o{D->pick + slot, D->log + (int64_t)q.real * D->npath + q.step, D->err, D->nband + slot};
+  int nsrc = 0;
+  if (threadIdx.x == 0) {
+    int k = 0;
+    if (st.hard) { mp[k] = D->maps + (slot * D->nmap + 1) * g.P; sp[k] = D->sorted + (slot * D->nmap + 1) * g.P; k++; }
+    mp[k] = D->maps + (slot * D->nmap + 0) * g.P; sp[k] = D->sorted + (slot * D->nmap + 0) * g.P; k++;
+    for (int a = 0; a < D->naux; a++) {
+      mp[k] = D->maps + (slot * D->nmap + 2 + a) * g.P;
+      sp[k] = D->sorted + (slot * D->nmap + 2 + a) * g.P;
+      k++;
+    }
+    sh.flag = k;
+  }
+  __syncthreads();
+  nsrc = sh.flag;
+  __syncthreads();
+  int32_t* ev = D->ev + slot * g.P;
+  const int64_t nev = relax_db(g.P, nsrc, mp, sp, D->tol, D->bits + slot * D->nmap * D->nw32, ev, sh);
+  if (nev < 0) {
+    if (threadIdx.x == 0) { set_err(D->err, IQ_KERR_NOCAND); *o.pick = -2; }
     return;
   }
-  if (nev == 1) {
-    if (threadIdx.x == 0) p[0] = 1.0;
-    __syncthreads();
-  } else {
-    for (int64_t i = threadIdx.x; i < nev; i += FB) p[i] = 1.0;
-    __syncthreads();
-    for (int j = 0; j < nsrc; j++) {
-      for (int64_t i = threadIdx.x; i < nev; i += FB) vals[i] = maps[j][ev[i]];
-      __syncthreads();
-      if (!blk_dense_rank(vals, nev, rank, sv, dp, sh)) {
-        if (threadIdx.x == 0) { set_err(o.err, IQ_KERR_SORTMAX); *o.pick = -2; }
-        return;
-      }
-      tau_accumulate(nev, rank, p, sh);
-    }
-    tau_finish(nev, p);
+  double* vals = D->vals + slot * g.P;
+  double* p = D->sd + slot * IQ_SORT_MAX;
+  int32_t* si = D->si + slot * IQ_SORT_MAX;
+  bool allconst;
+  auto srcv = [&](int j, int64_t i) { return mp[j][ev[i]]; };
+  if (!tau_weights(nev, nsrc, srcv, vals, si, p, &allconst, sv, dp, sh)) {
+    if (threadIdx.x == 0) { set_err(D->err, IQ_KERR_SORTMAX); *o.pick = -2; }
+    return;
   }
-  publish_pick(ev, nev, p, u, forced, s1, s2, o, sh);
+  double* s1 = D->s1 + slot * D->s1_stride;
+  if (nev == 1) publish_pick(ev, 1, nullptr, 1.0, q.u, q.forced, s1, s2, o, sh);
+  else if (allconst) publish_pick(ev, nev, nullptr, tau_constant_weight(nev), q.u, q.forced, s1, s2, o, sh);
+  else publish_pick(ev, nev, p, 0.0, q.u, q.forced, s1, s2, o, sh);
 }
 
 // ---------------------------------------------------------------------------
-// Standalone entry points (iq_band_pick / iq_taumodel / iq_sample)
+// Standalone entry points (iq_band_pick / iq_relaxation / iq_taumodel / iq_sample)
 // ---------------------------------------------------------------------------
 extern "C" __global__ void __launch_bounds__(FB)
-k_band_pick_host(const double* __restrict__ D, int64_t n, double tol, int32_t* __restrict__ out,
+k_band_pick_host(const double* __restrict__ Dm, int64_t n, double tol, int32_t* __restrict__ out,
                  int64_t* __restrict__ count) {
   __shared__ SelScratch sh;
   double lm = __builtin_inf();
-  for (int64_t i = threadIdx.x; i < n; i += FB) lm = fmin(lm, D[i]);
+  for (int64_t i = threadIdx.x; i < n; i += FB) lm = fmin(lm, Dm[i]);
   const double m = blk_min_f64(lm, sh);
   const double thr = (1.0 + tol) * m;
   const int64_t per = (n + FB - 1) / FB;
   const int64_t i0 = threadIdx.x * per, i1 = (i0 + per < n) ? i0 + per : n;
   int64_t c = 0;
-  for (int64_t i = i0; i < i1; i++) c += D[i] <= thr;
+  for (int64_t i = i0; i < i1; i++) c += Dm[i] <= thr;
   int64_t tot;
   int64_t off = blk_excl_scan(c, &tot, sh);
   for (int64_t i = i0; i < i1; i++)
-    if (D[i] <= thr) out[off++] = (int32_t)i;
+    if (Dm[i] <= thr) out[off++] = (int32_t)i;
   if (threadIdx.x == 0) *count = tot;
+}
+
+extern "C" __global__ void __launch_bounds__(FB)
+k_relax_host(int64_t P, int nsrc, const double* const* __restrict__ maps, const int32_t* const* __restrict__ sorted,
+             double tol, uint32_t* __restrict__ bits, int32_t* __restrict__ ev, int64_t* __restrict__ count,
+             int32_t* __restrict__ err) {
+  __shared__ SelScratch sh;
+  const int64_t n = relax_db(P, nsrc, maps, sorted, tol, bits, ev, sh);
+  if (threadIdx.x == 0) {
+    if (n < 0) set_err(err, IQ_KERR_NOCAND);
+    *count = n < 0 ? 0 : n;
+  }
 }
 
 extern "C" __global__ void __launch_bounds__(FB)
@@ -480,26 +566,26 @@ k_taumodel_host(const int32_t* __restrict__ ev, int64_t nev, int nsrc, const dou
   __shared__ SelScratch sh;
   __shared__ double sv[IQ_SORT_MAX];
   __shared__ int32_t dp[IQ_SORT_MAX];
-  if (nev == 1) { if (threadIdx.x == 0) p[0] = 1.0; return; }
+  if (nev == 1) { if (threadIdx.x == 0) p[0] = 1.0; return; }  // taumodel.jl:9
   for (int64_t i = threadIdx.x; i < nev; i += FB) p[i] = 1.0;
   __syncthreads();
   for (int j = 0; j < nsrc; j++) {
     for (int64_t i = threadIdx.x; i < nev; i += FB) vals[i] = maps[j][ev[i]];
     __syncthreads();
-    if (!blk_dense_rank(vals, nev, rank, sv, dp, sh)) {
+    bool c;
+    if (!blk_dense_rank(vals, nev, rank, &c, sv, dp, sh)) {
       if (threadIdx.x == 0) set_err(err, IQ_KERR_SORTMAX);
       return;
     }
-    tau_accumulate(nev, rank, p, sh);
+    tau_accumulate(nev, rank, c, p, sh);
   }
   tau_finish(nev, p);
 }
 
 extern "C" __global__ void __launch_bounds__(FB)
-k_sample_host(const double* __restrict__ p, int64_t n, double u, double* __restrict__ s1,
-              int64_t* __restrict__ out) {
+k_sample_host(const double* __restrict__ p, int64_t n, double u, double* __restrict__ s1, int64_t* __restrict__ out) {
   __shared__ SelScratch sh;
   __shared__ double s2[S2MAX];
-  int64_t i = blk_sample(p, n, u, s1, s2, sh);
+  int64_t i = blk_sample(p, 0.0, n, u, s1, s2, sh);
   if (threadIdx.x == 0) *out = i;
 }

@@ -323,7 +323,7 @@ def pad_symmetric(aux, padsize):
 # ---------------------------------------------------------------------------
 def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None, tol=0.1,
           path="raster", nreal=1, debug=False, uniforms=None, forced=None, seed=0,
-          gr_freq=GR_FREQ, return_trace=False, max_steps=None):
+          gr_freq=GR_FREQ, return_trace=False, max_steps=None, simpath=None):
     T = np.asarray(trainimg)
     N = T.ndim
     if simsize is None:
@@ -340,7 +340,9 @@ def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None,
     disabled = finddisabled(T, geo)
     skipped, datainds = findskipped(hard, geo)
     rng = np.random.default_rng(seed)
-    simpath = genpath(rng, geo["ntiles"], path, list(datainds))
+    if simpath is None:
+        simpath = genpath(rng, geo["ntiles"], path, list(datainds))
+    simpath = list(simpath)
     if max_steps is not None:  # bounded sample: the first tile steps only
         simpath = simpath[:max_steps]
     npath = len(simpath)

@@ -26,7 +26,9 @@ def _smooth(rng, shape, width=5, dtype=np.float32):
 def _compare(T, tile, sim, seed=0, **kw):
     (reals, cuts, voxs), tr = iqhip.iqsim(T, tile, sim, rng=np.random.default_rng(seed), debug=True,
                                           return_trace=True, **kw)
-    (rr, rc, rv), rtr = iqref.iqsim(T, tile, sim, uniforms=tr["uniforms"], debug=True, return_trace=True, **kw)
+    # the oracle replays the product's path; genpath itself is checked in test_host.py
+    (rr, rc, rv), rtr = iqref.iqsim(T, tile, sim, uniforms=tr["uniforms"], debug=True, return_trace=True,
+                                    simpath=tr["path"], **kw)
     assert list(tr["path"]) == list(rtr["path"])
     assert np.array_equal(tr["chosen"], rtr["chosen"])
     for a, b in zip(reals, rr):
@@ -89,3 +91,28 @@ def test_sim_masked_ti_and_grid():
     shape = {(i, 4, k): float("nan") for i in range(20) for k in range(20)}
     print("masked ti + grid", flush=True)
     _compare(T, (10, 10, 10), (20, 20, 20), hard=shape)
+
+
+@pytest.mark.parametrize("path", ["random", "dilation"])
+def test_sim_nonraster_paths(path):
+    # general wave construction: tiles run as soon as their earlier-path neighbours are pasted
+    rng = np.random.default_rng(6)
+    T = _smooth(rng, (30, 30, 10), 5, np.float32)
+    print("path", path, flush=True)
+    _compare(T, (10, 10, 5), (40, 40, 15), overlap=(0.3, 0.3, 0.4), path=path, seed=9)
+
+
+def test_sim_data_first_path_2d():
+    rng = np.random.default_rng(7)
+    T = _smooth(rng, (50, 50), 5, np.float64)
+    hard = {(3, 4): 0.5, (40, 7): -0.25, (20, 33): 1.0}
+    print("data-first", flush=True)
+    _compare(T, (12, 12), (55, 55), hard=hard, seed=4)
+
+
+def test_sim_large_overlap_runs_sequentially():
+    # overlap above one half: tiles two apart overlap, every tile is its own wave
+    rng = np.random.default_rng(8)
+    T = _smooth(rng, (30, 30), 5, np.float32)
+    print("overlap 0.6", flush=True)
+    _compare(T, (10, 10), (30, 30), overlap=(0.6, 0.6))
