@@ -552,7 +552,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
     for (int q = 0; q < st.nslab; q++) {
       const IqSlab& sl = st.slab[q];
       int64_t n = (int64_t)sl.sz[0] * sl.sz[1] * sl.sz[2] / sl.sz[sl.dim] * std::max(0, sl.sz[sl.dim] - 2);
-      if (n <= IQ_PR_NLDS) pl->cut_lds_max = std::max(pl->cut_lds_max, (size_t)n * CUT_NODE_BYTES);
+      if (n <= IQ_PR_NLDS) pl->cut_lds_max = std::max(pl->cut_lds_max, (size_t)n * CUT_FAST_NODE_BYTES);
       maxn = std::max(maxn, n);
     }
   const int64_t cutper = maxn > IQ_PR_NLDS ? maxn * CUT_NODE_BYTES : 0;
@@ -1043,7 +1043,7 @@ int iq_graphcut(iq_ctx* ctx, const void* A, const void* B, int dtype, const int6
     char* scratch = nullptr;
     size_t lds = 0;
     if (n > IQ_PR_NLDS) scratch = m.alloc<char>((size_t)n * CUT_NODE_BYTES);
-    else lds = (size_t)n * CUT_NODE_BYTES;
+    else lds = (size_t)n * CUT_FAST_NODE_BYTES;
     if (lds > 64 * 1024) {
       HC(hipFuncSetAttribute((const void*)k_graphcut_host<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       HC(hipFuncSetAttribute((const void*)k_graphcut_host<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -272,6 +272,254 @@ __device__ void cut_slab(const SlabView<T>& A, const SlabView<T>& B, const int s
   if (threadIdx.x == 0 && rounds) atomicAdd((unsigned long long*)rounds, (unsigned long long)total);
 }
 
+// Register-resident form of cut_slab for slabs of at most NPT * CB interior
+// nodes: same schedule, same arithmetic, bit-identical result.  A work-item
+// owns nodes u = tid + k * CB; their coordinates, excess, sink residual and
+// flags stay in registers (only the owner touches them); heights (double
+// buffered) and the edge residual / push records live in LDS.  The gather of
+// round k runs at the start of round k+1 (push records double buffered), so a
+// round costs two barriers.
+struct CutLds {
+  double *rf, *rb, *pr0, *pr1;
+  int32_t *h0, *h1;
+};
+
+#define CUT_FAST_NODE_BYTES (8 * 12 + 4 * 2)
+
+template <typename T, int NPT>
+__device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
+                              uint8_t* out, char* mem, int32_t* err, int64_t* rounds) {
+  const int L = s[dim];
+  const int nvox = s[0] * s[1] * s[2];
+  for (int p = threadIdx.x; p < nvox; p += CB) {
+    int c[3] = {p % s[0], (p / s[0]) % s[1], p / (s[0] * s[1])};
+    uint8_t m = (c[dim] == L - 1) ? 0 : 1;
+    out[p] = m ^ side;
+  }
+  if (L <= 2) return;
+  int si[3] = {s[0], s[1], s[2]};
+  si[dim] = L - 2;
+  const int n = si[0] * si[1] * si[2];
+  const int HMAX = n + 1;
+  const int ist[3] = {1, si[0], si[0] * si[1]};
+  CutLds S;
+  S.rf = (double*)mem;
+  S.rb = S.rf + 3 * n;
+  S.pr0 = S.rb + 3 * n;
+  S.pr1 = S.pr0 + 3 * n;
+  S.h0 = (int32_t*)(S.pr1 + 3 * n);
+  S.h1 = S.h0 + n;
+  // per owned node: coordinates, flags (bit d: plus edge d, bit 3: sink edge, bits 4+d: minus edge d)
+  int fl[NPT];
+  int uu[NPT];
+  double e[NPT], rt[NPT];
+#pragma unroll
+  for (int k = 0; k < NPT; k++) {
+    const int u = threadIdx.x + k * CB;
+    uu[k] = u;
+    fl[k] = 0;
+    e[k] = 0.0;
+    rt[k] = 0.0;
+    if (u < n) {
+      int ic[3] = {u % si[0], (u / si[0]) % si[1], u / (si[0] * si[1])};
+      int pc[3] = {ic[0], ic[1], ic[2]};
+      pc[dim] += 1;
+      int f = 0;
+      for (int d = 0; d < 3; d++) {
+        double c = 0.0;
+        if (ic[d] + 1 < si[d]) { f |= 1 << d; c = slab_cap<T>(A, B, s, pc[0], pc[1], pc[2], d); }
+        if (ic[d] > 0) f |= 16 << d;
+        S.rf[3 * u + d] = c;
+        S.rb[3 * u + d] = c;
+        S.pr0[3 * u + d] = 0.0;
+        S.pr1[3 * u + d] = 0.0;
+      }
+      if (pc[dim] == 1) {
+        int q[3] = {pc[0], pc[1], pc[2]};
+        q[dim] -= 1;
+        e[k] = e[k] + slab_cap<T>(A, B, s, q[0], q[1], q[2], dim);
+      }
+      if (pc[dim] == L - 2) { f |= 8; rt[k] = slab_cap<T>(A, B, s, pc[0], pc[1], pc[2], dim); }
+      fl[k] = f;
+    }
+  }
+  __syncthreads();
+
+  int64_t total = 0;
+  int32_t* h = S.h0;   // committed heights
+  int32_t* hn = S.h1;  // next heights
+  for (;;) {
+    // global relabel (exact residual distance to the sink)
+#pragma unroll
+    for (int k = 0; k < NPT; k++)
+      if (uu[k] < n) h[uu[k]] = ((fl[k] & 8) && rt[k] > 0.0) ? 1 : HMAX;
+    __syncthreads();
+    for (int lev = 1;; lev++) {
+      int ch = 0;
+#pragma unroll
+      for (int k = 0; k < NPT; k++) {
+        const int u = uu[k];
+        if (u >= n || h[u] != HMAX) continue;
+        bool hit = false;
+        for (int d = 0; d < 3 && !hit; d++) {
+          if (fl[k] & (16 << d)) {
+            int v = u - ist[d];
+            if (S.rb[3 * v + d] > 0.0 && h[v] == lev) hit = true;
+          }
+          if (!hit && (fl[k] & (1 << d))) {
+            int v = u + ist[d];
+            if (S.rf[3 * u + d] > 0.0 && h[v] == lev) hit = true;
+          }
+        }
+        if (hit) { h[u] = lev + 1; ch = 1; }
+      }
+      if (!__syncthreads_or(ch)) break;
+    }
+    int act = 0;
+#pragma unroll
+    for (int k = 0; k < NPT; k++)
+      if (uu[k] < n) act |= (e[k] > 0.0 && h[uu[k]] < HMAX);
+    if (!__syncthreads_or(act)) break;
+    if (total > IQ_PR_LIMIT) {
+      if (threadIdx.x == 0) atomicCAS(err, 0, IQ_KERR_ITER);
+      break;
+    }
+    double* prc = S.pr0;  // records written by this round's push
+    double* pro = S.pr1;  // records of the previous round (gathered, then cleared)
+    bool stop = false;
+    for (int it = 0; it <= IQ_GR_FREQ; it++) {
+      // phase A: gather the previous round's pushes, then relabel
+      int a2 = 0;
+#pragma unroll
+      for (int k = 0; k < NPT; k++) {
+        const int u = uu[k];
+        if (u >= n) continue;
+        if (it > 0) {
+          double ex = e[k];
+          for (int d = 0; d < 3; d++) {
+            if (fl[k] & (16 << d)) {
+              double q = pro[3 * (u - ist[d]) + d];
+              if (q > 0.0) ex += q;
+            }
+            if (fl[k] & (1 << d)) {
+              double q = pro[3 * u + d];
+              if (q < 0.0) ex += -q;
+            }
+          }
+          e[k] = ex;
+        }
+        const int hu = h[u];
+        int nh = hu;
+        const bool active = e[k] > 0.0 && hu < HMAX;
+        a2 |= active;
+        if (active && it < IQ_GR_FREQ) {
+          bool adm = false;
+          int mh = HMAX;
+          if ((fl[k] & 8) && rt[k] > 0.0) { if (hu == 1) adm = true; mh = 0; }
+          for (int d = 0; d < 3; d++) {
+            if (fl[k] & (16 << d)) {
+              int v = u - ist[d];
+              if (S.rb[3 * v + d] > 0.0) { int hv = h[v]; if (hu == hv + 1) adm = true; if (hv < mh) mh = hv; }
+            }
+            if (fl[k] & (1 << d)) {
+              int v = u + ist[d];
+              if (S.rf[3 * u + d] > 0.0) { int hv = h[v]; if (hu == hv + 1) adm = true; if (hv < mh) mh = hv; }
+            }
+          }
+          if (!adm) nh = (mh + 1 < HMAX) ? mh + 1 : HMAX;
+        }
+        hn[u] = nh;
+      }
+      if (it > 0) {
+        total++;
+        if (!__syncthreads_or(a2)) { stop = true; break; }
+      } else {
+        __syncthreads();
+      }
+      if (it == IQ_GR_FREQ) break;
+      { int32_t* t = h; h = hn; hn = t; }
+      // phase B: push with the new heights; clear the gathered records
+#pragma unroll
+      for (int k = 0; k < NPT; k++) {
+        const int u = uu[k];
+        if (u >= n) continue;
+        pro[3 * u] = 0.0; pro[3 * u + 1] = 0.0; pro[3 * u + 2] = 0.0;
+        double ex = e[k];
+        const int hu = h[u];
+        if (!(ex > 0.0 && hu < HMAX)) continue;
+        if ((fl[k] & 8) && hu == 1 && rt[k] > 0.0) {
+          double r = rt[k];
+          double dl = ex < r ? ex : r;
+          ex -= dl;
+          rt[k] = r - dl;
+        }
+        for (int d = 0; d < 3 && ex > 0.0; d++) {
+          if (fl[k] & (16 << d)) {
+            int v = u - ist[d];
+            if (hu == h[v] + 1) {
+              double r = S.rb[3 * v + d];
+              if (r > 0.0) {
+                double dl = ex < r ? ex : r;
+                ex -= dl;
+                S.rb[3 * v + d] = r - dl;
+                S.rf[3 * v + d] += dl;
+                prc[3 * v + d] = -dl;
+              }
+            }
+          }
+          if (ex > 0.0 && (fl[k] & (1 << d))) {
+            int v = u + ist[d];
+            if (hu == h[v] + 1) {
+              double r = S.rf[3 * u + d];
+              if (r > 0.0) {
+                double dl = ex < r ? ex : r;
+                ex -= dl;
+                S.rf[3 * u + d] = r - dl;
+                S.rb[3 * u + d] += dl;
+                prc[3 * u + d] = dl;
+              }
+            }
+          }
+        }
+        e[k] = ex;
+      }
+      __syncthreads();
+      { double* t = prc; prc = pro; pro = t; }
+    }
+    // records of the last gathered round are cleared before the next cycle
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+      const int u = uu[k];
+      if (u < n) { pro[3 * u] = 0.0; pro[3 * u + 1] = 0.0; pro[3 * u + 2] = 0.0;
+                   prc[3 * u] = 0.0; prc[3 * u + 1] = 0.0; prc[3 * u + 2] = 0.0; }
+    }
+    __syncthreads();
+    (void)stop;
+  }
+#pragma unroll
+  for (int k = 0; k < NPT; k++) {
+    const int u = uu[k];
+    if (u >= n) continue;
+    int ic[3] = {u % si[0], (u / si[0]) % si[1], u / (si[0] * si[1])};
+    int pc[3] = {ic[0], ic[1], ic[2]};
+    pc[dim] += 1;
+    uint8_t m = (h[u] < HMAX) ? 0 : 1;
+    out[pc[0] + s[0] * (pc[1] + s[1] * pc[2])] = m ^ side;
+  }
+  if (threadIdx.x == 0 && rounds) atomicAdd((unsigned long long*)rounds, (unsigned long long)total);
+}
+
+template <typename T>
+__device__ void cut_dispatch(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
+                             uint8_t* out, char* lds, char* gmem, int32_t* err, int64_t* rounds) {
+  const int n = s[0] * s[1] * s[2] / s[dim] * (s[dim] > 2 ? s[dim] - 2 : 0);
+  if (n <= CB) cut_slab_fast<T, 1>(A, B, s, dim, side, out, lds, err, rounds);
+  else if (n <= 2 * CB) cut_slab_fast<T, 2>(A, B, s, dim, side, out, lds, err, rounds);
+  else if (n <= 4 * CB) cut_slab_fast<T, 4>(A, B, s, dim, side, out, lds, err, rounds);
+  else if (n <= IQ_PR_NLDS) cut_slab_fast<T, 6>(A, B, s, dim, side, out, lds, err, rounds);
+  else cut_slab<T>(A, B, s, dim, side, out, gmem, err, rounds);
+}
+
 // Batched cut: grid (slab, query).  A = the realization's grid over the
 // tile, B = the picked training-image window (iqsim.jl:244-273).
 template <typename T>
@@ -296,10 +544,8 @@ k_graphcut(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   B.base = (const T*)D->ti + iq_lin3(g.ti, rc[0] + sl.lo[0], rc[1] + sl.lo[1], rc[2] + sl.lo[2]);
   B.st[0] = 1; B.st[1] = g.ti[0]; B.st[2] = (int64_t)g.ti[0] * g.ti[1];
   uint8_t* out = D->keep + (slot * IQ_MAX_SLAB + blockIdx.x) * g.tilevox;
-  const int n = s[0] * s[1] * s[2] / s[sl.dim] * (s[sl.dim] > 2 ? s[sl.dim] - 2 : 0);
-  char* mem = n <= IQ_PR_NLDS ? cut_lds
-                              : D->cutscratch + (slot * IQ_MAX_SLAB + blockIdx.x) * D->cutscratch_per;
-  cut_slab<T>(A, B, s, sl.dim, (uint8_t)sl.side, out, mem, D->err, D->rounds);
+  char* gmem = D->cutscratch ? D->cutscratch + (slot * IQ_MAX_SLAB + blockIdx.x) * D->cutscratch_per : nullptr;
+  cut_dispatch<T>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, gmem, D->err, D->rounds);
 }
 
 template __global__ void k_graphcut<float>(const IqDev*, const IqQuery*);
@@ -315,8 +561,7 @@ k_graphcut_host(const T* __restrict__ a, const T* __restrict__ b, int s0, int s1
   SlabView<T> A, B;
   A.base = a; A.st[0] = 1; A.st[1] = s0; A.st[2] = (int64_t)s0 * s1;
   B.base = b; B.st[0] = 1; B.st[1] = s0; B.st[2] = (int64_t)s0 * s1;
-  const int n = s0 * s1 * s2 / s[dim] * (s[dim] > 2 ? s[dim] - 2 : 0);
-  cut_slab<T>(A, B, s, dim, (uint8_t)0, out, n <= IQ_PR_NLDS ? cut_lds : gmem, err, nullptr);
+  cut_dispatch<T>(A, B, s, dim, (uint8_t)0, out, cut_lds, gmem, err, nullptr);
 }
 
 template __global__ void k_graphcut_host<float>(const float*, const float*, int, int, int, int, uint8_t*, char*, int32_t*);
