@@ -6,6 +6,7 @@
 #include "iq_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -169,21 +170,87 @@ std::vector<IqBox> decompose(const std::vector<uint8_t>& bits, const int t[3], c
   return cells;
 }
 
-// FP32 scan configurations: R, LX, LY, LZ
+// FP32 scan configurations.  kind 1: k_scan_f32 (R x-candidates per
+// work-item); kind 2: k_scan2_f32 (R x RY block, lanes over x and z, four
+// wavefronts over y).  A work-group covers (R*LX) x (RY*LY) x LZ candidates.
 struct ScanCfg {
-  int R, LX, LY, LZ;
+  int kind, R, RY, LX, LY, LZ;
 };
-const ScanCfg kScanCfg[] = {{4, 8, 8, 4}, {4, 16, 4, 4}, {2, 16, 8, 2}, {4, 8, 16, 2}};
-const int kNumScanCfg = 4;
+const ScanCfg kScanCfg[] = {{1, 4, 1, 8, 8, 4}, {1, 4, 1, 16, 4, 4}, {1, 2, 1, 16, 8, 2}, {1, 4, 1, 8, 16, 2},
+                            {2, 4, 4, 8, 4, 8}, {2, 4, 4, 16, 4, 4}};
+const int kNumScanCfg = 6;
+const size_t kScanLdsMax = 100 * 1024;
 
-size_t scan_lds_bytes(const ScanCfg& c, const std::vector<IqBox>& boxes) {
+size_t scan_lds_bytes(const ScanCfg& c, const std::vector<IqBox>& boxes, bool perz) {
   size_t m = 0;
   for (const IqBox& b : boxes) {
-    int S = c.LX + b.wxp / c.R;
-    size_t n = (size_t)c.R * S * (c.LY + b.wy - 1) * (c.LZ + b.wz - 1);
+    int wxp = (b.wx + 3) / 4 * 4;
+    int S = c.LX + wxp / c.R;
+    size_t n;
+    if (c.kind == 1) {
+      n = (size_t)c.R * S * (c.LY + b.wy - 1) * (c.LZ + b.wz - 1);
+    } else {
+      size_t rowst = (size_t)c.R * S;
+      size_t ryt = (size_t)c.LY * c.RY + b.wy - 1;
+      size_t plane = ((ryt * rowst + 55) / 64) * 64 + 8;
+      n = plane * (perz ? c.LZ : c.LZ + b.wz - 1);
+    }
     m = std::max(m, n * sizeof(float));
   }
   return m;
+}
+
+bool bf16_exact(const float* v, int64_t n) {
+  for (int64_t i = 0; i < n; i++) {
+    uint32_t u;
+    std::memcpy(&u, &v[i], 4);
+    if ((u & 0xffffu) != 0u) return false;
+  }
+  return true;
+}
+
+// K table of the MFMA scan over the union of the overlap masks: per box,
+// 8-voxel chunks along x or y (whichever pads less), an even number per box.
+bool build_ktab(const std::vector<uint8_t>& ubits, const int t3[3], const std::vector<int> brk[3], IqKTab& kt,
+                std::vector<IqChunk>& chunks) {
+  std::vector<IqBox> boxes = decompose(ubits, t3, brk);
+  if ((int)boxes.size() > IQ_MAX_KBOX) return false;
+  kt.nbox = 0;
+  int step = 0;
+  for (const IqBox& b : boxes) {
+    auto r8 = [](int w) { return (w + 7) / 8 * 8; };
+    const double ex = (double)b.wx / r8(b.wx), ey = (double)b.wy / r8(b.wy);
+    const int dir = (ey > ex) ? 1 : 0;
+    IqKBox kb{};
+    kb.x0 = b.x0; kb.y0 = b.y0; kb.z0 = b.z0; kb.wz = b.wz;
+    kb.wxr = dir == 0 ? r8(b.wx) : b.wx;
+    kb.wyr = dir == 1 ? r8(b.wy) : b.wy;
+    const size_t c0 = chunks.size();
+    for (int z = 0; z < b.wz; z++) {
+      if (dir == 0) {
+        for (int y = 0; y < b.wy; y++)
+          for (int x = 0; x < b.wx; x += 8)
+            chunks.push_back(IqChunk{(int16_t)(b.x0 + x), (int16_t)(b.y0 + y), (int16_t)(b.z0 + z), 0,
+                                     (int8_t)std::min(8, b.wx - x)});
+      } else {
+        for (int x = 0; x < b.wx; x++)
+          for (int y = 0; y < b.wy; y += 8)
+            chunks.push_back(IqChunk{(int16_t)(b.x0 + x), (int16_t)(b.y0 + y), (int16_t)(b.z0 + z), 1,
+                                     (int8_t)std::min(8, b.wy - y)});
+      }
+    }
+    if ((chunks.size() - c0) % 2) {  // zero chunk: same base, no voxels
+      IqChunk z = chunks.back();
+      z.n = 0;
+      chunks.push_back(z);
+    }
+    kb.s0 = step;
+    step += (int)(chunks.size() - c0) / 2;
+    kb.s1 = step;
+    kt.box[kt.nbox++] = kb;
+  }
+  kt.nstep = step;
+  return true;
 }
 
 bool fits_f32(const double* v, int64_t n) {
@@ -232,6 +299,12 @@ struct iq_plan {
   std::vector<std::vector<int64_t>> waves;  // path positions per wave (dependency order)
   int scan_cfg = -1;
   size_t scan_lds = 0;
+  bool scan_perz = false;
+  bool mfma = false;        // multi-query MFMA scan
+  bool mfma_split = true;   // FP32 data: three bf16 products
+  size_t mfma_lds = 0;
+  double mfma_gerr = 0.0;
+  int64_t mfma_ntile_max = 0;
   int nbz = 1;
   int64_t nslots = 0;
   DMem mem;
@@ -499,26 +572,76 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
   H.masks = pl->mem.upload(pl->masks.data(), pl->masks.size(), s);
   H.a2maps = (const double* const*)pl->mem.upload(a2ptr.data(), a2ptr.size(), s);
   // one scan configuration for the plan: every mask must fit its LDS, the
-  // most used mask decides the tiling
+  // most used mask decides the tiling; 3-D grids prefer the 2-D register block
   if (pl->fast && !hmasks.empty()) {
     size_t best_mi = std::max_element(mask_uses.begin(), mask_uses.end()) - mask_uses.begin();
     double bestcost = 1e300;
     for (int c = 0; c < kNumScanCfg; c++) {
-      size_t lmax = 0;
-      for (const HostMask& hm : hmasks) lmax = std::max(lmax, scan_lds_bytes(kScanCfg[c], hm.boxes));
-      if (lmax > 64 * 1024) continue;
-      size_t l = 0;
       const ScanCfg& sc = kScanCfg[c];
-      double padded = (double)nblk(g.dist[0], sc.R * sc.LX) * sc.R * sc.LX * nblk(g.dist[1], sc.LY) * sc.LY *
-                      nblk(g.dist[2], sc.LZ) * sc.LZ;
-      l = scan_lds_bytes(sc, hmasks[best_mi].boxes);
-      double cost = padded * (1.0 + 0.02 * (double)l / 16384.0);
-      if (cost < bestcost) { bestcost = cost; pl->scan_cfg = c; pl->scan_lds = lmax; }
+      if (sc.kind == 2 && g.dist[2] < sc.LZ / 2) continue;
+      bool perz = false;
+      size_t lmax = 0;
+      for (const HostMask& hm : hmasks) lmax = std::max(lmax, scan_lds_bytes(sc, hm.boxes, false));
+      if (sc.kind == 2 && lmax > 80 * 1024) {
+        perz = true;
+        lmax = 0;
+        for (const HostMask& hm : hmasks) lmax = std::max(lmax, scan_lds_bytes(sc, hm.boxes, true));
+      }
+      if (lmax > (sc.kind == 1 ? (size_t)64 * 1024 : kScanLdsMax)) continue;
+      double padded = (double)nblk(g.dist[0], sc.R * sc.LX) * sc.R * sc.LX * nblk(g.dist[1], sc.LY * sc.RY) * sc.LY *
+                      sc.RY * nblk(g.dist[2], sc.LZ) * sc.LZ;
+      size_t l = scan_lds_bytes(sc, hmasks[best_mi].boxes, perz);
+      double cost = padded * (1.0 + 0.02 * (double)l / 16384.0) * (sc.kind == 2 ? 0.4 : 1.0);
+      if (cost < bestcost) { bestcost = cost; pl->scan_cfg = c; pl->scan_lds = lmax; pl->scan_perz = perz; }
     }
     if (pl->scan_cfg < 0) pl->fast = false;
     else pl->nbz = (int)nblk(g.dist[2], kScanCfg[pl->scan_cfg].LZ);
+    if (pl->fast && pl->scan_lds > 64 * 1024) {
+      HC(hipFuncSetAttribute((const void*)k_scan2_f32<4, 4, 8, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kScanLdsMax));
+      HC(hipFuncSetAttribute((const void*)k_scan2_f32<4, 4, 16, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kScanLdsMax));
+    }
   }
   if (!pl->fast) H.ti32 = nullptr;
+  // MFMA scan over the union of the masks
+  {
+    const char* env = std::getenv("IQHIP_SCAN");
+    const bool want = !(env && std::string(env) == "valu");
+    std::vector<uint8_t> ubits((size_t)g.tilevox, 0);
+    for (const HostMask& hm : hmasks)
+      for (int32_t t = 0; t < g.tilevox; t++) ubits[t] |= hm.bits[t];
+    std::vector<int> brk[3];
+    for (int k = 0; k < 3; k++) {
+      brk[k] = {0, t3[k]};
+      if (pl->ovl[k] > 1) { brk[k].push_back((int)pl->ovl[k]); brk[k].push_back((int)pl->spacing[k]); }
+      std::sort(brk[k].begin(), brk[k].end());
+      brk[k].erase(std::unique(brk[k].begin(), brk[k].end()), brk[k].end());
+      brk[k].erase(std::remove_if(brk[k].begin(), brk[k].end(), [&](int v) { return v < 0 || v > t3[k]; }), brk[k].end());
+    }
+    std::vector<IqChunk> chunks;
+    if (want && pl->fast && !hmasks.empty() && build_ktab(ubits, t3, brk, H.ktab, chunks)) {
+      size_t lmax = 0;
+      for (int b = 0; b < H.ktab.nbox; b++) {
+        const IqKBox& kb = H.ktab.box[b];
+        lmax = std::max(lmax, (size_t)(32 + kb.wxr - 1) * (16 + kb.wyr - 1) * kb.wz * 4);
+      }
+      if (lmax <= 150 * 1024) {
+        pl->mfma = true;
+        pl->mfma_lds = lmax;
+        pl->mfma_split = !bf16_exact(H.ti32, tivox);
+        const double kterms = 16.0 * H.ktab.nstep * (pl->mfma_split ? 3.0 : 1.0);
+        const double ga = 2.0 * kterms * 0x1p-24 / (1.0 - 2.0 * kterms * 0x1p-24);
+        pl->mfma_gerr = 1.02 * ga + (pl->mfma_split ? 4.0 * 0x1p-18 : 0.0);
+        H.chunks = pl->mem.upload(chunks.data(), chunks.size(), s);
+        std::vector<uint8_t> mb((size_t)hmasks.size() * g.tilevox);
+        for (size_t m = 0; m < hmasks.size(); m++) std::memcpy(&mb[m * g.tilevox], hmasks[m].bits.data(), g.tilevox);
+        H.maskbits = pl->mem.upload(mb.data(), mb.size(), s);
+        if (lmax > 64 * 1024) {
+          HC(hipFuncSetAttribute((const void*)k_scan_mfma<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lmax));
+          HC(hipFuncSetAttribute((const void*)k_scan_mfma<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lmax));
+        }
+      }
+    }
+  }
 
   // ---- hard and soft data
   if (pl->hard) H.hardgrid = pl->mem.upload(d->hardgrid, (size_t)g.padvox, s);
@@ -565,6 +688,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
   const double budget = 24.0 * (1ull << 30);
   int64_t cap = std::max<int64_t>(1, (int64_t)(budget / (double)per_slot));
   cap = std::min<int64_t>(cap, 65535 / std::max(1, pl->nbz));
+  if (pl->mfma) cap = std::min<int64_t>(cap, (int64_t)(65535 / std::max(1, g.dist[2])) * 32);
   cap = std::min<int64_t>(cap, 4096);
   pl->nslots = std::min(widest, cap);
   const int64_t NS = pl->nslots;
@@ -608,6 +732,12 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
     pl->sort_tmp_bytes = iq_sort_temp_bytes((int)P);
     pl->sort_tmp = pl->mem.alloc<char>(pl->sort_tmp_bytes);
   }
+  if (pl->mfma) {
+    pl->mfma_ntile_max = (NS + 31) / 32;
+    const size_t nb = (size_t)pl->mfma_ntile_max * H.ktab.nstep * 64;
+    H.bh = pl->mem.alloc<uint4>(nb);
+    H.bl = pl->mem.alloc<uint4>(nb);
+  }
   H.err = pl->mem.alloc<int32_t>(1);
   H.rounds = pl->mem.alloc<int64_t>(1);
 
@@ -630,13 +760,17 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
 void launch_scan_batch(iq_plan* pl, const IqQuery* q, int nq, hipStream_t s) {
   const IqGeo& g = pl->geo;
   const ScanCfg& c = kScanCfg[pl->scan_cfg];
-  dim3 grid(nblk(g.dist[0], c.R * c.LX), nblk(g.dist[1], c.LY), (unsigned)(pl->nbz * nq));
+  dim3 grid(nblk(g.dist[0], c.R * c.LX), nblk(g.dist[1], c.LY * c.RY), (unsigned)(pl->nbz * nq));
   const size_t lds = pl->scan_lds;
+  const IqDev* D = pl->dd;
+  const int perz = pl->scan_perz ? 1 : 0;
   switch (pl->scan_cfg) {
-    case 0: hipLaunchKernelGGL((k_scan_f32<4, 8, 8, 4>), grid, dim3(256), lds, s, (const IqDev*)pl->dd, q, pl->nbz); break;
-    case 1: hipLaunchKernelGGL((k_scan_f32<4, 16, 4, 4>), grid, dim3(256), lds, s, (const IqDev*)pl->dd, q, pl->nbz); break;
-    case 2: hipLaunchKernelGGL((k_scan_f32<2, 16, 8, 2>), grid, dim3(256), lds, s, (const IqDev*)pl->dd, q, pl->nbz); break;
-    default: hipLaunchKernelGGL((k_scan_f32<4, 8, 16, 2>), grid, dim3(256), lds, s, (const IqDev*)pl->dd, q, pl->nbz); break;
+    case 0: hipLaunchKernelGGL((k_scan_f32<4, 8, 8, 4>), grid, dim3(256), lds, s, D, q, pl->nbz); break;
+    case 1: hipLaunchKernelGGL((k_scan_f32<4, 16, 4, 4>), grid, dim3(256), lds, s, D, q, pl->nbz); break;
+    case 2: hipLaunchKernelGGL((k_scan_f32<2, 16, 8, 2>), grid, dim3(256), lds, s, D, q, pl->nbz); break;
+    case 3: hipLaunchKernelGGL((k_scan_f32<4, 8, 16, 2>), grid, dim3(256), lds, s, D, q, pl->nbz); break;
+    case 4: hipLaunchKernelGGL((k_scan2_f32<4, 4, 8, 8>), grid, dim3(256), lds, s, D, q, pl->nbz, perz); break;
+    default: hipLaunchKernelGGL((k_scan2_f32<4, 4, 16, 4>), grid, dim3(256), lds, s, D, q, pl->nbz, perz); break;
   }
   HCL();
 }
@@ -663,7 +797,19 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
         }
         HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));
       }
-      launch_scan_batch(pl, q, nq, s);
+      if (pl->mfma) {
+        const int ntile = (nq + 31) / 32;
+        hipLaunchKernelGGL(k_mfma_bprep, dim3(pl->hd.ktab.nstep, ntile), dim3(64), 0, s, D, q, nq);
+        HCL();
+        dim3 grid(nblk(g.dist[0], 32), nblk(g.dist[1], 16), (unsigned)(g.dist[2] * ntile));
+        if (pl->mfma_split)
+          hipLaunchKernelGGL((k_scan_mfma<true>), grid, dim3(256), pl->mfma_lds, s, D, q, nq, pl->mfma_gerr);
+        else
+          hipLaunchKernelGGL((k_scan_mfma<false>), grid, dim3(256), pl->mfma_lds, s, D, q, nq, pl->mfma_gerr);
+        HCL();
+      } else {
+        launch_scan_batch(pl, q, nq, s);
+      }
       if (pl->prof) {
         HC(hipEventRecord(pl->prof_ev[pl->prof_used + 1], s));
         pl->prof_used += 2;

@@ -65,6 +65,22 @@ struct IqQuery {
   int64_t forced;  // pick to replay, or -1
 };
 
+// MFMA scan: the K dimension is a list of 8-voxel chunks over the union of
+// all overlap masks, grouped by box; a chunk runs along x (dir 0) or y (dir 1).
+struct IqChunk {
+  int16_t jx, jy, jz;  // first voxel (tile coordinates)
+  int8_t dir, n;       // direction, valid voxels (0..8)
+};
+struct IqKBox {
+  int32_t x0, y0, z0, wxr, wyr, wz;  // staged region of the box (x/y extents padded to the chunk runs)
+  int32_t s0, s1;                    // k-steps [s0, s1) (two chunks per k-step)
+};
+#define IQ_MAX_KBOX 8
+struct IqKTab {
+  int32_t nbox, nstep;
+  IqKBox box[IQ_MAX_KBOX];
+};
+
 // Geometry shared by all kernels of a plan.
 struct IqGeo {
   int32_t ti[3];     // training-image extents
@@ -131,6 +147,12 @@ struct IqDev {
   uint32_t* bits;    int64_t nw32;         // nmap bitmaps of P bits
   int32_t* ev;                             // pattern database (P)
   double* vals;                            // P
+  // MFMA scan
+  IqKTab ktab;
+  const IqChunk* chunks;
+  const uint8_t* maskbits;                 // per mask: tilevox membership bytes
+  uint4* bh;                               // B fragments (hi), [ntile][nstep][64]
+  uint4* bl;                               // B fragments (lo)
 };
 
 __host__ __device__ inline int64_t iq_lin3(const int32_t d[3], int64_t x, int64_t y, int64_t z) {

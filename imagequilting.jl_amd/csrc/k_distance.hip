@@ -277,6 +277,320 @@ k_scan_f32(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nbz)
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 scan with a 2-D register block (3-D candidate grids): a work-item owns
+// R x-candidates times RY y-candidates at one z; the lanes of a wavefront span
+// x (LX) and z (LZ) and the four wavefronts span y, so every loop over
+// training-image rows is uniform across the wavefront.  One LDS row segment
+// (R + 3 words) feeds up to 4 * R * RY fused multiply-adds: the row at offset
+// ty serves candidate row iy with query row jy = ty - iy.  LDS rows use the
+// R-interleaved layout of k_scan_f32 (conflict-free lane reads); the z-plane
+// stride is padded to 8 words modulo 64 banks.
+// Grid: x, y candidate blocks; z = candidate z-block + nbz * query.
+// ---------------------------------------------------------------------------
+template <int R, int RY, int LX, int LZ>
+__global__ void __launch_bounds__(256)
+k_scan2_f32(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nbz, int perz) {
+  constexpr int LY = 256 / (LX * LZ);
+  extern __shared__ float lds[];
+  const IqQuery q = qs[blockIdx.z / nbz];
+  const IqStep& st = D->steps[q.step];
+  if (q.multi || st.mask < 0) return;
+  const IqGeo& g = D->g;
+  const IqMask* mask = &D->masks[st.mask];
+  const int64_t slot = q.slot;
+  const float* __restrict__ ti = D->ti32;
+  const float* __restrict__ kv32 = D->kv32 + slot * D->kv32_stride;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int lx = lane % LX, lz = lane / LX, ly = tid >> 6;
+  const int cx0 = blockIdx.x * (LX * R), cy0 = blockIdx.y * (LY * RY), cz0 = (blockIdx.z % nbz) * LZ;
+  float acc[RY][R];
+#pragma unroll
+  for (int a = 0; a < RY; a++)
+#pragma unroll
+    for (int i = 0; i < R; i++) acc[a][i] = 0.0f;
+  const int nbox = mask->nbox;
+  for (int b = 0; b < nbox; b++) {
+    const IqBox bx = mask->box[b];
+    const int wxp = bx.wxp, wy = bx.wy, wz = bx.wz;
+    const int S = LX + wxp / R;
+    const int RXL = R * LX + wxp - 1;
+    const int RYT = LY * RY + wy - 1, RZ = LZ + wz - 1;
+    const int rowst = R * S;
+    const int plane = ((RYT * rowst + 55) / 64) * 64 + 8;  // = 8 (mod 64)
+    const int gx0 = cx0 + bx.x0, gy0 = cy0 + bx.y0, gz0 = cz0 + bx.z0;
+    // perz: stage the LZ planes of one jz at a time (large tiles), else all RZ planes once
+    const int nplanes = perz ? LZ : RZ;
+    const int nst = RXL * RYT * nplanes;
+    const float* kb = kv32 + bx.kofs;
+    const int nty = RY + wy - 1;
+    for (int jz = 0; jz < wz; jz++) {
+      if (jz == 0 || perz) {
+        if (jz > 0) __syncthreads();
+        const int zoff = perz ? jz : 0;
+        for (int i = tid; i < nst; i += 256) {
+          int x = i % RXL, y = (i / RXL) % RYT, z = i / (RXL * RYT);
+          int gx = gx0 + x, gy = gy0 + y, gz = gz0 + zoff + z;
+          float v = 0.0f;
+          if (gx < g.ti[0] && gy < g.ti[1] && gz < g.ti[2]) v = ti[iq_lin3(g.ti, gx, gy, gz)];
+          lds[z * plane + y * rowst + (x % R) * S + x / R] = v;
+        }
+        __syncthreads();
+      }
+      const float* zp = lds + (lz + (perz ? 0 : jz)) * plane + lx + (ly * RY) * rowst;
+      const float* kz = kb + jz * wy * wxp;
+      for (int ty = 0; ty < nty; ty++) {
+        const float* rowp = zp + ty * rowst;
+        for (int jx = 0; jx < wxp; jx += 4) {
+          const float* rp = rowp + jx / R;
+          float v[R + 3];
+#pragma unroll
+          for (int m = 0; m < R + 3; m++) v[m] = rp[(m % R) * S + m / R];
+#pragma unroll
+          for (int a = 0; a < RY; a++) {
+            const int jy = ty - a;
+            if (jy >= 0 && jy < wy) {
+              const float* kp = kz + jy * wxp + jx;
+              const float k0 = kp[0], k1 = kp[1], k2 = kp[2], k3 = kp[3];
+#pragma unroll
+              for (int i = 0; i < R; i++) {
+                acc[a][i] = __builtin_fmaf(v[i], k0, acc[a][i]);
+                acc[a][i] = __builtin_fmaf(v[i + 1], k1, acc[a][i]);
+                acc[a][i] = __builtin_fmaf(v[i + 2], k2, acc[a][i]);
+                acc[a][i] = __builtin_fmaf(v[i + 3], k3, acc[a][i]);
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const double* a2map = D->a2maps[mask->a2];
+  const double b2 = D->b2[slot * (2 + D->naux)];
+  const double V = (double)mask->V;
+  const double u32 = 0x1p-24, u64 = 0x1p-53;
+  const double g32 = V * u32 / (1.0 - V * u32), g64 = V * u64 / (1.0 - V * u64);
+  float* lo_out = D->lo32 + slot * g.P;
+  float lmin = __builtin_inff();
+  const int cz = cz0 + lz;
+#pragma unroll
+  for (int a = 0; a < RY; a++) {
+    const int cy = cy0 + ly * RY + a;
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+      const int cx = cx0 + R * lx + i;
+      if (cx < g.dist[0] && cy < g.dist[1] && cz < g.dist[2]) {
+        int64_t c = iq_lin3(g.dist, cx, cy, cz);
+        float lo = __builtin_inff(), hi = __builtin_inff();
+        if (!D->disabled[c]) {
+          double a2 = a2map[c];
+          double sq = __builtin_sqrt(a2 * b2) * (1.0 + 0x1p-30);
+          double d = __builtin_fabs((a2 - 2.0 * (double)acc[a][i]) + b2);
+          double e = (2.0 * (g32 + g64) * sq + 4.0 * u64 * (a2 + b2 + 2.0 * sq) + 2.0 * V * 0x1p-148) * (1.0 + 0x1p-30)
+                     + 4.0 * u64 * d;
+          lo = iq_f32_down(d - e > 0.0 ? d - e : 0.0);
+          hi = iq_f32_up(d + e);
+        }
+        lo_out[c] = lo;
+        lmin = fminf(lmin, hi);
+      }
+    }
+  }
+  __shared__ float red[4];
+  for (int o = 32; o > 0; o >>= 1) lmin = fminf(lmin, __shfl_xor(lmin, o));
+  if (lane == 0) red[ly] = lmin;
+  __syncthreads();
+  if (tid == 0) {
+    float m = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
+    atomicMin(&D->gmin_hi[slot], __float_as_uint(m));
+  }
+}
+
+template __global__ void k_scan2_f32<4, 4, 8, 8>(const IqDev*, const IqQuery*, int, int);
+template __global__ void k_scan2_f32<4, 4, 16, 4>(const IqDev*, const IqQuery*, int, int);
+
+// ---------------------------------------------------------------------------
+// MFMA scan (multi-query GEMM): AB[c, q] = sum_k A[c, k] * B[k, q] with
+// A[c, k] = TI[c + voxel(k)] and B[k, q] = query q's value at voxel(k) (zero
+// outside q's mask), on v_mfma_f32_32x32x16_bf16: 32 candidates (consecutive
+// x) by 32 queries by 16 voxels per instruction, FP32 accumulation.  FP32
+// data is split t = hi + lo (two bf16) and three products are accumulated
+// (hi*hi, hi*lo, lo*hi); bf16-exact data (small integer codes) needs one.
+// Each wavefront owns four 32-candidate rows (consecutive y) of one z, the
+// group (four wavefronts) sixteen rows; the training-image block of each box
+// is staged in LDS as words (hi | lo << 16), and A fragments are gathered
+// from it along the chunk runs, so any run alignment works.
+// Grid: x blocks (32), y blocks (16), z = candidate z + dist_z * query tile.
+// ---------------------------------------------------------------------------
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 iq_bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float iq_f32x16;
+
+__device__ inline uint32_t iq_bf16_rne(float f) {  // round-to-nearest-even bf16 bits of a finite float
+  uint32_t u = __float_as_uint(f);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+__device__ inline uint32_t iq_split_word(float t) {  // hi | lo << 16
+  uint32_t h = iq_bf16_rne(t);
+  float r = t - __uint_as_float(h << 16);
+  return h | (iq_bf16_rne(r) << 16);
+}
+
+// B fragments of a batch: lane l of (tile, step) holds the eight values of
+// chunk 2*step + l/32 for query tile*32 + l%32, as bf16 hi and lo.
+__global__ void __launch_bounds__(64)
+k_mfma_bprep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq) {
+  const int step = blockIdx.x, tile = blockIdx.y, lane = threadIdx.x;
+  const int nstep = D->ktab.nstep;
+  const int qi = tile * 32 + (lane & 31);
+  uint32_t h[4] = {0u, 0u, 0u, 0u}, l[4] = {0u, 0u, 0u, 0u};
+  if (qi < nq) {
+    const IqQuery q = qs[qi];
+    const IqStep& st = D->steps[q.step];
+    if (!q.multi && st.mask >= 0) {
+      const IqGeo& g = D->g;
+      const IqChunk ch = D->chunks[2 * step + (lane >> 5)];
+      const uint8_t* mb = D->maskbits + (int64_t)st.mask * g.tilevox;
+      const float* sim32 = D->ti_f32 ? (const float*)D->sims + (int64_t)q.real * g.padvox : nullptr;
+      const double* sim64 = D->ti_f32 ? nullptr : (const double*)D->sims + (int64_t)q.real * g.padvox;
+      for (int e = 0; e < ch.n; e++) {
+        const int x = ch.jx + (ch.dir == 0 ? e : 0), y = ch.jy + (ch.dir == 1 ? e : 0), z = ch.jz;
+        const int tv = x + g.tile[0] * (y + g.tile[1] * z);
+        if (!mb[tv]) continue;
+        const int64_t gi = iq_lin3(g.pad, st.origin[0] + x, st.origin[1] + y, st.origin[2] + z);
+        const float v = sim32 ? sim32[gi] : (float)sim64[gi];
+        const uint32_t wv = iq_split_word(v);
+        h[e >> 1] |= (wv & 0xffffu) << (16 * (e & 1));
+        l[e >> 1] |= (wv >> 16) << (16 * (e & 1));
+      }
+    }
+  }
+  const int64_t o = ((int64_t)tile * nstep + step) * 64 + lane;
+  D->bh[o] = make_uint4(h[0], h[1], h[2], h[3]);
+  D->bl[o] = make_uint4(l[0], l[1], l[2], l[3]);
+}
+
+template <bool SPLIT>
+__global__ void __launch_bounds__(256)
+k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq, double gerr) {
+  extern __shared__ uint32_t lw[];
+  __shared__ float red[4][32];
+  const IqGeo& g = D->g;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row = lane & 31, kb = lane >> 5;
+  const int tile = blockIdx.z / g.dist[2];
+  const int cz = blockIdx.z % g.dist[2];
+  const int cx0 = blockIdx.x * 32, cy0 = blockIdx.y * 16;
+  const int nstep = D->ktab.nstep;
+  const float* __restrict__ ti = D->ti32;
+  iq_f32x16 acc[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc[i][v] = 0.0f;
+  for (int b = 0; b < D->ktab.nbox; b++) {
+    const IqKBox kbx = D->ktab.box[b];
+    const int RX = 32 + kbx.wxr - 1, RY = 16 + kbx.wyr - 1;
+    const int n = RX * RY * kbx.wz;
+    const int gx0 = cx0 + kbx.x0, gy0 = cy0 + kbx.y0, gz0 = cz + kbx.z0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+      int x = i % RX, y = (i / RX) % RY, z = i / (RX * RY);
+      int gx = gx0 + x, gy = gy0 + y, gz = gz0 + z;
+      float v = 0.0f;
+      if (gx < g.ti[0] && gy < g.ti[1] && gz < g.ti[2]) v = ti[iq_lin3(g.ti, gx, gy, gz)];
+      lw[i] = iq_split_word(v);
+    }
+    __syncthreads();
+    for (int step = kbx.s0; step < kbx.s1; step++) {
+      const int64_t o = ((int64_t)tile * nstep + step) * 64 + lane;
+      const uint4 bhv = D->bh[o];
+      iq_bf16x8 bh = __builtin_bit_cast(iq_bf16x8, bhv);
+      iq_bf16x8 bl;
+      if (SPLIT) bl = __builtin_bit_cast(iq_bf16x8, D->bl[o]);
+      const IqChunk ch = D->chunks[2 * step + kb];
+      const int es = ch.dir == 0 ? 1 : RX;
+      const int base = (ch.jz - kbx.z0) * RX * RY + (ch.jy - kbx.y0 + 4 * w) * RX + (ch.jx - kbx.x0) + row;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int a0 = base + i * RX;
+        uint32_t wv[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) wv[e] = lw[a0 + e * es];
+        uint4 ah, al;
+        ah.x = __builtin_amdgcn_perm(wv[1], wv[0], 0x05040100u);
+        ah.y = __builtin_amdgcn_perm(wv[3], wv[2], 0x05040100u);
+        ah.z = __builtin_amdgcn_perm(wv[5], wv[4], 0x05040100u);
+        ah.w = __builtin_amdgcn_perm(wv[7], wv[6], 0x05040100u);
+        acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(iq_bf16x8, ah), bh, acc[i], 0, 0, 0);
+        if (SPLIT) {
+          al.x = __builtin_amdgcn_perm(wv[1], wv[0], 0x07060302u);
+          al.y = __builtin_amdgcn_perm(wv[3], wv[2], 0x07060302u);
+          al.z = __builtin_amdgcn_perm(wv[5], wv[4], 0x07060302u);
+          al.w = __builtin_amdgcn_perm(wv[7], wv[6], 0x07060302u);
+          acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(iq_bf16x8, ah), bl, acc[i], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(iq_bf16x8, al), bh, acc[i], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // epilogue: lane holds C[row'][col] for col = lane % 32 (its query)
+  const int qi = tile * 32 + row;
+  float lmin = __builtin_inff();
+  bool live = false;
+  int64_t slot = 0;
+  if (qi < nq) {
+    const IqQuery q = qs[qi];
+    const IqStep& st = D->steps[q.step];
+    live = !q.multi && st.mask >= 0;
+    slot = q.slot;
+    if (live) {
+      const IqMask& m = D->masks[st.mask];
+      const double* a2map = D->a2maps[m.a2];
+      const double b2 = D->b2[slot * (2 + D->naux)];
+      const double V = (double)m.V;
+      const double u64 = 0x1p-53;
+      const double g64 = V * u64 / (1.0 - V * u64);
+      float* lo_out = D->lo32 + slot * g.P;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int cy = cy0 + 4 * w + i;
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+          const int cx = cx0 + 8 * (v >> 2) + 4 * kb + (v & 3);
+          if (cx < g.dist[0] && cy < g.dist[1]) {
+            const int64_t c = iq_lin3(g.dist, cx, cy, cz);
+            float lo = __builtin_inff(), hi = __builtin_inff();
+            if (!D->disabled[c]) {
+              const double a2 = a2map[c];
+              const double sq = __builtin_sqrt(a2 * b2) * (1.0 + 0x1p-30);
+              const double d = __builtin_fabs((a2 - 2.0 * (double)acc[i][v]) + b2);
+              const double e = (2.0 * (gerr + g64) * sq + 4.0 * u64 * (a2 + b2 + 2.0 * sq) + 1e3 * V * 0x1p-125) *
+                               (1.0 + 0x1p-30) + 4.0 * u64 * d;
+              lo = iq_f32_down(d - e > 0.0 ? d - e : 0.0);
+              hi = iq_f32_up(d + e);
+            }
+            lo_out[c] = lo;
+            lmin = fminf(lmin, hi);
+          }
+        }
+      }
+    }
+  }
+  lmin = fminf(lmin, __shfl_xor(lmin, 32));
+  if (kb == 0) red[w][row] = lmin;
+  __syncthreads();
+  if (w == 0 && kb == 0 && live) {
+    const float m = fminf(fminf(red[0][row], red[1][row]), fminf(red[2][row], red[3][row]));
+    atomicMin(&D->gmin_hi[slot], __float_as_uint(m));
+  }
+}
+
+template __global__ void k_scan_mfma<true>(const IqDev*, const IqQuery*, int, double);
+template __global__ void k_scan_mfma<false>(const IqDev*, const IqQuery*, int, double);
+
 #define IQ_SCAN_INST(R, LX, LY, LZ) \
   template __global__ void k_scan_f32<R, LX, LY, LZ>(const IqDev*, const IqQuery*, int);
 IQ_SCAN_INST(4, 8, 8, 4)

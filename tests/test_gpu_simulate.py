@@ -116,3 +116,12 @@ def test_sim_large_overlap_runs_sequentially():
     T = _smooth(rng, (30, 30), 5, np.float32)
     print("overlap 0.6", flush=True)
     _compare(T, (10, 10), (30, 30), overlap=(0.6, 0.6))
+
+
+def test_sim_valu_scan_path(monkeypatch):
+    # the FP32 VALU scan (selected with IQHIP_SCAN=valu) against the oracle
+    monkeypatch.setenv("IQHIP_SCAN", "valu")
+    rng = np.random.default_rng(10)
+    T = _smooth(rng, (30, 30, 12), 5, np.float32)
+    print("valu scan", flush=True)
+    _compare(T, (10, 10, 4), (40, 40, 12), overlap=(0.25, 0.25, 0.25))
