@@ -322,6 +322,7 @@ struct iq_plan {
   double prof_flop = 0.0, prof_sec = 0.0;
   int64_t prof_launches = 0;
   std::vector<double> batch_flop;
+  int64_t cut_stats[3] = {0, 0, 0};  // push rounds, relabel BFS levels, cuts of the last run
   // stable sorts of the multi-source maps
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
@@ -739,7 +740,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
     H.bl = pl->mem.alloc<uint4>(nb);
   }
   H.err = pl->mem.alloc<int32_t>(1);
-  H.rounds = pl->mem.alloc<int64_t>(1);
+  H.rounds = pl->mem.alloc<int64_t>(3);  // push rounds, relabel BFS levels, cuts
 
   // ---- results
   const size_t esz = pl->dtype == IQ_F32 ? 4 : 8;
@@ -901,7 +902,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   if (!qs.empty()) HC(hipMemcpyAsync(pl->dq, qs.data(), qs.size() * sizeof(IqQuery), hipMemcpyHostToDevice, s));
   HC(hipEventRecord(pl->e0, s));
   hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(64), 0, s, H.err, (int64_t)1, 0);
-  hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, H.rounds, (int64_t)1, (int64_t)0);
+  hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, H.rounds, (int64_t)3, (int64_t)0);
   hipLaunchKernelGGL((k_fill<int64_t>), dim3(256), dim3(256), 0, s, H.log, pl->nreal * pl->npath, (int64_t)-1);
   hipLaunchKernelGGL((k_fill<T>), dim3(1024), dim3(256), 0, s, (T*)H.sims, pl->nreal * g.padvox, (T)0);
   hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s, H.cuts, pl->nreal * g.padvox, (uint8_t)0);
@@ -920,6 +921,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   }
   int32_t code = 0;
   HC(hipMemcpy(&code, H.err, sizeof(int32_t), hipMemcpyDeviceToHost));
+  HC(hipMemcpy(pl->cut_stats, H.rounds, 3 * sizeof(int64_t), hipMemcpyDeviceToHost));
   check_kernel_error(code);
 }
 
@@ -1279,6 +1281,14 @@ int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double
   if (launches) *launches = plan->prof_launches;
   if (seconds) *seconds = plan->prof_sec;
   if (flop) *flop = plan->prof_flop;
+  return IQ_OK;
+}
+
+int iq_plan_cut_stats(iq_plan* plan, int64_t* rounds, int64_t* levels, int64_t* cuts) {
+  if (!plan) return fail(nullptr, IQ_ERR_ARG, "null plan");
+  if (rounds) *rounds = plan->cut_stats[0];
+  if (levels) *levels = plan->cut_stats[1];
+  if (cuts) *cuts = plan->cut_stats[2];
   return IQ_OK;
 }
 

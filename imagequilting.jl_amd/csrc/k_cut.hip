@@ -309,9 +309,11 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
   S.pr1 = S.pr0 + 3 * n;
   S.h0 = (int32_t*)(S.pr1 + 3 * n);
   S.h1 = S.h0 + n;
-  // per owned node: coordinates, flags (bit d: plus edge d, bit 3: sink edge, bits 4+d: minus edge d)
+  // per owned node: flags (bit d: plus edge d, bit 3: sink edge, bits 4+d: minus edge d),
+  // neighbour indices (clamped to the node itself where absent), excess, sink residual
   int fl[NPT];
   int uu[NPT];
+  int vm[NPT][3], vp[NPT][3];
   double e[NPT], rt[NPT];
 #pragma unroll
   for (int k = 0; k < NPT; k++) {
@@ -320,6 +322,8 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
     fl[k] = 0;
     e[k] = 0.0;
     rt[k] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) { vm[k][d] = u < n ? u : 0; vp[k][d] = u < n ? u : 0; }
     if (u < n) {
       int ic[3] = {u % si[0], (u / si[0]) % si[1], u / (si[0] * si[1])};
       int pc[3] = {ic[0], ic[1], ic[2]};
@@ -327,8 +331,8 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
       int f = 0;
       for (int d = 0; d < 3; d++) {
         double c = 0.0;
-        if (ic[d] + 1 < si[d]) { f |= 1 << d; c = slab_cap<T>(A, B, s, pc[0], pc[1], pc[2], d); }
-        if (ic[d] > 0) f |= 16 << d;
+        if (ic[d] + 1 < si[d]) { f |= 1 << d; c = slab_cap<T>(A, B, s, pc[0], pc[1], pc[2], d); vp[k][d] = u + ist[d]; }
+        if (ic[d] > 0) { f |= 16 << d; vm[k][d] = u - ist[d]; }
         S.rf[3 * u + d] = c;
         S.rb[3 * u + d] = c;
         S.pr0[3 * u + d] = 0.0;
@@ -345,7 +349,7 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
   }
   __syncthreads();
 
-  int64_t total = 0;
+  int64_t total = 0, levels = 0;
   int32_t* h = S.h0;   // committed heights
   int32_t* hn = S.h1;  // next heights
   for (;;) {
@@ -354,25 +358,32 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
     for (int k = 0; k < NPT; k++)
       if (uu[k] < n) h[uu[k]] = ((fl[k] & 8) && rt[k] > 0.0) ? 1 : HMAX;
     __syncthreads();
+    // residual reachability towards the sink does not change during the BFS: load it once
+    bool canm[NPT][3], canp[NPT][3];
+#pragma unroll
+    for (int k = 0; k < NPT; k++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        canm[k][d] = (fl[k] & (16 << d)) && S.rb[3 * vm[k][d] + d] > 0.0;
+        canp[k][d] = (fl[k] & (1 << d)) && S.rf[3 * uu[k] + d] > 0.0;
+      }
     for (int lev = 1;; lev++) {
       int ch = 0;
 #pragma unroll
       for (int k = 0; k < NPT; k++) {
         const int u = uu[k];
-        if (u >= n || h[u] != HMAX) continue;
+        if (u >= n) continue;
+        const int hu = h[u];
+        int hm[3], hp[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) { hm[d] = h[vm[k][d]]; hp[d] = h[vp[k][d]]; }
+        if (hu != HMAX) continue;
         bool hit = false;
-        for (int d = 0; d < 3 && !hit; d++) {
-          if (fl[k] & (16 << d)) {
-            int v = u - ist[d];
-            if (S.rb[3 * v + d] > 0.0 && h[v] == lev) hit = true;
-          }
-          if (!hit && (fl[k] & (1 << d))) {
-            int v = u + ist[d];
-            if (S.rf[3 * u + d] > 0.0 && h[v] == lev) hit = true;
-          }
-        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) hit |= (canm[k][d] && hm[d] == lev) || (canp[k][d] && hp[d] == lev);
         if (hit) { h[u] = lev + 1; ch = 1; }
       }
+      levels++;
       if (!__syncthreads_or(ch)) break;
     }
     int act = 0;
@@ -386,7 +397,6 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
     }
     double* prc = S.pr0;  // records written by this round's push
     double* pro = S.pr1;  // records of the previous round (gathered, then cleared)
-    bool stop = false;
     for (int it = 0; it <= IQ_GR_FREQ; it++) {
       // phase A: gather the previous round's pushes, then relabel
       int a2 = 0;
@@ -394,21 +404,27 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
       for (int k = 0; k < NPT; k++) {
         const int u = uu[k];
         if (u >= n) continue;
+        double qm[3], qp[3], rbm[3], rfp[3];
+        int hm[3], hp[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          qm[d] = pro[3 * vm[k][d] + d];
+          qp[d] = pro[3 * u + d];
+          rbm[d] = S.rb[3 * vm[k][d] + d];
+          rfp[d] = S.rf[3 * u + d];
+          hm[d] = h[vm[k][d]];
+          hp[d] = h[vp[k][d]];
+        }
+        const int hu = h[u];
         if (it > 0) {
           double ex = e[k];
+#pragma unroll
           for (int d = 0; d < 3; d++) {
-            if (fl[k] & (16 << d)) {
-              double q = pro[3 * (u - ist[d]) + d];
-              if (q > 0.0) ex += q;
-            }
-            if (fl[k] & (1 << d)) {
-              double q = pro[3 * u + d];
-              if (q < 0.0) ex += -q;
-            }
+            if ((fl[k] & (16 << d)) && qm[d] > 0.0) ex += qm[d];
+            if ((fl[k] & (1 << d)) && qp[d] < 0.0) ex += -qp[d];
           }
           e[k] = ex;
         }
-        const int hu = h[u];
         int nh = hu;
         const bool active = e[k] > 0.0 && hu < HMAX;
         a2 |= active;
@@ -416,15 +432,10 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
           bool adm = false;
           int mh = HMAX;
           if ((fl[k] & 8) && rt[k] > 0.0) { if (hu == 1) adm = true; mh = 0; }
+#pragma unroll
           for (int d = 0; d < 3; d++) {
-            if (fl[k] & (16 << d)) {
-              int v = u - ist[d];
-              if (S.rb[3 * v + d] > 0.0) { int hv = h[v]; if (hu == hv + 1) adm = true; if (hv < mh) mh = hv; }
-            }
-            if (fl[k] & (1 << d)) {
-              int v = u + ist[d];
-              if (S.rf[3 * u + d] > 0.0) { int hv = h[v]; if (hu == hv + 1) adm = true; if (hv < mh) mh = hv; }
-            }
+            if ((fl[k] & (16 << d)) && rbm[d] > 0.0) { if (hu == hm[d] + 1) adm = true; if (hm[d] < mh) mh = hm[d]; }
+            if ((fl[k] & (1 << d)) && rfp[d] > 0.0) { if (hu == hp[d] + 1) adm = true; if (hp[d] < mh) mh = hp[d]; }
           }
           if (!adm) nh = (mh + 1 < HMAX) ? mh + 1 : HMAX;
         }
@@ -432,7 +443,7 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
       }
       if (it > 0) {
         total++;
-        if (!__syncthreads_or(a2)) { stop = true; break; }
+        if (!__syncthreads_or(a2)) break;
       } else {
         __syncthreads();
       }
@@ -447,37 +458,42 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
         double ex = e[k];
         const int hu = h[u];
         if (!(ex > 0.0 && hu < HMAX)) continue;
+        double rbm[3], rfp[3];
+        int hm[3], hp[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          rbm[d] = S.rb[3 * vm[k][d] + d];
+          rfp[d] = S.rf[3 * u + d];
+          hm[d] = h[vm[k][d]];
+          hp[d] = h[vp[k][d]];
+        }
         if ((fl[k] & 8) && hu == 1 && rt[k] > 0.0) {
           double r = rt[k];
           double dl = ex < r ? ex : r;
           ex -= dl;
           rt[k] = r - dl;
         }
-        for (int d = 0; d < 3 && ex > 0.0; d++) {
-          if (fl[k] & (16 << d)) {
-            int v = u - ist[d];
-            if (hu == h[v] + 1) {
-              double r = S.rb[3 * v + d];
-              if (r > 0.0) {
-                double dl = ex < r ? ex : r;
-                ex -= dl;
-                S.rb[3 * v + d] = r - dl;
-                S.rf[3 * v + d] += dl;
-                prc[3 * v + d] = -dl;
-              }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          if (ex > 0.0 && (fl[k] & (16 << d)) && hu == hm[d] + 1) {
+            double r = rbm[d];
+            if (r > 0.0) {
+              const int v = vm[k][d];
+              double dl = ex < r ? ex : r;
+              ex -= dl;
+              S.rb[3 * v + d] = r - dl;
+              S.rf[3 * v + d] += dl;
+              prc[3 * v + d] = -dl;
             }
           }
-          if (ex > 0.0 && (fl[k] & (1 << d))) {
-            int v = u + ist[d];
-            if (hu == h[v] + 1) {
-              double r = S.rf[3 * u + d];
-              if (r > 0.0) {
-                double dl = ex < r ? ex : r;
-                ex -= dl;
-                S.rf[3 * u + d] = r - dl;
-                S.rb[3 * u + d] += dl;
-                prc[3 * u + d] = dl;
-              }
+          if (ex > 0.0 && (fl[k] & (1 << d)) && hu == hp[d] + 1) {
+            double r = rfp[d];
+            if (r > 0.0) {
+              double dl = ex < r ? ex : r;
+              ex -= dl;
+              S.rf[3 * u + d] = r - dl;
+              S.rb[3 * u + d] += dl;
+              prc[3 * u + d] = dl;
             }
           }
         }
@@ -494,7 +510,6 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
                    prc[3 * u] = 0.0; prc[3 * u + 1] = 0.0; prc[3 * u + 2] = 0.0; }
     }
     __syncthreads();
-    (void)stop;
   }
 #pragma unroll
   for (int k = 0; k < NPT; k++) {
@@ -506,7 +521,11 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
     uint8_t m = (h[u] < HMAX) ? 0 : 1;
     out[pc[0] + s[0] * (pc[1] + s[1] * pc[2])] = m ^ side;
   }
-  if (threadIdx.x == 0 && rounds) atomicAdd((unsigned long long*)rounds, (unsigned long long)total);
+  if (threadIdx.x == 0 && rounds) {
+    atomicAdd((unsigned long long*)rounds, (unsigned long long)total);
+    atomicAdd((unsigned long long*)rounds + 1, (unsigned long long)levels);
+    atomicAdd((unsigned long long*)rounds + 2, 1ull);
+  }
 }
 
 template <typename T>
@@ -516,7 +535,7 @@ __device__ void cut_dispatch(const SlabView<T>& A, const SlabView<T>& B, const i
   if (n <= CB) cut_slab_fast<T, 1>(A, B, s, dim, side, out, lds, err, rounds);
   else if (n <= 2 * CB) cut_slab_fast<T, 2>(A, B, s, dim, side, out, lds, err, rounds);
   else if (n <= 4 * CB) cut_slab_fast<T, 4>(A, B, s, dim, side, out, lds, err, rounds);
-  else if (n <= IQ_PR_NLDS) cut_slab_fast<T, 6>(A, B, s, dim, side, out, lds, err, rounds);
+  else if (n <= IQ_PR_NLDS) cut_slab<T>(A, B, s, dim, side, out, lds, err, rounds);
   else cut_slab<T>(A, B, s, dim, side, out, gmem, err, rounds);
 }
 

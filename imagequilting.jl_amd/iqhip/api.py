@@ -206,6 +206,13 @@ class Plan:
         self.ctx.check(self.ctx._L.iq_plan_scan_stats(self.handle, _ptr(n), t[0:].ctypes.data, t[1:].ctypes.data))
         return int(n[0]), float(t[0]), float(t[1])
 
+    def cut_stats(self):
+        """(push rounds, relabel BFS levels, cuts) of the last run."""
+        v = np.zeros(3, dtype=np.int64)
+        self.ctx.check(self.ctx._L.iq_plan_cut_stats(self.handle, v[0:].ctypes.data, v[1:].ctypes.data,
+                                                     v[2:].ctypes.data))
+        return int(v[0]), int(v[1]), int(v[2])
+
     def results(self, grids=True, chosen=True, cuts=False):
         padvox = int(np.prod(self.padsize))
         out = np.zeros(self.nreal * padvox, dtype=self.dtype) if grids else None

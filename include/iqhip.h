@@ -147,6 +147,11 @@ int iq_plan_last_time(iq_plan* plan, double* seconds);
 int iq_plan_profile(iq_plan* plan, int enable);
 int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double* flop);
 
+/* Min-cut work of the last iq_plan_run (graphcut.jl:73 replacement):
+ * push-relabel rounds, global-relabel BFS levels and cuts (register-resident
+ * kernel; slabs beyond its size limit are not counted).  Diagnostic only. */
+int iq_plan_cut_stats(iq_plan* plan, int64_t* rounds, int64_t* levels, int64_t* cuts);
+
 /* Convenience: create + run + results + destroy. */
 int iq_simulate(iq_ctx* ctx, const iq_plan_desc* desc, const double* uniforms, const int64_t* forced,
                 void* out, int64_t* chosen, double* cutgrid);
