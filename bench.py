@@ -8,14 +8,18 @@ raster path, no soft or hard data.  One step = one complete simulation (all
 paste) with the training image already resident in HBM.
 
 Also reported:
-  roofline      the FP32 distance scan (dominant kernel), timed live with HIP
-                events on the library's stream around every scan launch during
-                the timed steps; algorithmic FLOPs = 2 * candidates * masked
-                voxels per launch.
+  roofline      the MFMA distance scan k_scan_mfma (dominant kernel), timed
+                live with HIP events on the library's stream around every scan
+                launch during the timed steps; algorithmic FLOPs = 2 *
+                candidates * masked voxels per query, summed over the queries
+                of a launch.  `traffic` (HBM bytes per launch) comes from the
+                rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/, scaled
+                by the same counters on a copy of known size (--copy-mb).
   cpu_baseline  the FP64 CPU restatement (oracle/, OpenMP) on a bounded sample
                 of the same workload: the first tile steps of one realization.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-sample N]
+       python bench.py --copy-mb M    (device copy of M MiB only: counter calibration)
 """
 import argparse
 import json
@@ -27,7 +31,36 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "imagequilting.jl_amd"))
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix peak (published)
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense bf16 MFMA peak (published)
 HBM_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "scan_mfma_grf3d_100x100x50_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per k_scan_mfma launch from the committed counter summary
+    (line 'traffic_bytes_per_launch: <bytes>'), or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(mib: int):
+    """Device-to-device copy of `mib` MiB, five times: the reference kernel
+    for scaling FETCH_SIZE / WRITE_SIZE (each copy reads and writes mib MiB)."""
+    import torch
+    n = mib << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n, "copies": 5}), flush=True)
 
 
 def cpu_baseline(w, sample_steps: int):
@@ -59,7 +92,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=169,
                     help="tile steps timed on the CPU (0 = skip); 169 = the first z-layer of tiles")
+    ap.add_argument("--copy-mb", type=int, default=0, help="only run the copy used to calibrate HBM counters")
     args = ap.parse_args()
+    if args.copy_mb > 0:
+        copy_calibration(args.copy_mb)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU design is described, not built "
               "(realizations shard across GPUs with no communication).", file=sys.stderr)
@@ -82,12 +119,12 @@ def main():
     lib.iq_ctx_synchronize(ctx.handle)
     t0 = time.perf_counter()
     dev_time = 0.0
-    launches, scan_sec, scan_flop = 0, 0.0, 0.0
+    launches, scan_sec, scan_flop, mfma_flop = 0, 0.0, 0.0, 0.0
     for i in range(args.steps):
         plan.run(U[args.warmup + i])  # returns after the stream has drained
         dev_time += plan.last_time()
-        n, s, f = plan.scan_stats()
-        launches, scan_sec, scan_flop = launches + n, scan_sec + s, scan_flop + f
+        n, s, f, mf = plan.scan_stats()
+        launches, scan_sec, scan_flop, mfma_flop = launches + n, scan_sec + s, scan_flop + f, mfma_flop + mf
     lib.iq_ctx_synchronize(ctx.handle)
     elapsed = time.perf_counter() - t0
     cut_rounds, cut_levels, ncuts = plan.cut_stats()
@@ -121,11 +158,13 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
-            "kernel": "k_scan_f32",
+            "traffic": measured_traffic(),
+            "kernel": "k_scan_mfma",
             "launches": launches,
             "avg_launch_us": scan_sec / max(launches, 1) * 1e6,
             "flop_per_launch": scan_flop / max(launches, 1),
+            "mfma_bf16_issued_tflops": mfma_flop / scan_sec / 1e12 if scan_sec > 0 else 0.0,
+            "mfma_bf16_util": (mfma_flop / scan_sec / 1e12 / PEAK_BF16_TFLOPS) if scan_sec > 0 else 0.0,
         },
     }
     plan.close()

@@ -319,7 +319,7 @@ struct iq_plan {
   bool prof = false;
   std::vector<hipEvent_t> prof_ev;
   int64_t prof_used = 0;
-  double prof_flop = 0.0, prof_sec = 0.0;
+  double prof_flop = 0.0, prof_sec = 0.0, prof_mflop = 0.0;
   int64_t prof_launches = 0;
   std::vector<double> batch_flop;
   int64_t cut_stats[3] = {0, 0, 0};  // push rounds, relabel BFS levels, cuts of the last run
@@ -680,7 +680,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
       maxn = std::max(maxn, n);
     }
   const int64_t cutper = maxn > IQ_PR_NLDS ? maxn * CUT_NODE_BYTES : 0;
-  size_t per_slot = (size_t)nmap * g.tilevox * 8 + (size_t)kmax * 4 + 64 + (size_t)P * (4 + 8 + 4 + 4) + nw64 * 8 +
+  size_t per_slot = (size_t)nmap * g.tilevox * 8 + (size_t)kmax * 4 + 64 + (size_t)P * (4 + 8 + 4 + 4 + 16) + nw64 * 8 +
                     (size_t)IQ_SORT_MAX * 12 + (size_t)(nw64 + 1) * 8 + (size_t)IQ_MAX_SLAB * g.tilevox +
                     (size_t)IQ_MAX_SLAB * cutper;
   if (pl->any_multi) per_slot += (size_t)nmap * P * 12 + (size_t)nmap * nw32 * 4 + (size_t)P * 12;
@@ -706,6 +706,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
   H.nw64 = nw64;
   H.band = pl->mem.alloc<int32_t>((size_t)NS * P);
   H.sup = pl->mem.alloc<int32_t>((size_t)NS * P);
+  H.rkey = pl->mem.alloc<unsigned long long>((size_t)NS * 2 * P);
   H.nsup = pl->mem.alloc<int32_t>(NS);
   H.sd = pl->mem.alloc<double>((size_t)NS * IQ_SORT_MAX);
   H.si = pl->mem.alloc<int32_t>((size_t)NS * IQ_SORT_MAX);
@@ -788,27 +789,31 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
   HCL();
   if (has_plain) {
     if (pl->fast) {
-      if (pl->prof) {
-        if (pl->prof_used + 2 > (int64_t)pl->prof_ev.size()) {
-          hipEvent_t a, b;
-          HC(hipEventCreate(&a));
-          HC(hipEventCreate(&b));
-          pl->prof_ev.push_back(a);
-          pl->prof_ev.push_back(b);
-        }
-        HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));
+      if (pl->prof && pl->prof_used + 2 > (int64_t)pl->prof_ev.size()) {
+        hipEvent_t a, b;
+        HC(hipEventCreate(&a));
+        HC(hipEventCreate(&b));
+        pl->prof_ev.push_back(a);
+        pl->prof_ev.push_back(b);
       }
       if (pl->mfma) {
         const int ntile = (nq + 31) / 32;
         hipLaunchKernelGGL(k_mfma_bprep, dim3(pl->hd.ktab.nstep, ntile), dim3(64), 0, s, D, q, nq);
         HCL();
         dim3 grid(nblk(g.dist[0], 32), nblk(g.dist[1], 16), (unsigned)(g.dist[2] * ntile));
+        if (pl->prof) {
+          HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));
+          // 4 wavefronts x 4 row blocks x k-steps x (3 or 1) 32x32x16 MFMAs per group
+          pl->prof_mflop += (double)grid.x * grid.y * grid.z * 16.0 * pl->hd.ktab.nstep * (pl->mfma_split ? 3.0 : 1.0) *
+                            (2.0 * 32 * 32 * 16);
+        }
         if (pl->mfma_split)
           hipLaunchKernelGGL((k_scan_mfma<true>), grid, dim3(256), pl->mfma_lds, s, D, q, nq, pl->mfma_gerr);
         else
           hipLaunchKernelGGL((k_scan_mfma<false>), grid, dim3(256), pl->mfma_lds, s, D, q, nq, pl->mfma_gerr);
         HCL();
       } else {
+        if (pl->prof) HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));
         launch_scan_batch(pl, q, nq, s);
       }
       if (pl->prof) {
@@ -863,6 +868,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   IqDev& H = pl->hd;
   pl->prof_used = 0;
   pl->prof_flop = 0.0;
+  pl->prof_mflop = 0.0;
   pl->prof_sec = 0.0;
   pl->prof_launches = 0;
   pl->batch_flop.clear();
@@ -1136,10 +1142,11 @@ int iq_taumodel(iq_ctx* ctx, const int64_t* events, int64_t nev, const double* D
     double* vals = m.alloc<double>(nev);
     int32_t* rank = m.alloc<int32_t>(nev);
     double* p = m.alloc<double>(nev);
+    unsigned long long* keys = m.alloc<unsigned long long>(2 * nev);
     int32_t* err = m.alloc<int32_t>(1);
     HC(hipMemsetAsync(err, 0, sizeof(int32_t), s));
     hipLaunchKernelGGL(k_taumodel_host, dim3(1), dim3(FB), 0, s, (const int32_t*)dev, nev, naux + 1, dmaps, vals, rank,
-                       p, err);
+                       p, keys, err);
     HCL();
     int32_t code = 0;
     HC(hipMemcpyAsync(&code, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1276,11 +1283,12 @@ int iq_plan_profile(iq_plan* plan, int enable) {
   return IQ_OK;
 }
 
-int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double* flop) {
+int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double* flop, double* mfma_flop) {
   if (!plan) return fail(nullptr, IQ_ERR_ARG, "null plan");
   if (launches) *launches = plan->prof_launches;
   if (seconds) *seconds = plan->prof_sec;
   if (flop) *flop = plan->prof_flop;
+  if (mfma_flop) *mfma_flop = plan->prof_mflop;
   return IQ_OK;
 }
 

@@ -132,7 +132,8 @@ struct IqDev {
   double* dexact;                          // P per slot
   unsigned long long* bitmap; int64_t nw64;
   int32_t* band;                           // P per slot
-  int32_t* sup;                            // superset list, P per slot
+  int32_t* sup;                            // superset list, P per slot (then ranks of a large band)
+  unsigned long long* rkey;                // 2P per slot: radix keys of a large band (then its weights)
   int32_t* nsup;
   double* sd;                              // IQ_SORT_MAX per slot (band distances, then weights)
   int32_t* si;                             // IQ_SORT_MAX per slot (ranks)

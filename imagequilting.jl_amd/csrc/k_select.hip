@@ -84,18 +84,143 @@ __device__ void blk_bitonic(double* sv, int m) {
   }
 }
 
+__device__ unsigned long long blk_or_u64(unsigned long long v, SelScratch& sh) {
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) sh.i64[threadIdx.x >> 6] = (int64_t)v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long r = 0;
+    for (int i = 0; i < NW; i++) r |= (unsigned long long)sh.i64[i];
+    sh.i64[NW] = (int64_t)r;
+  }
+  __syncthreads();
+  unsigned long long r = (unsigned long long)sh.i64[NW];
+  __syncthreads();
+  return r;
+}
+
+// Lanes of the wavefront whose 8-bit digit equals this lane's (valid lanes only).
+__device__ inline unsigned long long digit_peers(bool valid, int d) {
+  unsigned long long peers = __ballot(valid);
+  for (int b = 0; b < 8; b++) {
+    const unsigned long long m = __ballot((d >> b) & 1);
+    peers &= ((d >> b) & 1) ? m : ~m;
+  }
+  return peers;
+}
+
+// Stable ascending LSD radix sort (8-bit digits) of n >= 1 keys in global
+// memory by one workgroup.  Each wavefront owns a contiguous segment of the
+// keys (a multiple of 64); inside a 64-key tile, lanes with equal digits are
+// ordered by lane through ballots.  Counts live in cnt (NW * 256 ints of LDS,
+// digit-major), so the scatter order is (digit, wavefront, position).
+// Digits on which all keys agree are skipped.  Returns the buffer holding the
+// sorted keys (a or b).
+__device__ unsigned long long* blk_radix_sort(unsigned long long* a, unsigned long long* b, int64_t n, int32_t* cnt,
+                                              SelScratch& sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long k0 = a[0];
+  unsigned long long diff = 0;
+  for (int64_t i = threadIdx.x; i < n; i += FB) diff |= a[i] ^ k0;
+  diff = blk_or_u64(diff, sh);
+  const int64_t seg = (((n + NW - 1) / NW) + 63) & ~(int64_t)63;
+  const int64_t s0 = (int64_t)w * seg < n ? (int64_t)w * seg : n;
+  const int64_t s1 = s0 + seg < n ? s0 + seg : n;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (int sh8 = 0; sh8 < 64; sh8 += 8) {
+    if (((diff >> sh8) & 0xffull) == 0) continue;
+    for (int i = threadIdx.x; i < NW * 256; i += FB) cnt[i] = 0;
+    __syncthreads();
+    for (int64_t t = s0; t < s1; t += 64) {
+      const int64_t i = t + lane;
+      const bool v = i < s1;
+      const int d = v ? (int)((a[i] >> sh8) & 0xffull) : 0;
+      const unsigned long long peers = digit_peers(v, d);
+      if (v && (peers & lt) == 0) cnt[d * NW + w] += __popcll(peers);
+    }
+    __syncthreads();
+    {
+      const int e0 = threadIdx.x * 4;  // NW * 256 == 4 * FB entries
+      const int c0 = cnt[e0], c1 = cnt[e0 + 1], c2 = cnt[e0 + 2], c3 = cnt[e0 + 3];
+      const int ex = (int)blk_excl_scan(c0 + c1 + c2 + c3, nullptr, sh);
+      cnt[e0] = ex;
+      cnt[e0 + 1] = ex + c0;
+      cnt[e0 + 2] = ex + c0 + c1;
+      cnt[e0 + 3] = ex + c0 + c1 + c2;
+    }
+    __syncthreads();
+    for (int64_t t = s0; t < s1; t += 64) {
+      const int64_t i = t + lane;
+      const bool v = i < s1;
+      const unsigned long long key = v ? a[i] : 0ull;
+      const int d = (int)((key >> sh8) & 0xffull);
+      const unsigned long long peers = digit_peers(v, d);
+      if (v) b[cnt[d * NW + w] + __popcll(peers & lt)] = key;
+      if (v && (peers & lt) == 0) cnt[d * NW + w] += __popcll(peers);
+    }
+    __syncthreads();
+    unsigned long long* x = a;
+    a = b;
+    b = x;
+  }
+  return a;
+}
+
+// Order-preserving integer key of a non-NaN double (-0 and +0 share a key).
+__device__ inline unsigned long long f64_key(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x == 0.0 ? 0.0 : x);
+  return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+
+// Dense ranks of any number of non-NaN values val(i), i < n: sort a copy of
+// their order-preserving keys, compact the distinct ones, binary-search each
+// value.  keys: 2n words of global scratch; cnt: NW * 256 ints of LDS.
+template <class F>
+__device__ void blk_dense_rank_big(F val, int64_t n, int32_t* rank, unsigned long long* keys, int32_t* cnt,
+                                   SelScratch& sh) {
+  unsigned long long* ka = keys;
+  unsigned long long* kb = keys + n;
+  for (int64_t i = threadIdx.x; i < n; i += FB) ka[i] = f64_key(val(i));
+  __syncthreads();
+  unsigned long long* s = blk_radix_sort(ka, kb, n, cnt, sh);
+  unsigned long long* u = s == ka ? kb : ka;
+  const int64_t per = (n + FB - 1) / FB;
+  const int64_t i0 = threadIdx.x * per < n ? threadIdx.x * per : n;
+  const int64_t i1 = i0 + per < n ? i0 + per : n;
+  int64_t local = 0;
+  for (int64_t i = i0; i < i1; i++) local += (i == 0 || s[i] != s[i - 1]);
+  int64_t nd;
+  int64_t off = blk_excl_scan(local, &nd, sh);
+  for (int64_t i = i0; i < i1; i++)
+    if (i == 0 || s[i] != s[i - 1]) u[off++] = s[i];
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < n; i += FB) {
+    const unsigned long long v = f64_key(val(i));
+    int64_t lo = 0, hi = nd;  // first distinct value >= v
+    while (lo < hi) { int64_t mid = (lo + hi) >> 1; if (u[mid] < v) lo = mid + 1; else hi = mid; }
+    rank[i] = (int32_t)(lo + 1);
+  }
+  __syncthreads();
+}
+
 // Dense ranks (ties share a rank, taumodel.jl:21-31) of vals[0..n).  When all
 // values are equal every rank is 1 and *constant is set (rank is not
-// written).  Returns false for more than IQ_SORT_MAX values with several
-// distinct ones.
+// written).  Up to IQ_SORT_MAX values are ranked in LDS (sv, dp); more go
+// through blk_dense_rank_big with the global scratch `keys` (2n words; the
+// LDS of sv then holds the radix counts).  keys == nullptr: false for more
+// than IQ_SORT_MAX values with several distinct ones.
 __device__ bool blk_dense_rank(const double* vals, int64_t n, int32_t* rank, bool* constant, double* sv,
-                               int32_t* dp, SelScratch& sh) {
+                               int32_t* dp, SelScratch& sh, unsigned long long* keys = nullptr) {
   double lmin = __builtin_inf(), lmax = -__builtin_inf();
   for (int64_t i = threadIdx.x; i < n; i += FB) { lmin = fmin(lmin, vals[i]); lmax = fmax(lmax, vals[i]); }
   double mn = blk_min_f64(lmin, sh), mx = blk_max_f64(lmax, sh);
   *constant = mn == mx;
   if (*constant) return true;
-  if (n > IQ_SORT_MAX) return false;
+  if (n > IQ_SORT_MAX) {
+    if (!keys) return false;
+    blk_dense_rank_big([&](int64_t i) { return vals[i]; }, n, rank, keys, (int32_t*)sv, sh);
+    return true;
+  }
   int m = 1;
   while (m < n) m <<= 1;
   for (int i = threadIdx.x; i < m; i += FB) sv[i] = i < n ? vals[i] : __builtin_inf();
@@ -268,11 +393,12 @@ __device__ void publish_pick(const int32_t* db, int64_t n, const double* p, doub
 }
 
 // Tau weights of n events over nsrc sources (values of source j at event i are
-// src_vals(j, i), staged into vals) -> p (or constant).  False on overflow of
-// the LDS ranker.
+// src_vals(j, i), staged into vals) -> p (or constant).  rank and p hold n
+// entries; keys (2n words, or nullptr) is the scratch of the large-n ranker.
+// False when n exceeds the LDS ranker and no keys are given.
 template <class F>
 __device__ bool tau_weights(int64_t n, int nsrc, F src_vals, double* vals, int32_t* rank, double* p, bool* allconst,
-                            double* sv, int32_t* dp, SelScratch& sh) {
+                            double* sv, int32_t* dp, SelScratch& sh, unsigned long long* keys = nullptr) {
   *allconst = true;
   if (n == 1) return true;
   bool started = false;
@@ -280,9 +406,8 @@ __device__ bool tau_weights(int64_t n, int nsrc, F src_vals, double* vals, int32
     for (int64_t i = threadIdx.x; i < n; i += FB) vals[i] = src_vals(j, i);
     __syncthreads();
     bool c;
-    if (!blk_dense_rank(vals, n, rank, &c, sv, dp, sh)) return false;
+    if (!blk_dense_rank(vals, n, rank, &c, sv, dp, sh, keys)) return false;
     if (!c) {
-      if (n > IQ_SORT_MAX) return false;
       if (!started) {
         for (int64_t i = threadIdx.x; i < n; i += FB) p[i] = 1.0;
         __syncthreads();
@@ -377,17 +502,24 @@ k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     publish_pick(band, nb, nullptr, tau_constant_weight(nb), q.u, q.forced, s1, s2, o, sh);
     return;
   }
-  if (!small) {
-    if (threadIdx.x == 0) { set_err(D->err, IQ_KERR_SORTMAX); *o.pick = -2; }
-    return;
+  double* p = sd;
+  int32_t* rk = si;
+  if (small) {
+    bool c;
+    blk_dense_rank(sd, nb, si, &c, sv, dp, sh);
+  } else {
+    // large band: radix-sorted ranks; ranks go to the (consumed) superset
+    // list, weights over the key scratch once the ranks are known
+    unsigned long long* keys = D->rkey + slot * 2 * g.P;
+    rk = D->sup + slot * g.P;
+    blk_dense_rank_big([&](int64_t i) { return dexact[band[i]]; }, nb, rk, keys, (int32_t*)sv, sh);
+    p = (double*)keys;
   }
-  bool c;
-  blk_dense_rank(sd, nb, si, &c, sv, dp, sh);
-  for (int64_t i = threadIdx.x; i < nb; i += FB) sd[i] = 1.0;  // sd now holds the weights
+  for (int64_t i = threadIdx.x; i < nb; i += FB) p[i] = 1.0;
   __syncthreads();
-  tau_accumulate(nb, si, false, sd, sh);
-  tau_finish(nb, sd);
-  publish_pick(band, nb, sd, 0.0, q.u, q.forced, s1, s2, o, sh);
+  tau_accumulate(nb, rk, false, p, sh);
+  tau_finish(nb, p);
+  publish_pick(band, nb, p, 0.0, q.u, q.forced, s1, s2, o, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -511,11 +643,12 @@ k_relax_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     return;
   }
   double* vals = D->vals + slot * g.P;
-  double* p = D->sd + slot * IQ_SORT_MAX;
-  int32_t* si = D->si + slot * IQ_SORT_MAX;
+  const bool small = nev <= IQ_SORT_MAX;  // else ranks and weights in P-sized scratch
+  double* p = small ? D->sd + slot * IQ_SORT_MAX : D->dexact + slot * g.P;
+  int32_t* si = small ? D->si + slot * IQ_SORT_MAX : D->sup + slot * g.P;
   bool allconst;
   auto srcv = [&](int j, int64_t i) { return mp[j][ev[i]]; };
-  if (!tau_weights(nev, nsrc, srcv, vals, si, p, &allconst, sv, dp, sh)) {
+  if (!tau_weights(nev, nsrc, srcv, vals, si, p, &allconst, sv, dp, sh, D->rkey + slot * 2 * g.P)) {
     if (threadIdx.x == 0) { set_err(D->err, IQ_KERR_SORTMAX); *o.pick = -2; }
     return;
   }
@@ -562,7 +695,7 @@ k_relax_host(int64_t P, int nsrc, const double* const* __restrict__ maps, const 
 extern "C" __global__ void __launch_bounds__(FB)
 k_taumodel_host(const int32_t* __restrict__ ev, int64_t nev, int nsrc, const double* const* __restrict__ maps,
                 double* __restrict__ vals, int32_t* __restrict__ rank, double* __restrict__ p,
-                int32_t* __restrict__ err) {
+                unsigned long long* __restrict__ keys, int32_t* __restrict__ err) {
   __shared__ SelScratch sh;
   __shared__ double sv[IQ_SORT_MAX];
   __shared__ int32_t dp[IQ_SORT_MAX];
@@ -573,7 +706,7 @@ k_taumodel_host(const int32_t* __restrict__ ev, int64_t nev, int nsrc, const dou
     for (int64_t i = threadIdx.x; i < nev; i += FB) vals[i] = maps[j][ev[i]];
     __syncthreads();
     bool c;
-    if (!blk_dense_rank(vals, nev, rank, &c, sv, dp, sh)) {
+    if (!blk_dense_rank(vals, nev, rank, &c, sv, dp, sh, keys)) {
       if (threadIdx.x == 0) set_err(err, IQ_KERR_SORTMAX);
       return;
     }

@@ -98,7 +98,7 @@ def load(path: str = LIB_PATH):
             "iq_plan_results": [vp, vp, vp, vp, vp],
             "iq_plan_last_time": [vp, P(f64)],
             "iq_plan_profile": [vp, c_int],
-            "iq_plan_scan_stats": [vp, vp, vp, vp],
+            "iq_plan_scan_stats": [vp, vp, vp, vp, vp],
             "iq_plan_cut_stats": [vp, vp, vp, vp],
             "iq_simulate": [vp, vp, vp, vp, vp, vp, vp],
         }

@@ -200,11 +200,13 @@ class Plan:
         self.ctx.check(self.ctx._L.iq_plan_profile(self.handle, int(bool(enable))))
 
     def scan_stats(self):
-        """(launches, seconds, flop) of the FP32 scan during the last run."""
+        """(launches, seconds, algorithmic flop, issued MFMA flop) of the
+        distance scan during the last run (profiling enabled)."""
         n = np.zeros(1, dtype=np.int64)
-        t = np.zeros(2, dtype=np.float64)
-        self.ctx.check(self.ctx._L.iq_plan_scan_stats(self.handle, _ptr(n), t[0:].ctypes.data, t[1:].ctypes.data))
-        return int(n[0]), float(t[0]), float(t[1])
+        t = np.zeros(3, dtype=np.float64)
+        self.ctx.check(self.ctx._L.iq_plan_scan_stats(self.handle, _ptr(n), t[0:].ctypes.data, t[1:].ctypes.data,
+                                                      t[2:].ctypes.data))
+        return int(n[0]), float(t[0]), float(t[1]), float(t[2])
 
     def cut_stats(self):
         """(push rounds, relabel BFS levels, cuts) of the last run."""

@@ -139,13 +139,14 @@ int iq_plan_results(iq_ctx* ctx, iq_plan* plan, void* out, int64_t* chosen, doub
  * context's stream). */
 int iq_plan_last_time(iq_plan* plan, double* seconds);
 
-/* Live timing of the FP32 distance scan (the dominant kernel) inside
+/* Live timing of the distance scan (the dominant kernel) inside
  * iq_plan_run: when enabled, every scan launch is bracketed by HIP events on
  * the context's stream.  iq_plan_scan_stats reports the launches of the last
- * run, their summed device time and their algorithmic FLOPs (2 * candidates *
- * masked voxels per launch).  Diagnostic only; off by default. */
+ * run, their summed device time, their algorithmic FLOPs (2 * candidates *
+ * masked voxels per query) and the FLOPs of the bf16 matrix instructions the
+ * MFMA scan issues (0 for the VALU scan).  Diagnostic only; off by default. */
 int iq_plan_profile(iq_plan* plan, int enable);
-int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double* flop);
+int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double* flop, double* mfma_flop);
 
 /* Min-cut work of the last iq_plan_run (graphcut.jl:73 replacement):
  * push-relabel rounds, global-relabel BFS levels and cuts (register-resident

@@ -653,6 +653,7 @@ typedef struct {
   int64_t* chosen;           /* nreal x npath, -1 for skipped */
   double* cutgrid;           /* nreal x padsize or NULL (debug) */
   int64_t* cut_iters;        /* total push-relabel rounds, or NULL */
+  int64_t* nband;            /* nreal x npath pattern-database sizes (0 for skipped), or NULL */
 } ora_sim_args;
 
 static void tile_offsets(const uint8_t* m, const int64_t ts[3], int64_t* jx, int64_t* jy, int64_t* jz, int64_t* V) {
@@ -787,6 +788,7 @@ int ora_simulate(const ora_sim_args* a) {
       }
       int64_t rind = db[pick];
       a->chosen[r * a->npath + sidx] = rind;
+      if (a->nband) a->nband[r * a->npath + sidx] = ndb;
       int64_t cs[3] = {rind % a->distsize[0], (rind / a->distsize[0]) % a->distsize[1], rind / (a->distsize[0] * a->distsize[1])};
 
       /* boundary cut mask (iqsim.jl:252-275) */

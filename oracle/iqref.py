@@ -386,7 +386,7 @@ def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None,
             ("auxti", ctypes.c_void_p), ("tol", ctypes.c_double), ("nreal", ctypes.c_int64),
             ("uniforms", ctypes.c_void_p), ("forced", ctypes.c_void_p), ("gr_freq", ctypes.c_int),
             ("out", ctypes.c_void_p), ("chosen", ctypes.c_void_p), ("cutgrid", ctypes.c_void_p),
-            ("cut_iters", ctypes.c_void_p),
+            ("cut_iters", ctypes.c_void_p), ("nband", ctypes.c_void_p),
         ]
 
     a = Args()
@@ -412,6 +412,8 @@ def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None,
     a.chosen = chosen.ctypes.data
     a.cutgrid = None if cut is None else cut.ctypes.data
     a.cut_iters = cut_iters.ctypes.data
+    nband = np.zeros(nreal * npath, dtype=np.int64)
+    a.nband = nband.ctypes.data
     rc = lib().ora_simulate(ctypes.byref(a))
     if rc != ORA_OK:
         raise RuntimeError(f"oracle simulate failed: {rc}")
@@ -432,7 +434,7 @@ def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None,
         if debug:
             cuts.append(cg[crop])
     trace = dict(chosen=chosen.reshape(nreal, npath), path=simpath, geo=geo, uniforms=uniforms,
-                 cut_iters=int(cut_iters[0]))
+                 cut_iters=int(cut_iters[0]), nband=nband.reshape(nreal, npath))
     res = (reals, cuts, voxs) if debug else reals
     return (res, trace) if return_trace else res
 

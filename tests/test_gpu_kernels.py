@@ -99,6 +99,22 @@ def test_taumodel_matches_oracle_bitwise(ctx):
         assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), trial
 
 
+def test_taumodel_large_event_sets_bitwise(ctx):
+    # more events than the LDS ranker holds: radix-sorted dense ranks
+    rng = np.random.default_rng(18)
+    n = 60000
+    D = rng.random(n) * 5.0                          # distinct values
+    T = np.round(rng.random(n) * 30) - 10.0          # ties, negatives
+    T[rng.choice(n, 50, replace=False)] = -0.0       # -0 and +0 tie
+    T[rng.choice(n, 50, replace=False)] = 0.0
+    C = np.full(n, 2.5)                              # constant source: neutral
+    for ev in (np.arange(n), np.sort(rng.choice(n, 9000, replace=False))):
+        for Ds in ([], [T], [T, C, D[::-1].copy()]):
+            got = iqhip.taumodel(ev, D, Ds, ctx=ctx)
+            ref = iqref.taumodel(ev, D, Ds)
+            assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), (len(ev), len(Ds))
+
+
 # ------------------------------------------------------------------ sample
 def test_sample_matches_oracle(ctx):
     rng = np.random.default_rng(9)

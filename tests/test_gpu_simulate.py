@@ -40,6 +40,7 @@ def _compare(T, tile, sim, seed=0, **kw):
     for a, b in zip(cuts, rc):
         assert np.array_equal(a, b, equal_nan=True)
     assert voxs == rv
+    tr["nband"] = rtr["nband"]
     return reals, tr
 
 
@@ -125,3 +126,36 @@ def test_sim_valu_scan_path(monkeypatch):
     T = _smooth(rng, (30, 30, 12), 5, np.float32)
     print("valu scan", flush=True)
     _compare(T, (10, 10, 4), (40, 40, 12), overlap=(0.25, 0.25, 0.25))
+
+
+def test_sim_large_tied_band():
+    # binary noise, random path (overlaps on several sides from unrelated
+    # windows): bands of thousands of candidates over a few distinct integer
+    # distances (radix-ranked beyond the LDS ranker)
+    rng = np.random.default_rng(11)
+    T = (rng.random((140, 140)) > 0.5).astype(np.float64)
+    print("large tied band", flush=True)
+    _, tr = _compare(T, (10, 10), (46, 46), overlap=(0.3, 0.3), tol=1.0, path="random")
+    nb = tr["nband"]
+    assert ((nb > 4096) & (nb < 131 * 131)).any()
+
+
+def test_sim_large_continuous_band():
+    rng = np.random.default_rng(12)
+    T = rng.standard_normal((120, 120)).astype(np.float32)
+    print("large continuous band", flush=True)
+    _, tr = _compare(T, (10, 10), (46, 46), overlap=(0.3, 0.3), tol=1.0, path="random")
+    nb = tr["nband"]
+    assert ((nb > 4096) & (nb < 111 * 111)).any()
+
+
+def test_sim_soft_large_database():
+    # first tile: D == 0, the database is the first tenth of the soft order
+    # (> 4096 events with distinct soft distances)
+    rng = np.random.default_rng(13)
+    T = _smooth(rng, (230, 230), 5, np.float32)
+    aux_ti = _smooth(rng, (230, 230), 3, np.float32)
+    aux = _smooth(rng, (40, 40), 3, np.float32)
+    print("soft large database", flush=True)
+    _, tr = _compare(T, (10, 10), (40, 40), soft=[(aux, aux_ti)], tol=0.1)
+    assert (tr["nband"] > 4096).any()
