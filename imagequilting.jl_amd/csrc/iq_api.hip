@@ -266,17 +266,23 @@ bool fits_range(const float* v, int64_t n) {
   return true;
 }
 
-void geo_init(IqGeo& g, const int64_t ti[3], const int64_t tile[3], const int64_t pad[3]) {
+void geo_init(IqGeo& g, const int64_t ti[3], const int64_t tile[3]) {
   for (int d = 0; d < 3; d++) {
     g.ti[d] = (int32_t)ti[d];
     g.tile[d] = (int32_t)tile[d];
-    g.pad[d] = (int32_t)pad[d];
     g.dist[d] = (int32_t)(ti[d] - tile[d] + 1);
   }
   g.P = (int64_t)g.dist[0] * g.dist[1] * g.dist[2];
-  g.padvox = (int64_t)g.pad[0] * g.pad[1] * g.pad[2];
   g.tilevox = g.tile[0] * g.tile[1] * g.tile[2];
 }
+
+// One simulation grid of a plan (iqsim.jl:99-115) and its realizations.
+struct HostGrid {
+  int64_t ntiles[3], pad[3], padvox = 0, npath = 0, nreal = 0;
+  int64_t step0 = 0;  // first entry of its steps
+  int64_t real0 = 0;  // first realization
+  std::vector<std::vector<int64_t>> waves;  // path positions per wave (dependency order)
+};
 
 }  // namespace
 
@@ -287,8 +293,12 @@ struct iq_plan {
   iq_ctx* ctx = nullptr;
   int dtype = IQ_F64;
   IqGeo geo{};
-  int64_t ovl[3], spacing[3], ntiles[3];
-  int64_t npath = 0, nreal = 0, nen = 0;
+  int64_t ovl[3], spacing[3];
+  int64_t nreal = 0, nen = 0;  // realizations over all grids; enabled candidates
+  int64_t nlog = 0, nvox = 0;  // pick-log entries and grid voxels over all realizations
+  std::vector<HostGrid> grids;
+  std::vector<IqReal> reals;
+  size_t nwaves = 0;           // lockstep waves (the longest grid's)
   double tol = 0.1;
   int naux = 0;
   bool fast = false;
@@ -296,7 +306,6 @@ struct iq_plan {
   bool any_multi = false;
   std::vector<IqStep> steps;
   std::vector<IqMask> masks;
-  std::vector<std::vector<int64_t>> waves;  // path positions per wave (dependency order)
   int scan_cfg = -1;
   size_t scan_lds = 0;
   bool scan_perz = false;
@@ -338,27 +347,29 @@ namespace {
 // wave never overlap, every neighbour pasted before a tile in path order is
 // pasted in an earlier wave and none after it is, so masks, windows and cuts
 // are exactly those of the sequential loop.
-std::vector<std::vector<int64_t>> build_waves(const iq_plan* pl, const int64_t* path, bool sequential) {
+std::vector<std::vector<int64_t>> build_waves(const HostGrid& G, const IqStep* steps, const int64_t* path,
+                                              bool sequential) {
   std::vector<std::vector<int64_t>> waves;
-  const int64_t nt = pl->ntiles[0] * pl->ntiles[1] * pl->ntiles[2];
+  const int64_t* nt3 = G.ntiles;
+  const int64_t nt = nt3[0] * nt3[1] * nt3[2];
   std::vector<int64_t> wave_of((size_t)nt, -1);
   int64_t last = -1;
-  for (int64_t i = 0; i < pl->npath; i++) {
-    if (!pl->steps[i].active) continue;
+  for (int64_t i = 0; i < G.npath; i++) {
+    if (!steps[i].active) continue;
     int64_t w;
     if (sequential) {
       w = ++last;
     } else {
       const int64_t ind = path[i];
-      int64_t c[3] = {ind % pl->ntiles[0], (ind / pl->ntiles[0]) % pl->ntiles[1], ind / (pl->ntiles[0] * pl->ntiles[1])};
+      int64_t c[3] = {ind % nt3[0], (ind / nt3[0]) % nt3[1], ind / (nt3[0] * nt3[1])};
       w = 0;
       for (int dz = -1; dz <= 1; dz++)
         for (int dy = -1; dy <= 1; dy++)
           for (int dx = -1; dx <= 1; dx++) {
             int64_t nb[3] = {c[0] + dx, c[1] + dy, c[2] + dz};
-            if (nb[0] < 0 || nb[1] < 0 || nb[2] < 0 || nb[0] >= pl->ntiles[0] || nb[1] >= pl->ntiles[1] || nb[2] >= pl->ntiles[2])
+            if (nb[0] < 0 || nb[1] < 0 || nb[2] < 0 || nb[0] >= nt3[0] || nb[1] >= nt3[1] || nb[2] >= nt3[2])
               continue;
-            int64_t wi = wave_of[nb[0] + pl->ntiles[0] * (nb[1] + pl->ntiles[1] * nb[2])];
+            int64_t wi = wave_of[nb[0] + nt3[0] * (nb[1] + nt3[1] * nb[2])];
             if (wi >= 0) w = std::max(w, wi + 1);
           }
       wave_of[ind] = w;
@@ -369,17 +380,16 @@ std::vector<std::vector<int64_t>> build_waves(const iq_plan* pl, const int64_t* 
   return waves;
 }
 
-void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
-  if (!d || !d->ti || !d->disabled || !d->path || !d->skipped) bad(IQ_ERR_ARG, "null plan input");
+void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) {
+  if (!descs || ngrid <= 0) bad(IQ_ERR_ARG, "no simulation grid");
+  const iq_plan_desc* d = &descs[0];  // training image, tile geometry, tol and soft pairs of every grid
+  if (!d->ti || !d->disabled) bad(IQ_ERR_ARG, "null plan input");
   if (d->ti_dtype != IQ_F32 && d->ti_dtype != IQ_F64) bad(IQ_ERR_ARG, "ti_dtype must be IQ_F32 or IQ_F64");
-  if (d->nreal <= 0) bad(IQ_ERR_ARG, "invalid number of realizations");
   if (!(d->tol > 0.0 && d->tol <= 1.0)) bad(IQ_ERR_ARG, "tolerance must be in range (0,1]");
   if (d->naux < 0 || d->naux > IQ_MAX_AUX) bad(IQ_ERR_ARG, "naux must be in 0..8");
   hipStream_t s = ctx->stream;
   pl->ctx = ctx;
   pl->dtype = d->ti_dtype;
-  pl->npath = d->npath;
-  pl->nreal = d->nreal;
   pl->tol = d->tol;
   pl->naux = d->naux;
   bool half = true;
@@ -387,37 +397,66 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
     if (d->tilesize[k] <= 0 || d->tilesize[k] > d->tisize[k]) bad(IQ_ERR_ARG, "invalid tile size");
     if (d->ovlsize[k] < 0 || d->ovlsize[k] > d->tilesize[k]) bad(IQ_ERR_ARG, "invalid overlap size");
     if (d->spacing[k] != d->tilesize[k] - d->ovlsize[k]) bad(IQ_ERR_ARG, "spacing must be tilesize - ovlsize");
-    if (d->ntiles[k] <= 0) bad(IQ_ERR_ARG, "invalid ntiles");
-    if (d->padsize[k] != d->ntiles[k] * d->spacing[k] + d->ovlsize[k]) bad(IQ_ERR_ARG, "padsize must be ntiles*(tilesize-ovlsize)+ovlsize");
     pl->ovl[k] = d->ovlsize[k];
     pl->spacing[k] = d->spacing[k];
-    pl->ntiles[k] = d->ntiles[k];
     if (2 * d->ovlsize[k] > d->tilesize[k] || d->spacing[k] < 1) half = false;
   }
-  geo_init(pl->geo, d->tisize, d->tilesize, d->padsize);
+  geo_init(pl->geo, d->tisize, d->tilesize);
   IqGeo& g = pl->geo;
-  if (g.P > (int64_t)S2MAX * IQ_SB * IQ_SB || g.padvox >= (1ll << 40))
+  if (g.P > (int64_t)S2MAX * IQ_SB * IQ_SB)
     bad(IQ_ERR_UNSUPPORTED, "more than " + std::to_string((int64_t)S2MAX * IQ_SB * IQ_SB) + " candidate patterns");
   const int64_t tivox = (int64_t)g.ti[0] * g.ti[1] * g.ti[2];
-  const int64_t nt = pl->ntiles[0] * pl->ntiles[1] * pl->ntiles[2];
-  for (int64_t i = 0; i < d->npath; i++)
-    if (d->path[i] < 0 || d->path[i] >= nt) bad(IQ_ERR_ARG, "path entry outside the tile grid");
   const int t3[3] = {g.tile[0], g.tile[1], g.tile[2]};
+  for (int gi = 0; gi < ngrid; gi++) {
+    const iq_plan_desc* e = &descs[gi];
+    if (!e->path || !e->skipped) bad(IQ_ERR_ARG, "null plan input");
+    if (e->nreal <= 0) bad(IQ_ERR_ARG, "invalid number of realizations");
+    if (e->ti_dtype != d->ti_dtype || e->naux != d->naux || e->tol != d->tol)
+      bad(IQ_ERR_ARG, "grids of one plan share the training image, tol and soft pairs");
+    HostGrid G;
+    for (int k = 0; k < 3; k++) {
+      if (e->tisize[k] != d->tisize[k] || e->tilesize[k] != d->tilesize[k] || e->ovlsize[k] != d->ovlsize[k] ||
+          e->spacing[k] != d->spacing[k])
+        bad(IQ_ERR_ARG, "grids of one plan share the training image and tile geometry");
+      if (e->ntiles[k] <= 0) bad(IQ_ERR_ARG, "invalid ntiles");
+      if (e->padsize[k] != e->ntiles[k] * e->spacing[k] + e->ovlsize[k]) bad(IQ_ERR_ARG, "padsize must be ntiles*(tilesize-ovlsize)+ovlsize");
+      G.ntiles[k] = e->ntiles[k];
+      G.pad[k] = e->padsize[k];
+    }
+    G.padvox = G.pad[0] * G.pad[1] * G.pad[2];
+    if (G.pad[0] >= (1ll << 31) || G.pad[1] >= (1ll << 31) || G.pad[2] >= (1ll << 31) || G.padvox >= (1ll << 40))
+      bad(IQ_ERR_UNSUPPORTED, "simulation grid too large");
+    G.npath = e->npath;
+    G.nreal = e->nreal;
+    const int64_t nt = G.ntiles[0] * G.ntiles[1] * G.ntiles[2];
+    for (int64_t i = 0; i < e->npath; i++)
+      if (e->path[i] < 0 || e->path[i] >= nt) bad(IQ_ERR_ARG, "path entry outside the tile grid");
+    G.real0 = pl->nreal;
+    pl->nreal += G.nreal;
+    pl->grids.push_back(G);
+  }
 
-  // ---- steps and masks (iqsim.jl:172-205, 252-275), shared by all realizations
-  std::vector<uint8_t> pasted((size_t)nt, 0);
+  // ---- steps and masks (iqsim.jl:172-205, 252-275): per grid, shared by its
+  // realizations; mask shapes shared by all grids
   std::map<std::vector<uint8_t>, int> mask_index;
   std::vector<HostMask> hmasks;
   std::vector<int32_t> htvox;
   std::vector<int64_t> mask_uses;
-  pl->hard = d->hardgrid != nullptr;
-  for (int64_t i = 0; i < d->npath; i++) {
+  pl->hard = false;
+  for (int gi = 0; gi < ngrid; gi++) {
+  const iq_plan_desc* e = &descs[gi];
+  HostGrid& G = pl->grids[gi];
+  G.step0 = (int64_t)pl->steps.size();
+  if (e->hardgrid) pl->hard = true;
+  const int64_t nt = G.ntiles[0] * G.ntiles[1] * G.ntiles[2];
+  std::vector<uint8_t> pasted((size_t)nt, 0);
+  for (int64_t i = 0; i < e->npath; i++) {
     IqStep st{};
-    int64_t ind = d->path[i];
+    int64_t ind = e->path[i];
     if (pasted[ind]) bad(IQ_ERR_ARG, "path visits a tile twice");
-    if (d->skipped[ind]) { st.active = 0; st.mask = -1; pasted[ind] = 0; pl->steps.push_back(st); continue; }
+    if (e->skipped[ind]) { st.active = 0; st.mask = -1; pasted[ind] = 0; pl->steps.push_back(st); continue; }
     st.active = 1;
-    int64_t ti3[3] = {ind % pl->ntiles[0], (ind / pl->ntiles[0]) % pl->ntiles[1], ind / (pl->ntiles[0] * pl->ntiles[1])};
+    int64_t ti3[3] = {ind % G.ntiles[0], (ind / G.ntiles[0]) % G.ntiles[1], ind / (G.ntiles[0] * G.ntiles[1])};
     for (int k = 0; k < 3; k++) st.origin[k] = (int32_t)(ti3[k] * pl->spacing[k]);
     std::vector<uint8_t> bits((size_t)g.tilevox, 0);
     int any = 0;
@@ -426,8 +465,8 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
       for (int side = 0; side < 2; side++) {
         int64_t nb[3] = {ti3[0], ti3[1], ti3[2]};
         nb[k] += side == 0 ? -1 : 1;
-        if (nb[k] < 0 || nb[k] >= pl->ntiles[k]) continue;
-        if (!pasted[nb[0] + pl->ntiles[0] * (nb[1] + pl->ntiles[1] * nb[2])]) continue;
+        if (nb[k] < 0 || nb[k] >= G.ntiles[k]) continue;
+        if (!pasted[nb[0] + G.ntiles[0] * (nb[1] + G.ntiles[1] * nb[2])]) continue;
         IqSlab sl{};
         sl.dim = k;
         sl.side = side;
@@ -470,12 +509,12 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
       mask_uses[st.mask]++;
     }
     // hard data in the tile (iqsim.jl:210-219, indicator!/event! utils.jl:18-36)
-    if (d->hardgrid) {
+    if (e->hardgrid) {
       st.hoff = (int32_t)htvox.size();
       for (int z = 0; z < t3[2]; z++)
         for (int y = 0; y < t3[1]; y++)
           for (int x = 0; x < t3[0]; x++) {
-            double v = d->hardgrid[iq_lin3(g.pad, st.origin[0] + x, st.origin[1] + y, st.origin[2] + z)];
+            double v = e->hardgrid[st.origin[0] + x + G.pad[0] * (st.origin[1] + y + G.pad[1] * (int64_t)(st.origin[2] + z))];
             if (!std::isnan(v)) htvox.push_back(x + t3[0] * (y + t3[1] * z));
           }
       st.hcnt = (int32_t)htvox.size() - st.hoff;
@@ -485,7 +524,9 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
     pasted[ind] = 1;
     pl->steps.push_back(st);
   }
-  pl->waves = build_waves(pl, d->path, !half);
+  G.waves = build_waves(G, pl->steps.data() + G.step0, e->path, !half);
+  pl->nwaves = std::max(pl->nwaves, G.waves.size());
+  }
 
   // ---- offset tables: per mask, tile-linear and TI-linear offsets (ascending column-major)
   std::vector<int32_t> tvox, toff;
@@ -644,14 +685,12 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
     }
   }
 
-  // ---- hard and soft data
-  if (pl->hard) H.hardgrid = pl->mem.upload(d->hardgrid, (size_t)g.padvox, s);
-  std::vector<const double*> auxp, auxtip, a2sp;
+  // ---- hard and soft data: grids per simulation grid, soft training images shared
+  std::vector<const double*> auxtip, a2sp;
   if (pl->naux > 0) {
-    if (!d->aux || !d->auxti) bad(IQ_ERR_ARG, "null soft data");
+    if (!d->auxti) bad(IQ_ERR_ARG, "null soft data");
     for (int a = 0; a < pl->naux; a++) {
-      if (!d->aux[a] || !d->auxti[a]) bad(IQ_ERR_ARG, "null soft data");
-      auxp.push_back(pl->mem.upload(d->aux[a], (size_t)g.padvox, s));
+      if (!d->auxti[a]) bad(IQ_ERR_ARG, "null soft data");
       double* at = pl->mem.upload(d->auxti[a], (size_t)tivox, s);
       auxtip.push_back(at);
       double* a2 = pl->mem.alloc<double>(g.P);
@@ -660,10 +699,37 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
       HCL();
       a2sp.push_back(a2);
     }
-    H.aux = (const double* const*)pl->mem.upload(auxp.data(), auxp.size(), s);
     H.auxti = (const double* const*)pl->mem.upload(auxtip.data(), auxtip.size(), s);
     H.a2soft = (const double* const*)pl->mem.upload(a2sp.data(), a2sp.size(), s);
   }
+  std::vector<std::vector<const double*>> gaux((size_t)ngrid);  // alive until the final synchronize
+  int64_t sim_off = 0, log_off = 0;
+  for (int gi = 0; gi < ngrid; gi++) {
+    const iq_plan_desc* e = &descs[gi];
+    const HostGrid& G = pl->grids[gi];
+    IqReal R{};
+    for (int k = 0; k < 3; k++) R.pad[k] = (int32_t)G.pad[k];
+    R.step0 = (int32_t)G.step0;
+    if (e->hardgrid) R.hard = pl->mem.upload(e->hardgrid, (size_t)G.padvox, s);
+    if (pl->naux > 0) {
+      if (!e->aux) bad(IQ_ERR_ARG, "null soft data");
+      for (int a = 0; a < pl->naux; a++) {
+        if (!e->aux[a]) bad(IQ_ERR_ARG, "null soft data");
+        gaux[gi].push_back(pl->mem.upload(e->aux[a], (size_t)G.padvox, s));
+      }
+      R.aux = (const double* const*)pl->mem.upload(gaux[gi].data(), gaux[gi].size(), s);
+    }
+    for (int64_t r = 0; r < G.nreal; r++) {
+      R.sim_off = sim_off;
+      R.log_off = log_off;
+      pl->reals.push_back(R);
+      sim_off += G.padvox;
+      log_off += G.npath;
+    }
+  }
+  pl->nvox = sim_off;
+  pl->nlog = log_off;
+  H.reals = pl->mem.upload(pl->reals.data(), pl->reals.size(), s);
 
   // ---- per-slot scratch: as many slots as the widest batch, within a memory budget
   const int64_t P = g.P;
@@ -685,7 +751,12 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
                     (size_t)IQ_MAX_SLAB * cutper;
   if (pl->any_multi) per_slot += (size_t)nmap * P * 12 + (size_t)nmap * nw32 * 4 + (size_t)P * 12;
   int64_t widest = 1;
-  for (auto& w : pl->waves) widest = std::max<int64_t>(widest, (int64_t)w.size() * pl->nreal);
+  for (size_t k = 0; k < pl->nwaves; k++) {
+    int64_t width = 0;  // lockstep wave k over every grid and realization
+    for (const HostGrid& G : pl->grids)
+      if (k < G.waves.size()) width += (int64_t)G.waves[k].size() * G.nreal;
+    widest = std::max(widest, width);
+  }
   const double budget = 24.0 * (1ull << 30);
   int64_t cap = std::max<int64_t>(1, (int64_t)(budget / (double)per_slot));
   cap = std::min<int64_t>(cap, 65535 / std::max(1, pl->nbz));
@@ -745,10 +816,9 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* d, iq_plan* pl) {
 
   // ---- results
   const size_t esz = pl->dtype == IQ_F32 ? 4 : 8;
-  H.sims = pl->mem.alloc<char>((size_t)pl->nreal * g.padvox * esz);
-  H.cuts = pl->mem.alloc<uint8_t>((size_t)pl->nreal * g.padvox);
-  H.log = pl->mem.alloc<int64_t>((size_t)pl->nreal * std::max<int64_t>(pl->npath, 1));
-  H.npath = pl->npath;
+  H.sims = pl->mem.alloc<char>((size_t)pl->nvox * esz);
+  H.cuts = pl->mem.alloc<uint8_t>((size_t)pl->nvox);
+  H.log = pl->mem.alloc<int64_t>((size_t)std::max<int64_t>(pl->nlog, 1));
   pl->dd = pl->mem.upload(&pl->hd, 1, s);
   HC(hipEventCreate(&pl->e0));
   HC(hipEventCreate(&pl->e1));
@@ -876,18 +946,23 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   std::vector<IqQuery> qs;
   struct Batch { int64_t off; int n; bool plain, multi; };
   std::vector<Batch> batches;
-  for (const auto& wave : pl->waves) {
+  for (size_t k = 0; k < pl->nwaves; k++) {
+    // lockstep: wave k of every grid, all realizations
     std::vector<IqQuery> wq;
-    for (int64_t r = 0; r < pl->nreal; r++)
-      for (int64_t i : wave) {
-        IqQuery q{};
-        q.step = (int32_t)i;
-        q.real = (int32_t)r;
-        q.multi = (pl->steps[i].hard || pl->naux > 0) ? 1 : 0;
-        q.u = uniforms ? uniforms[r * pl->npath + i] : 0.5;
-        q.forced = forced ? forced[r * pl->npath + i] : -1;
-        wq.push_back(q);
-      }
+    for (const HostGrid& G : pl->grids) {
+      if (k >= G.waves.size()) continue;
+      for (int64_t r = G.real0; r < G.real0 + G.nreal; r++)
+        for (int64_t i : G.waves[k]) {
+          const int64_t st = G.step0 + i, lo = pl->reals[r].log_off + i;
+          IqQuery q{};
+          q.step = (int32_t)st;
+          q.real = (int32_t)r;
+          q.multi = (pl->steps[st].hard || pl->naux > 0) ? 1 : 0;
+          q.u = uniforms ? uniforms[lo] : 0.5;
+          q.forced = forced ? forced[lo] : -1;
+          wq.push_back(q);
+        }
+    }
     for (size_t b0 = 0; b0 < wq.size(); b0 += (size_t)pl->nslots) {
       Batch bt{(int64_t)qs.size(), 0, false, false};
       for (size_t k = b0; k < wq.size() && k < b0 + (size_t)pl->nslots; k++) {
@@ -909,9 +984,9 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   HC(hipEventRecord(pl->e0, s));
   hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(64), 0, s, H.err, (int64_t)1, 0);
   hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, H.rounds, (int64_t)3, (int64_t)0);
-  hipLaunchKernelGGL((k_fill<int64_t>), dim3(256), dim3(256), 0, s, H.log, pl->nreal * pl->npath, (int64_t)-1);
-  hipLaunchKernelGGL((k_fill<T>), dim3(1024), dim3(256), 0, s, (T*)H.sims, pl->nreal * g.padvox, (T)0);
-  hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s, H.cuts, pl->nreal * g.padvox, (uint8_t)0);
+  hipLaunchKernelGGL((k_fill<int64_t>), dim3(256), dim3(256), 0, s, H.log, pl->nlog, (int64_t)-1);
+  hipLaunchKernelGGL((k_fill<T>), dim3(1024), dim3(256), 0, s, (T*)H.sims, pl->nvox, (T)0);
+  hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s, H.cuts, pl->nvox, (uint8_t)0);
   hipLaunchKernelGGL((k_fill<int64_t>), dim3(16), dim3(256), 0, s, H.pick, pl->nslots, (int64_t)-1);
   HCL();
   for (const Batch& b : batches)
@@ -1014,8 +1089,7 @@ int iq_fastdistance(iq_ctx* ctx, const void* img, int img_dtype, const int64_t i
     if (!img || !kern || !out || !idims || !kdims) bad(IQ_ERR_ARG, "null argument");
     if (img_dtype != IQ_F32 && img_dtype != IQ_F64) bad(IQ_ERR_ARG, "img_dtype must be IQ_F32 or IQ_F64");
     IqGeo g{};
-    int64_t one[3] = {kdims[0], kdims[1], kdims[2]};
-    geo_init(g, idims, kdims, one);  // query grid = the kernel array itself
+    geo_init(g, idims, kdims);
     for (int d = 0; d < 3; d++)
       if (kdims[d] <= 0 || g.dist[d] <= 0) bad(IQ_ERR_ARG, "kernel larger than image");
     hipStream_t s = ctx->stream;
@@ -1223,7 +1297,18 @@ int iq_plan_create(iq_ctx* ctx, const iq_plan_desc* desc, iq_plan** out) {
   if (!out) return fail(ctx, IQ_ERR_ARG, "null output handle");
   iq_plan* pl = new (std::nothrow) iq_plan();
   if (!pl) return fail(ctx, IQ_ERR_NOMEM, "host allocation failed");
-  int rc = guard(ctx, [&] { plan_build(ctx, desc, pl); });
+  int rc = guard(ctx, [&] { plan_build(ctx, desc, 1, pl); });
+  if (rc != IQ_OK) { iq_plan_destroy(pl); return rc; }
+  *out = pl;
+  return IQ_OK;
+}
+
+int iq_plan_create_multi(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan** out) {
+  if (!ctx) return fail(nullptr, IQ_ERR_ARG, "null context");
+  if (!out) return fail(ctx, IQ_ERR_ARG, "null output handle");
+  iq_plan* pl = new (std::nothrow) iq_plan();
+  if (!pl) return fail(ctx, IQ_ERR_NOMEM, "host allocation failed");
+  int rc = guard(ctx, [&] { plan_build(ctx, descs, ngrid, pl); });
   if (rc != IQ_OK) { iq_plan_destroy(pl); return rc; }
   *out = pl;
   return IQ_OK;
@@ -1245,7 +1330,7 @@ int iq_plan_run(iq_ctx* ctx, iq_plan* plan, const double* uniforms, const int64_
   return guard(ctx, [&] {
     if (plan->ctx != ctx) bad(IQ_ERR_ARG, "plan belongs to another context");
     if (uniforms)
-      for (int64_t i = 0; i < plan->nreal * plan->npath; i++)
+      for (int64_t i = 0; i < plan->nlog; i++)
         if (!(uniforms[i] >= 0.0 && uniforms[i] < 1.0)) bad(IQ_ERR_ARG, "uniforms must lie in [0,1)");
     if (plan->dtype == IQ_F32) plan_run<float>(plan, uniforms, forced);
     else plan_run<double>(plan, uniforms, forced);
@@ -1256,14 +1341,13 @@ int iq_plan_results(iq_ctx* ctx, iq_plan* plan, void* out, int64_t* chosen, doub
   if (!ctx || !plan) return fail(ctx, IQ_ERR_ARG, "null context or plan");
   return guard(ctx, [&] {
     hipStream_t s = ctx->stream;
-    const IqGeo& g = plan->geo;
     const size_t esz = plan->dtype == IQ_F32 ? 4 : 8;
-    if (out) HC(hipMemcpyAsync(out, plan->hd.sims, (size_t)plan->nreal * g.padvox * esz, hipMemcpyDeviceToHost, s));
-    if (chosen && plan->npath)
-      HC(hipMemcpyAsync(chosen, plan->hd.log, (size_t)plan->nreal * plan->npath * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (out) HC(hipMemcpyAsync(out, plan->hd.sims, (size_t)plan->nvox * esz, hipMemcpyDeviceToHost, s));
+    if (chosen && plan->nlog)
+      HC(hipMemcpyAsync(chosen, plan->hd.log, (size_t)plan->nlog * sizeof(int64_t), hipMemcpyDeviceToHost, s));
     std::vector<uint8_t> c;
     if (cutgrid) {
-      c.resize((size_t)plan->nreal * g.padvox);
+      c.resize((size_t)plan->nvox);
       HC(hipMemcpyAsync(c.data(), plan->hd.cuts, c.size(), hipMemcpyDeviceToHost, s));
     }
     HC(hipStreamSynchronize(s));

@@ -44,7 +44,7 @@ struct IqSlab {
   int32_t lo[3], sz[3];
 };
 
-// One path position.  Identical for every realization of a plan.
+// One path position of one simulation grid (shared by the grid's realizations).
 struct IqStep {
   int32_t active;     // 0: skipped tile (iqsim.jl:174)
   int32_t origin[3];  // tile origin in the padded grid (iqsim.jl:180)
@@ -57,7 +57,7 @@ struct IqStep {
 
 // One tile step of one realization inside a batch (a wave of independent tiles).
 struct IqQuery {
-  int32_t step;    // path position
+  int32_t step;    // entry of the steps table (its grid's step0 + path position)
   int32_t real;    // realization
   int32_t slot;    // scratch slot
   int32_t multi;   // 1: relaxation / tau model over several sources
@@ -85,11 +85,20 @@ struct IqKTab {
 struct IqGeo {
   int32_t ti[3];     // training-image extents
   int32_t tile[3];
-  int32_t pad[3];    // padded simulation grid
   int32_t dist[3];   // candidate extents = ti - tile + 1
   int64_t P;         // number of candidates
-  int64_t padvox;
   int32_t tilevox;
+};
+
+// One realization: its simulation grid (realizations of one plan may have
+// grids of different sizes) and where its state lives.
+struct IqReal {
+  int32_t pad[3];              // padded simulation grid (iqsim.jl:114)
+  int32_t step0;               // first entry of its grid's steps
+  int64_t sim_off;             // first voxel of its grid in sims / cuts
+  int64_t log_off;             // pick-log entry of its path position 0
+  const double* hard;          // hard-data grid (NaN: none), or null
+  const double* const* aux;    // soft-data grids (padded), naux of them
 };
 
 // Everything a kernel needs, resident in device memory (read through the
@@ -112,14 +121,12 @@ struct IqDev {
   const int32_t* htoff;
   int32_t full_off;
   const double* const* a2maps;
-  const double* hardgrid;
-  const double* const* aux;
+  const IqReal* reals;
   const double* const* auxti;
   const double* const* a2soft;
   void* sims;
   uint8_t* cuts;
   int64_t* log;
-  int64_t npath;
   int32_t* err;
   int64_t* rounds;
   // per-slot scratch

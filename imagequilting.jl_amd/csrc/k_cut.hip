@@ -557,9 +557,10 @@ k_graphcut(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const int rc[3] = {(int)(pk % g.dist[0]), (int)((pk / g.dist[0]) % g.dist[1]),
                      (int)(pk / ((int64_t)g.dist[0] * g.dist[1]))};
   SlabView<T> A, B;
-  A.base = (const T*)D->sims + (int64_t)q.real * g.padvox +
-           iq_lin3(g.pad, st.origin[0] + sl.lo[0], st.origin[1] + sl.lo[1], st.origin[2] + sl.lo[2]);
-  A.st[0] = 1; A.st[1] = g.pad[0]; A.st[2] = (int64_t)g.pad[0] * g.pad[1];
+  const IqReal& R = D->reals[q.real];
+  A.base = (const T*)D->sims + R.sim_off +
+           iq_lin3(R.pad, st.origin[0] + sl.lo[0], st.origin[1] + sl.lo[1], st.origin[2] + sl.lo[2]);
+  A.st[0] = 1; A.st[1] = R.pad[0]; A.st[2] = (int64_t)R.pad[0] * R.pad[1];
   B.base = (const T*)D->ti + iq_lin3(g.ti, rc[0] + sl.lo[0], rc[1] + sl.lo[1], rc[2] + sl.lo[2]);
   B.st[0] = 1; B.st[1] = g.ti[0]; B.st[2] = (int64_t)g.ti[0] * g.ti[1];
   uint8_t* out = D->keep + (slot * IQ_MAX_SLAB + blockIdx.x) * g.tilevox;
@@ -601,8 +602,9 @@ k_paste(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const int rc[3] = {(int)(pk % g.dist[0]), (int)((pk / g.dist[0]) % g.dist[1]),
                      (int)(pk / ((int64_t)g.dist[0] * g.dist[1]))};
   const int ns = st.nslab;
-  T* sim = (T*)D->sims + (int64_t)q.real * g.padvox;
-  uint8_t* cut = D->cuts + (int64_t)q.real * g.padvox;
+  const IqReal& R = D->reals[q.real];
+  T* sim = (T*)D->sims + R.sim_off;
+  uint8_t* cut = D->cuts + R.sim_off;
   const T* ti = (const T*)D->ti;
   const uint8_t* keep = D->keep + slot * IQ_MAX_SLAB * g.tilevox;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < g.tilevox; j += gridDim.x * blockDim.x) {
@@ -614,7 +616,7 @@ k_paste(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
       if (a >= 0 && b >= 0 && c >= 0 && a < sl.sz[0] && b < sl.sz[1] && c < sl.sz[2])
         k |= keep[(int64_t)s * g.tilevox + a + sl.sz[0] * (b + sl.sz[1] * c)];
     }
-    int64_t gi = iq_lin3(g.pad, st.origin[0] + x, st.origin[1] + y, st.origin[2] + z);
+    int64_t gi = iq_lin3(R.pad, st.origin[0] + x, st.origin[1] + y, st.origin[2] + z);
     if (!k) sim[gi] = ti[iq_lin3(g.ti, rc[0] + x, rc[1] + y, rc[2] + z)];
     cut[gi] = k;
   }

@@ -125,12 +125,13 @@ k_query_prep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   }
   double* kv = D->kv + slot * D->kv_stride + (int64_t)src * g.tilevox;
   const int ox = st.origin[0], oy = st.origin[1], oz = st.origin[2];
-  const T* sim = (const T*)D->sims + (int64_t)q.real * g.padvox;
-  const double* dgrid = src == 1 ? D->hardgrid : (src >= 2 ? D->aux[src - 2] : nullptr);
+  const IqReal& R = D->reals[q.real];
+  const T* sim = (const T*)D->sims + R.sim_off;
+  const double* dgrid = src == 1 ? R.hard : (src >= 2 ? R.aux[src - 2] : nullptr);
   for (int j = threadIdx.x; j < V; j += blockDim.x) {
     int t = tv[j];
     int x = t % g.tile[0], y = (t / g.tile[0]) % g.tile[1], z = t / (g.tile[0] * g.tile[1]);
-    int64_t gi = iq_lin3(g.pad, ox + x, oy + y, oz + z);
+    int64_t gi = iq_lin3(R.pad, ox + x, oy + y, oz + z);
     double v = src == 0 ? (double)sim[gi] : dgrid[gi];
     kv[j] = v;
     if (j < PREP_LDS) kl[j] = v;
@@ -143,7 +144,7 @@ k_query_prep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
       for (int i = threadIdx.x; i < n; i += blockDim.x) {
         int x = i % bx.wxp, y = (i / bx.wxp) % bx.wy, z = i / (bx.wxp * bx.wy);
         float v = 0.0f;
-        if (x < bx.wx) v = (float)sim[iq_lin3(g.pad, ox + bx.x0 + x, oy + bx.y0 + y, oz + bx.z0 + z)];
+        if (x < bx.wx) v = (float)sim[iq_lin3(R.pad, ox + bx.x0 + x, oy + bx.y0 + y, oz + bx.z0 + z)];
         kv32[bx.kofs + i] = v;
       }
     }
@@ -453,13 +454,14 @@ k_mfma_bprep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq
       const IqGeo& g = D->g;
       const IqChunk ch = D->chunks[2 * step + (lane >> 5)];
       const uint8_t* mb = D->maskbits + (int64_t)st.mask * g.tilevox;
-      const float* sim32 = D->ti_f32 ? (const float*)D->sims + (int64_t)q.real * g.padvox : nullptr;
-      const double* sim64 = D->ti_f32 ? nullptr : (const double*)D->sims + (int64_t)q.real * g.padvox;
+      const IqReal& R = D->reals[q.real];
+      const float* sim32 = D->ti_f32 ? (const float*)D->sims + R.sim_off : nullptr;
+      const double* sim64 = D->ti_f32 ? nullptr : (const double*)D->sims + R.sim_off;
       for (int e = 0; e < ch.n; e++) {
         const int x = ch.jx + (ch.dir == 0 ? e : 0), y = ch.jy + (ch.dir == 1 ? e : 0), z = ch.jz;
         const int tv = x + g.tile[0] * (y + g.tile[1] * z);
         if (!mb[tv]) continue;
-        const int64_t gi = iq_lin3(g.pad, st.origin[0] + x, st.origin[1] + y, st.origin[2] + z);
+        const int64_t gi = iq_lin3(R.pad, st.origin[0] + x, st.origin[1] + y, st.origin[2] + z);
         const float v = sim32 ? sim32[gi] : (float)sim64[gi];
         const uint32_t wv = iq_split_word(v);
         h[e >> 1] |= (wv & 0xffffu) << (16 * (e & 1));

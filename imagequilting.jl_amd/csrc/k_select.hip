@@ -438,7 +438,8 @@ k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const IqGeo& g = D->g;
   const IqStep& st = D->steps[q.step];
   const int64_t slot = q.slot;
-  SelOut o{D->pick + slot, D->log + (int64_t)q.real * D->npath + q.step, D->err, D->nband + slot};
+  SelOut o{D->pick + slot, D->log + D->reals[q.real].log_off + (q.step - D->reals[q.real].step0), D->err,
+           D->nband + slot};
   double* s1 = D->s1 + slot * D->s1_stride;
   double* sd = D->sd + slot * IQ_SORT_MAX;
   int32_t* si = D->si + slot * IQ_SORT_MAX;
@@ -620,7 +621,8 @@ k_relax_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const IqGeo& g = D->g;
   const IqStep& st = D->steps[q.step];
   const int64_t slot = q.slot;
-  SelOut o{D->pick + slot, D->log + (int64_t)q.real * D->npath + q.step, D->err, D->nband + slot};
+  SelOut o{D->pick + slot, D->log + D->reals[q.real].log_off + (q.step - D->reals[q.real].step0), D->err,
+           D->nband + slot};
   int nsrc = 0;
   if (threadIdx.x == 0) {
     int k = 0;

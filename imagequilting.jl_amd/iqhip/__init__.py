@@ -17,7 +17,9 @@ from .api import (  # noqa: F401
     graphcut,
     imfilter,
     iqsim,
+    iqsim_batch,
     make_plan,
+    make_plan_multi,
     relaxation,
     sample,
     taumodel,

@@ -31,7 +31,7 @@ EXPORTS = (
     "iq_device_count", "iq_ctx_create", "iq_ctx_destroy", "iq_last_error", "iq_ctx_synchronize",
     "iq_imfilter", "iq_fastdistance", "iq_band_pick", "iq_relaxation", "iq_taumodel", "iq_sample",
     "iq_graphcut", "iq_plan_create", "iq_plan_destroy", "iq_plan_run", "iq_plan_results",
-    "iq_plan_last_time", "iq_plan_profile", "iq_plan_scan_stats", "iq_plan_cut_stats",
+    "iq_plan_create_multi", "iq_plan_last_time", "iq_plan_profile", "iq_plan_scan_stats", "iq_plan_cut_stats",
     "iq_simulate",
 )
 
@@ -93,6 +93,7 @@ def load(path: str = LIB_PATH):
             "iq_sample": [vp, vp, i64, f64, vp],
             "iq_graphcut": [vp, vp, vp, c_int, vp, c_int, vp],
             "iq_plan_create": [vp, vp, P(vp)],
+            "iq_plan_create_multi": [vp, vp, c_int, P(vp)],
             "iq_plan_destroy": [vp],
             "iq_plan_run": [vp, vp, vp, vp],
             "iq_plan_results": [vp, vp, vp, vp, vp],

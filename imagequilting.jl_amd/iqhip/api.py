@@ -176,19 +176,43 @@ def graphcut(A, B, dim: int, ctx: Optional[Context] = None) -> np.ndarray:
 # iqsim (iqsim.jl:50-315)
 # ---------------------------------------------------------------------------
 class Plan:
-    """Device-resident simulation inputs (iq_plan): TI, geometry, path, data."""
+    """Device-resident simulation inputs (iq_plan): TI, geometry, paths, data.
 
-    def __init__(self, ctx: Context, desc: _lib.PlanDesc, keep: list, npath: int, nreal: int, padsize, dtype):
+    A plan covers one or more simulation grids (``grids``: list of
+    dict(padsize, npath, nreal)); realizations are numbered grid by grid and
+    advance in lockstep.  Per-realization arrays (uniforms, forced picks,
+    chosen picks, output grids) follow that order."""
+
+    def __init__(self, ctx: Context, descs, keep: list, grids, dtype):
         self.ctx = ctx
         self._keep = keep
-        self.npath, self.nreal, self.padsize, self.dtype = npath, nreal, tuple(padsize), dtype
+        self.grids = [dict(padsize=tuple(g["padsize"]), npath=int(g["npath"]), nreal=int(g["nreal"])) for g in grids]
+        self.dtype = dtype
+        self.nreal = sum(g["nreal"] for g in self.grids)
+        self.real_npath = [g["npath"] for g in self.grids for _ in range(g["nreal"])]
+        self.real_padsize = [g["padsize"] for g in self.grids for _ in range(g["nreal"])]
+        self.npath = self.grids[0]["npath"]
+        self.padsize = self.grids[0]["padsize"]
+        arr = (_lib.PlanDesc * len(descs))(*descs)
         h = ctypes.c_void_p()
-        ctx.check(ctx._L.iq_plan_create(ctx.handle, ctypes.byref(desc), ctypes.byref(h)))
+        ctx.check(ctx._L.iq_plan_create_multi(ctx.handle, arr, len(descs), ctypes.byref(h)))
         self.handle = h
 
-    def run(self, uniforms: Optional[np.ndarray] = None, forced: Optional[np.ndarray] = None) -> None:
-        u = None if uniforms is None else np.ascontiguousarray(uniforms, dtype=np.float64).reshape(self.nreal, self.npath)
-        f = None if forced is None else np.ascontiguousarray(forced, dtype=np.int64).reshape(self.nreal, self.npath)
+    def _per_real(self, a, dtype):
+        """Concatenate per-realization rows (realization order) into one array."""
+        if a is None:
+            return None
+        if isinstance(a, (list, tuple)):
+            flat = np.concatenate([np.asarray(x, dtype=dtype).ravel() for x in a])
+        else:
+            flat = np.asarray(a, dtype=dtype).ravel()
+        if flat.size != sum(self.real_npath):
+            raise ValueError("need one entry per path position of every realization")
+        return np.ascontiguousarray(flat)
+
+    def run(self, uniforms=None, forced=None) -> None:
+        u = self._per_real(uniforms, np.float64)
+        f = self._per_real(forced, np.int64)
         self.ctx.check(self.ctx._L.iq_plan_run(self.ctx.handle, self.handle, _ptr(u), _ptr(f)))
 
     def last_time(self) -> float:
@@ -216,18 +240,23 @@ class Plan:
         return int(v[0]), int(v[1]), int(v[2])
 
     def results(self, grids=True, chosen=True, cuts=False):
-        padvox = int(np.prod(self.padsize))
-        out = np.zeros(self.nreal * padvox, dtype=self.dtype) if grids else None
-        ch = np.zeros(self.nreal * max(self.npath, 1), dtype=np.int64) if chosen else None
-        cg = np.zeros(self.nreal * padvox, dtype=np.float64) if cuts else None
+        """Per realization: padded grids, chosen picks (npath), cut grids."""
+        vox = [int(np.prod(p)) for p in self.real_padsize]
+        nvox, nlog = sum(vox), sum(self.real_npath)
+        out = np.zeros(max(nvox, 1), dtype=self.dtype) if grids else None
+        ch = np.zeros(max(nlog, 1), dtype=np.int64) if chosen else None
+        cg = np.zeros(max(nvox, 1), dtype=np.float64) if cuts else None
         self.ctx.check(self.ctx._L.iq_plan_results(self.ctx.handle, self.handle, _ptr(out), _ptr(ch), _ptr(cg)))
+        vo = np.concatenate([[0], np.cumsum(vox)])
+        lo = np.concatenate([[0], np.cumsum(self.real_npath)])
         res = {}
         if grids:
-            res["grids"] = [out[r * padvox:(r + 1) * padvox].reshape(self.padsize, order="F") for r in range(self.nreal)]
+            res["grids"] = [out[vo[r]:vo[r + 1]].reshape(self.real_padsize[r], order="F") for r in range(self.nreal)]
         if chosen:
-            res["chosen"] = ch.reshape(self.nreal, max(self.npath, 1))[:, : self.npath]
+            rows = [ch[lo[r]:lo[r + 1]] for r in range(self.nreal)]
+            res["chosen"] = np.stack(rows) if len(set(self.real_npath)) == 1 else rows
         if cuts:
-            res["cuts"] = [cg[r * padvox:(r + 1) * padvox].reshape(self.padsize, order="F") for r in range(self.nreal)]
+            res["cuts"] = [cg[vo[r]:vo[r + 1]].reshape(self.real_padsize[r], order="F") for r in range(self.nreal)]
         return res
 
     def close(self):
@@ -277,62 +306,95 @@ def make_plan(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=N
               nreal=1, rng=None, ctx: Optional[Context] = None):
     """Host preparation of iqsim.jl:65-139 and the upload of everything the
     tile loop reads.  Returns (plan, info)."""
+    plan, infos = make_plan_multi(trainimg, tilesize, [simsize], overlap=overlap, soft=soft,
+                                  hard=None if hard is None else [hard], tol=tol, path=path, nreal=nreal, rng=rng,
+                                  ctx=ctx)
+    return plan, infos[0]
+
+
+def make_plan_multi(trainimg, tilesize, simsizes, *, overlap=None, soft=(), hard=None, tol=0.1, path="raster",
+                    nreal=1, rng=None, ctx: Optional[Context] = None):
+    """One plan over several simulation grids (realizations of different
+    output sizes advancing in lockstep).  ``simsizes``: one size per grid
+    (None = the TI size); ``hard``: None or one dict per grid; ``soft``:
+    (data, dataTI) pairs shared by all grids (each padded to its grid);
+    ``nreal``: realizations per grid (int or one per grid).  Paths are drawn
+    from ``rng`` grid by grid.  Returns (plan, [info per grid])."""
     ctx = ctx or default_context()
     T = trainimg
     tis = np.shape(T)
     N = len(tis)
-    simsize = tuple(tis) if simsize is None else tuple(int(s) for s in simsize)
     overlap = tuple(1 / 6 for _ in range(N)) if overlap is None else tuple(overlap)
-    hard = dict(hard or {})
     soft = list(soft)
     rng = np.random.default_rng() if rng is None else rng
     tilesize = tuple(int(t) for t in tilesize)
-    _validate(T, tilesize, simsize, overlap, soft, hard, tol, path, nreal)
-    geo = prep.geometry(tis, tilesize, simsize, overlap)
-    if any(t > 1 and o == 1 for t, o in zip(tilesize, geo["ovlsize"])):
-        warnings.warn("Overlaps with only 1 voxel, check tilesize/overlap configuration")  # iqsim.jl:95-97
+    ng = len(simsizes)
+    if ng == 0:
+        raise ValueError("no simulation grid")
+    nreals = [int(nreal)] * ng if np.isscalar(nreal) else [int(n) for n in nreal]
+    hards = [dict() for _ in range(ng)] if hard is None else [dict(h or {}) for h in hard]
+    if len(nreals) != ng or len(hards) != ng:
+        raise ValueError("one nreal and one hard-data set per grid")
     TI = prep.prepare(T)
     dtype = np.float32 if TI.dtype == np.float32 else np.float64
     TI = _f(TI, dtype)
-    SOFT = [(_f(prep.prepare(prep.pad_symmetric(a, geo["padsize"])), np.float64), _f(prep.prepare(b), np.float64))
-            for a, b in soft]
-    disabled = _f(prep.finddisabled(T, geo), np.uint8)
-    skipped, datainds = prep.findskipped(hard, geo)
-    simpath = prep.genpath(rng, geo["ntiles"], path, list(datainds))
-    nt = int(np.prod(geo["ntiles"]))
-    skp = np.zeros(nt, dtype=np.uint8)
-    for s_ in skipped:
-        skp[s_] = 1
-    pth = np.ascontiguousarray(simpath, dtype=np.int64)
-    hardgrid = None
-    if hard:
-        hardgrid = np.full(geo["padsize"], np.nan, dtype=np.float64, order="F")
-        for k, v in hard.items():
-            hardgrid[tuple(k)] = v
-    d = _lib.PlanDesc()
-    d.ti = TI.ctypes.data
-    d.ti_dtype = _lib.IQ_F32 if dtype == np.float32 else _lib.IQ_F64
-    d.tisize[:] = list(_d3(tis))
-    for name in ("tilesize", "ovlsize", "spacing", "ntiles", "padsize"):
-        # a missing trailing axis is one voxel thick with no overlap
-        getattr(d, name)[:] = list(_d3(geo[name], 0 if name == "ovlsize" else 1))
-    d.disabled = disabled.ctypes.data
-    d.path = pth.ctypes.data
-    d.npath = len(simpath)
-    d.skipped = skp.ctypes.data
-    d.hardgrid = None if hardgrid is None else hardgrid.ctypes.data
-    d.naux = len(SOFT)
-    aux_p = (ctypes.c_void_p * max(1, len(SOFT)))(*[a.ctypes.data for a, _ in SOFT])
-    auxti_p = (ctypes.c_void_p * max(1, len(SOFT)))(*[b.ctypes.data for _, b in SOFT])
-    d.aux = ctypes.cast(aux_p, ctypes.c_void_p)
-    d.auxti = ctypes.cast(auxti_p, ctypes.c_void_p)
-    d.tol = float(tol)
-    d.nreal = int(nreal)
-    keep = [TI, SOFT, disabled, pth, skp, hardgrid, aux_p, auxti_p]
-    plan = Plan(ctx, d, keep, len(simpath), int(nreal), geo["padsize"], dtype)
-    info = dict(geo=geo, path=simpath, skipped=skipped, datainds=datainds, hard=hard, T=prep.element_type(T),
-                rng=rng)
-    return plan, info
+    ti_type = prep.element_type(T)
+    keep = [TI]
+    descs, grids, infos = [], [], []
+    disabled = None
+    SOFT_TI = [_f(prep.prepare(b), np.float64) for _, b in soft]
+    auxti_p = (ctypes.c_void_p * max(1, len(soft)))(*[b.ctypes.data for b in SOFT_TI])
+    keep += [SOFT_TI, auxti_p]
+    warned = False
+    for gi in range(ng):
+        simsize = tuple(tis) if simsizes[gi] is None else tuple(int(s) for s in simsizes[gi])
+        hd = hards[gi]
+        _validate(T, tilesize, simsize, overlap, soft, hd, tol, path, nreals[gi])
+        geo = prep.geometry(tis, tilesize, simsize, overlap)
+        if not warned and any(t > 1 and o == 1 for t, o in zip(tilesize, geo["ovlsize"])):
+            warnings.warn("Overlaps with only 1 voxel, check tilesize/overlap configuration")  # iqsim.jl:95-97
+            warned = True
+        if disabled is None:
+            disabled = _f(prep.finddisabled(T, geo), np.uint8)
+            keep.append(disabled)
+        aux = [_f(prep.prepare(prep.pad_symmetric(a, geo["padsize"])), np.float64) for a, _ in soft]
+        skipped, datainds = prep.findskipped(hd, geo)
+        simpath = prep.genpath(rng, geo["ntiles"], path, list(datainds))
+        nt = int(np.prod(geo["ntiles"]))
+        skp = np.zeros(nt, dtype=np.uint8)
+        for s_ in skipped:
+            skp[s_] = 1
+        pth = np.ascontiguousarray(simpath, dtype=np.int64)
+        hardgrid = None
+        if hd:
+            hardgrid = np.full(geo["padsize"], np.nan, dtype=np.float64, order="F")
+            for k, v in hd.items():
+                hardgrid[tuple(k)] = v
+        d = _lib.PlanDesc()
+        d.ti = TI.ctypes.data
+        d.ti_dtype = _lib.IQ_F32 if dtype == np.float32 else _lib.IQ_F64
+        d.tisize[:] = list(_d3(tis))
+        for name in ("tilesize", "ovlsize", "spacing", "ntiles", "padsize"):
+            # a missing trailing axis is one voxel thick with no overlap
+            getattr(d, name)[:] = list(_d3(geo[name], 0 if name == "ovlsize" else 1))
+        d.disabled = disabled.ctypes.data
+        d.path = pth.ctypes.data
+        d.npath = len(simpath)
+        d.skipped = skp.ctypes.data
+        d.hardgrid = None if hardgrid is None else hardgrid.ctypes.data
+        d.naux = len(soft)
+        aux_p = (ctypes.c_void_p * max(1, len(soft)))(*[a.ctypes.data for a in aux])
+        d.aux = ctypes.cast(aux_p, ctypes.c_void_p)
+        d.auxti = ctypes.cast(auxti_p, ctypes.c_void_p)
+        d.tol = float(tol)
+        d.nreal = nreals[gi]
+        keep += [aux, pth, skp, hardgrid, aux_p]
+        descs.append(d)
+        grids.append(dict(padsize=geo["padsize"], npath=len(simpath), nreal=nreals[gi]))
+        infos.append(dict(geo=geo, path=simpath, skipped=skipped, datainds=datainds, hard=hd, T=ti_type, rng=rng,
+                          nreal=nreals[gi]))
+    plan = Plan(ctx, descs, keep, grids, dtype)
+    return plan, infos
 
 
 def finish(info, grids, cuts=None):
@@ -354,6 +416,35 @@ def finish(info, grids, cuts=None):
         if cg is not None:
             outcuts.append(cg[crop])
     return reals, outcuts, voxs
+
+
+def iqsim_batch(trainimg, tilesize, simsizes, *, overlap=None, soft=(), hard=None, tol=0.1, path="raster", nreal=1,
+                debug=False, rng=None, uniforms=None, ctx: Optional[Context] = None, return_trace=False):
+    """iqsim over several output grids at once (one plan, lockstep waves).
+    Returns one result per grid, each what ``iqsim`` returns for it."""
+    plan, infos = make_plan_multi(trainimg, tilesize, simsizes, overlap=overlap, soft=soft, hard=hard, tol=tol,
+                                  path=path, nreal=nreal, rng=rng, ctx=ctx)
+    try:
+        if uniforms is None:
+            uniforms = [infos[0]["rng"].random(n) for n in plan.real_npath]
+        plan.run(uniforms)
+        res = plan.results(grids=True, chosen=return_trace, cuts=debug)
+    finally:
+        plan.close()
+    outs, traces, r0 = [], [], 0
+    for info in infos:
+        n = info["nreal"]
+        grids = res["grids"][r0:r0 + n]
+        cuts = res["cuts"][r0:r0 + n] if debug else None
+        reals, cuts, voxs = finish(info, grids, cuts)
+        outs.append((reals, cuts, voxs) if debug else reals)
+        if return_trace:
+            ch = res["chosen"][r0:r0 + n]
+            traces.append(dict(chosen=np.asarray([np.asarray(c) for c in ch]), path=info["path"], geo=info["geo"],
+                               uniforms=np.asarray([np.asarray(uniforms[r]) for r in range(r0, r0 + n)])
+                               if isinstance(uniforms, list) else None))
+        r0 += n
+    return (outs, traces) if return_trace else outs
 
 
 def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard: Optional[Dict] = None, tol=0.1,

@@ -121,6 +121,18 @@ typedef struct iq_plan_desc {
 
 /* Upload everything once (the TI stays resident in HBM). */
 int iq_plan_create(iq_ctx* ctx, const iq_plan_desc* desc, iq_plan** out);
+
+/* One plan over several simulation grids of different sizes (the nreal loop
+ * of iqsim.jl:163 with a grid per group of realizations; every grid's
+ * realizations advance in lockstep with all others).  descs[0..ngrid) share
+ * the training image, ti_dtype, tisize, tilesize, ovlsize, spacing, tol,
+ * naux and auxti (taken from descs[0], together with disabled); each has its
+ * own ntiles, padsize, path, npath, skipped, hardgrid, aux and nreal.
+ * Realizations are numbered grid by grid.  iq_plan_run / iq_plan_results
+ * then take and give, realization after realization, npath uniforms /
+ * forced picks / chosen picks and padsize grid voxels of that realization's
+ * grid.  iq_plan_create(ctx, d, p) == iq_plan_create_multi(ctx, d, 1, p). */
+int iq_plan_create_multi(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan** out);
 int iq_plan_destroy(iq_plan* plan);
 
 /* Run all realizations.  uniforms: nreal x npath (row r holds realization

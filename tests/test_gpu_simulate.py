@@ -159,3 +159,46 @@ def test_sim_soft_large_database():
     print("soft large database", flush=True)
     _, tr = _compare(T, (10, 10), (40, 40), soft=[(aux, aux_ti)], tol=0.1)
     assert (tr["nband"] > 4096).any()
+
+
+def _compare_batch(T, tile, sizes, nreal, seed=0, **kw):
+    outs, trs = iqhip.iqsim_batch(T, tile, sizes, nreal=nreal, rng=np.random.default_rng(seed), debug=True,
+                                  return_trace=True, **kw)
+    hard = kw.pop("hard", None)
+    for g, (size, out, tr) in enumerate(zip(sizes, outs, trs)):
+        kg = dict(kw)
+        if hard is not None:
+            kg["hard"] = hard[g]
+        (rr, rc, rv), rtr = iqref.iqsim(T, tile, size, nreal=nreal[g], uniforms=tr["uniforms"], debug=True,
+                                        return_trace=True, simpath=tr["path"], **kg)
+        assert np.array_equal(tr["chosen"], rtr["chosen"]), g
+        reals, cuts, voxs = out
+        for a, b in zip(reals, rr):
+            assert np.array_equal(np.asarray(a, dtype=object), np.asarray(b, dtype=object)), g
+        for a, b in zip(cuts, rc):
+            assert np.array_equal(a, b, equal_nan=True), g
+        assert voxs == rv
+    return outs
+
+
+def test_sim_batch_uneven_grids_categorical():
+    # one plan, four output grids of different sizes, lockstep waves
+    rng = np.random.default_rng(14)
+    x = _smooth(rng, (30, 30, 8), 5, np.float64)
+    T = np.digitize(x, np.quantile(x, [0.65, 0.9])).astype(np.uint8)
+    sizes = [(35, 35, 8), (26, 44, 8), (50, 30, 8), (20, 20, 8)]
+    print("batch of uneven grids", flush=True)
+    outs = _compare_batch(T, (10, 10, 4), sizes, [1, 2, 1, 1], overlap=(1 / 4, 1 / 4, 1 / 4))
+    for size, reals in zip(sizes, outs):
+        assert reals[0][0].shape == size
+
+
+def test_sim_batch_soft_and_hard_per_grid():
+    rng = np.random.default_rng(15)
+    T = _smooth(rng, (40, 40), 5, np.float32)
+    aux_ti = _smooth(rng, (40, 40), 3, np.float32)
+    aux = _smooth(rng, (60, 60), 3, np.float32)
+    sizes = [(50, 50), (36, 58)]
+    hard = [{(3, 4): 0.5, (30, 20): -0.3}, {(10, 50): 0.1}]
+    print("batch soft + hard", flush=True)
+    _compare_batch(T, (10, 10), sizes, [1, 1], soft=[(aux, aux_ti)], hard=hard, tol=0.2)
