@@ -1396,3 +1396,5 @@ int iq_simulate(iq_ctx* ctx, const iq_plan_desc* desc, const double* uniforms, c
 }
 
 }  // extern "C"
+
+#include "iq_prep.hip"

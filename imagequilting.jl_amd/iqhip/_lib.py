@@ -32,7 +32,7 @@ EXPORTS = (
     "iq_imfilter", "iq_fastdistance", "iq_band_pick", "iq_relaxation", "iq_taumodel", "iq_sample",
     "iq_graphcut", "iq_plan_create", "iq_plan_destroy", "iq_plan_run", "iq_plan_results",
     "iq_plan_create_multi", "iq_plan_last_time", "iq_plan_profile", "iq_plan_scan_stats", "iq_plan_cut_stats",
-    "iq_simulate",
+    "iq_simulate", "iq_prepare", "iq_prepare_soft", "iq_finddisabled",
 )
 
 
@@ -102,6 +102,9 @@ def load(path: str = LIB_PATH):
             "iq_plan_scan_stats": [vp, vp, vp, vp, vp],
             "iq_plan_cut_stats": [vp, vp, vp, vp],
             "iq_simulate": [vp, vp, vp, vp, vp, vp, vp],
+            "iq_prepare": [vp, c_int, vp, vp, vp],
+            "iq_prepare_soft": [vp, c_int, vp, vp, vp, vp],
+            "iq_finddisabled": [vp, c_int, vp, vp, vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

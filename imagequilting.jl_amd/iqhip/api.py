@@ -76,6 +76,70 @@ def device_count() -> int:
 
 
 # ---------------------------------------------------------------------------
+# host preparation inside the C ABI (utils.jl:69-129): raw arrays in
+# ---------------------------------------------------------------------------
+_RAW_CODES = {"float32": 0, "float64": 1, "uint8": 2, "int8": 3, "uint16": 4, "int16": 5, "uint32": 6, "int32": 7,
+              "uint64": 8, "int64": 9}
+
+
+def _raw_array(a):
+    """Column-major data, raw element-type code and missing mask (or None)
+    of an input image (masked arrays and object arrays holding None carry
+    missing values)."""
+    T = prep.element_type(a)
+    miss = prep._missing_mask(a if isinstance(a, np.ma.MaskedArray) else np.asarray(a))
+    base = np.asarray(a.data) if isinstance(a, np.ma.MaskedArray) else np.asarray(a)
+    if base.dtype == object:
+        vals = np.zeros(base.shape, dtype=T if T.name in _RAW_CODES else np.float64)
+        keep = ~miss
+        vals[keep] = np.asarray(base[keep].tolist(), dtype=vals.dtype)
+        base = vals
+    if base.dtype == np.bool_:
+        base = base.astype(np.uint8)
+    if base.dtype.name not in _RAW_CODES:
+        base = base.astype(np.float64)
+    data = np.asfortranarray(base)
+    m = np.asfortranarray(miss.astype(np.uint8)) if miss.any() else None
+    return data, _RAW_CODES[data.dtype.name], m
+
+
+def prepare_native(img) -> np.ndarray:
+    """prepare (utils.jl:96-101) in the library: FP32 stays FP32, every other
+    element type becomes FP64, missing and NaN become 0."""
+    L = _lib.load()
+    data, code, m = _raw_array(img)
+    out = np.zeros(data.shape, dtype=np.float32 if code == 0 else np.float64, order="F")
+    dims = _d3(data.shape)
+    _lib.check(L.iq_prepare(data.ctypes.data, code, _ptr(m), dims.ctypes.data, out.ctypes.data))
+    return out
+
+
+def prepare_soft_native(aux, padsize) -> np.ndarray:
+    """Soft grid as the tile loop reads it (utils.jl:74-80): symmetric
+    padding to padsize, prepared, first padsize voxels per axis (FP64)."""
+    L = _lib.load()
+    data, code, m = _raw_array(aux)
+    pad = tuple(int(p) for p in padsize)
+    out = np.zeros(pad, dtype=np.float64, order="F")
+    dims, pad3 = _d3(data.shape), _d3(pad)
+    _lib.check(L.iq_prepare_soft(data.ctypes.data, code, _ptr(m), dims.ctypes.data, pad3.ctypes.data,
+                                 out.ctypes.data))
+    return out
+
+
+def finddisabled_native(trainimg, tilesize) -> np.ndarray:
+    """finddisabled (utils.jl:115-129) in the library."""
+    L = _lib.load()
+    data, code, m = _raw_array(trainimg)
+    ds = tuple(s - t + 1 for s, t in zip(data.shape, tilesize))
+    out = np.zeros(ds, dtype=np.uint8, order="F")
+    dims, tile3 = _d3(data.shape), _d3(tilesize)
+    _lib.check(L.iq_finddisabled(data.ctypes.data, code, _ptr(m), dims.ctypes.data, tile3.ctypes.data,
+                                 out.ctypes.data))
+    return out
+
+
+# ---------------------------------------------------------------------------
 # per-function entry points (each replaces one reference function)
 # ---------------------------------------------------------------------------
 def imfilter(img, krn, ctx: Optional[Context] = None) -> np.ndarray:
@@ -335,14 +399,13 @@ def make_plan_multi(trainimg, tilesize, simsizes, *, overlap=None, soft=(), hard
     hards = [dict() for _ in range(ng)] if hard is None else [dict(h or {}) for h in hard]
     if len(nreals) != ng or len(hards) != ng:
         raise ValueError("one nreal and one hard-data set per grid")
-    TI = prep.prepare(T)
-    dtype = np.float32 if TI.dtype == np.float32 else np.float64
-    TI = _f(TI, dtype)
+    TI = prepare_native(T)
+    dtype = TI.dtype.type
     ti_type = prep.element_type(T)
     keep = [TI]
     descs, grids, infos = [], [], []
     disabled = None
-    SOFT_TI = [_f(prep.prepare(b), np.float64) for _, b in soft]
+    SOFT_TI = [_f(prepare_native(b), np.float64) for _, b in soft]
     auxti_p = (ctypes.c_void_p * max(1, len(soft)))(*[b.ctypes.data for b in SOFT_TI])
     keep += [SOFT_TI, auxti_p]
     warned = False
@@ -355,9 +418,9 @@ def make_plan_multi(trainimg, tilesize, simsizes, *, overlap=None, soft=(), hard
             warnings.warn("Overlaps with only 1 voxel, check tilesize/overlap configuration")  # iqsim.jl:95-97
             warned = True
         if disabled is None:
-            disabled = _f(prep.finddisabled(T, geo), np.uint8)
+            disabled = finddisabled_native(T, tilesize)
             keep.append(disabled)
-        aux = [_f(prep.prepare(prep.pad_symmetric(a, geo["padsize"])), np.float64) for a, _ in soft]
+        aux = [prepare_soft_native(a, geo["padsize"]) for a, _ in soft]
         skipped, datainds = prep.findskipped(hd, geo)
         simpath = prep.genpath(rng, geo["ntiles"], path, list(datainds))
         nt = int(np.prod(geo["ntiles"]))

@@ -98,6 +98,55 @@ function graphcut_hip(A::AbstractArray{T,N}, B::AbstractArray{T,N}, dim::Integer
   BitArray(m .!= 0)
 end
 
+# raw element types of include/iqhip.h (iq_raw_dtype); others go through Float64
+const RAWCODE = Dict(Float32 => 0, Float64 => 1, UInt8 => 2, Int8 => 3, UInt16 => 4, Int16 => 5, UInt32 => 6,
+                     Int32 => 7, UInt64 => 8, Int64 => 9, Bool => 2)
+
+function rawarray(a::AbstractArray)
+  T = nonmissingtype(eltype(a))
+  S = T === Bool ? UInt8 : (haskey(RAWCODE, T) ? T : Float64)
+  data = Array{S}(undef, size(a))
+  miss = zeros(UInt8, size(a))
+  for (k, v) in enumerate(a)
+    if ismissing(v)
+      miss[k] = 1
+      data[k] = zero(S)
+    else
+      data[k] = S(v)
+    end
+  end
+  data, Cint(RAWCODE[S]), miss
+end
+
+"prepare (src/utils.jl:96-101) in the library."
+function prepare_hip(a::AbstractArray)
+  data, code, miss = rawarray(a)
+  out = Array{code == 0 ? Float32 : Float64}(undef, size(data))
+  check(ccall((:iq_prepare, libiqhip), Cint, (Ptr{Cvoid}, Cint, Ptr{UInt8}, Ref{NTuple{3,Int64}}, Ptr{Cvoid}),
+              data, code, miss, pad3(size(data)), out))
+  out
+end
+
+"Soft grid as the tile loop reads it (src/utils.jl:74-80), padsize Float64."
+function prepare_soft_hip(aux::AbstractArray, padsize::Tuple)
+  data, code, miss = rawarray(aux)
+  out = Array{Float64}(undef, padsize)
+  check(ccall((:iq_prepare_soft, libiqhip), Cint,
+              (Ptr{Cvoid}, Cint, Ptr{UInt8}, Ref{NTuple{3,Int64}}, Ref{NTuple{3,Int64}}, Ptr{Float64}),
+              data, code, miss, pad3(size(data)), pad3(padsize), out))
+  out
+end
+
+"finddisabled (src/utils.jl:115-129) in the library."
+function finddisabled_hip(trainimg::AbstractArray, tilesize::Tuple)
+  data, code, miss = rawarray(trainimg)
+  out = Array{UInt8}(undef, size(trainimg) .- tilesize .+ 1)
+  check(ccall((:iq_finddisabled, libiqhip), Cint,
+              (Ptr{Cvoid}, Cint, Ptr{UInt8}, Ref{NTuple{3,Int64}}, Ref{NTuple{3,Int64}}, Ptr{UInt8}),
+              data, code, miss, pad3(size(data)), pad3(tilesize), out))
+  out
+end
+
 """
     iqsim_hip(trainimg, tilesize, simsize=size(trainimg); kwargs...)
 
@@ -124,14 +173,14 @@ function iqsim_hip(trainimg::AbstractArray{T,N}, tilesize::Dims{N}, simsize::Dim
   ovlvol = prod(padsize) - prod(@. padsize - (ntiles - 1) * ovlsize)
   geo = (ntiles=ntiles, tilesize=tilesize, ovlsize=ovlsize, spacing=spacing, TIsize=TIsize, simsize=simsize,
          padsize=padsize, distsize=distsize)
-  # host preparation reused from ImageQuilting (src/utils.jl:69-204)
-  TI = ImageQuilting.prepare(trainimg)
+  # type conversion, soft padding and disabled windows inside the library
+  # (src/utils.jl:69-129): raw arrays in
+  TI = prepare_hip(trainimg)
   SOFT = map(soft) do (aux, auxTI)
-    append = padsize .- min.(padsize, size(aux))
-    AUX = ImageQuilting.prepare(ImageQuilting.padarray(aux, ImageQuilting.Pad(:symmetric, ntuple(i -> 0, N), append)))
-    Array{Float64}(AUX), Array{Float64}(ImageQuilting.prepare(auxTI))
+    prepare_soft_hip(aux, Tuple(padsize)), Array{Float64}(prepare_hip(auxTI))
   end
-  disabled = Array{UInt8}(ImageQuilting.finddisabled(trainimg, geo))
+  disabled = finddisabled_hip(trainimg, tilesize)
+  # tile skipping and paths reused from ImageQuilting (src/utils.jl:131-204)
   skipped, datainds = ImageQuilting.findskipped(hard, geo)
   simpath = Int64.(ImageQuilting.genpath(rng, ntiles, path, datainds)) .- 1
   skp = zeros(UInt8, prod(ntiles))

@@ -42,6 +42,13 @@ typedef enum iq_status {
 
 typedef enum iq_dtype { IQ_F32 = 0, IQ_F64 = 1 } iq_dtype;
 
+/* Element types of raw (unprepared) arrays: floats, and the integer /
+ * categorical types the reference converts to Float64 (utils.jl:96-101). */
+typedef enum iq_raw_dtype {
+  IQ_RAW_F32 = 0, IQ_RAW_F64 = 1, IQ_RAW_U8 = 2, IQ_RAW_I8 = 3, IQ_RAW_U16 = 4,
+  IQ_RAW_I16 = 5, IQ_RAW_U32 = 6, IQ_RAW_I32 = 7, IQ_RAW_U64 = 8, IQ_RAW_I64 = 9
+} iq_raw_dtype;
+
 typedef struct iq_ctx iq_ctx;
 typedef struct iq_plan iq_plan;
 
@@ -93,6 +100,28 @@ int iq_sample(iq_ctx* ctx, const double* p, int64_t n, double u, int64_t* out_in
  * sink in the residual graph of a maximum flow (keep A), 0 elsewhere. */
 int iq_graphcut(iq_ctx* ctx, const void* A, const void* B, int dtype, const int64_t dims[3], int dim,
                 uint8_t* out_mask);
+
+/* ---- host preparation of raw arrays (no device work, no context) ------ */
+
+/* prepare (utils.jl:96-101): raw array (extents dims, element type
+ * raw_dtype; missing[i] != 0 marks a missing value, NULL = none) to the
+ * element type the tile loop computes in: IQ_RAW_F32 stays FP32, every other
+ * type becomes FP64 (out holds prod(dims) elements of that type); missing
+ * and NaN values become 0. */
+int iq_prepare(const void* raw, int raw_dtype, const uint8_t* missing, const int64_t dims[3], void* out);
+
+/* Soft grid of one soft pair as the tile loop reads it (imagepreproc,
+ * utils.jl:74-80, read through views at grid coordinates, iqsim.jl:222-227):
+ * padarray(aux, Pad(:symmetric, 0, padsize - min(padsize, size(aux)))),
+ * prepared, first padsize voxels per axis.  out: prod(padsize) FP64. */
+int iq_prepare_soft(const void* aux, int raw_dtype, const uint8_t* missing, const int64_t dims[3],
+                    const int64_t padsize[3], double* out);
+
+/* finddisabled (utils.jl:115-129): disabled[c] = 1 when the training-image
+ * window at candidate c (extents tilesize) holds a missing or NaN voxel;
+ * disabled has tisize - tilesize + 1 extents. */
+int iq_finddisabled(const void* raw_ti, int raw_dtype, const uint8_t* missing, const int64_t tisize[3],
+                    const int64_t tilesize[3], uint8_t* disabled);
 
 /* ---- whole simulation (iqsim.jl:163-300) ------------------------------ */
 /* Everything the tile loop reads, prepared on the host exactly as

@@ -203,9 +203,18 @@ def geometry(tisize, tilesize, simsize, overlap):
                 distsize=distsize, TIsize=tuple(tisize), simsize=tuple(simsize), ovlvol=ovlvol)
 
 
+def _unmask(img):
+    """Masked entries of a numpy masked array are Julia `missing` (None)."""
+    if isinstance(img, np.ma.MaskedArray):
+        out = np.asarray(img.data).astype(object)
+        out[np.ma.getmaskarray(img)] = None
+        return out
+    return img
+
+
 def prepare(img):
     """utils.jl:96-102: missing (None / masked) and NaN become 0; non-float becomes Float64."""
-    a = np.asarray(img)
+    a = np.asarray(_unmask(img))
     if a.dtype == object:
         vals = [np.nan if v is None else v for v in a.ravel(order="F")]
         base = np.asarray([v for v in vals if not (isinstance(v, float) and math.isnan(v))])
@@ -221,7 +230,7 @@ def prepare(img):
 
 def finddisabled(trainimg, geo) -> np.ndarray:
     """utils.jl:115-129: candidate windows holding a NaN/missing voxel."""
-    a = np.asarray(trainimg)
+    a = np.asarray(_unmask(trainimg))
     if a.dtype == object:
         nanmask = np.vectorize(lambda v: v is None or (isinstance(v, float) and math.isnan(v)))(a)
     elif a.dtype.kind == "f":
@@ -336,7 +345,11 @@ def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None,
     is_f32 = TI.dtype == np.float32
     SOFT = []
     for aux, auxti in soft:
-        SOFT.append((prepare(pad_symmetric(aux, geo["padsize"])), prepare(auxti)))
+        # the tile loop reads AUX at grid coordinates (views, iqsim.jl:222-227):
+        # only its first padsize voxels per axis matter
+        A = prepare(pad_symmetric(aux, geo["padsize"]))
+        A = np.ascontiguousarray(A[tuple(slice(0, p) for p in geo["padsize"])])
+        SOFT.append((A, prepare(auxti)))
     disabled = finddisabled(T, geo)
     skipped, datainds = findskipped(hard, geo)
     rng = np.random.default_rng(seed)

@@ -126,3 +126,51 @@ def test_validation_mirrors_reference_asserts():
         iqhip.api._validate(T, (5, 5), (10, 10), (0.2, 0.2), [], {}, 0.1, "raster", 0)
     with pytest.raises(ValueError, match="outside"):
         iqhip.api._validate(T, (5, 5), (10, 10), (0.2, 0.2), [], {(10, 0): 1.0}, 0.1, "raster", 1)
+
+
+# ----------------------------------------------- host preparation in the C ABI
+def _raw_cases():
+    rng = np.random.default_rng(30)
+    f32 = rng.standard_normal((7, 6, 3)).astype(np.float32)
+    f32[1, 2, 0] = np.nan
+    f64 = rng.standard_normal((9, 5))
+    f64[0, 0] = np.nan
+    u8 = rng.integers(0, 3, (8, 8, 2)).astype(np.uint8)
+    i64 = rng.integers(-5, 5, (6, 7)).astype(np.int64)
+    obj = np.array([[1, None, 3], [None, 5, 6]], dtype=object)
+    masked = np.ma.masked_array(rng.integers(0, 4, (5, 5)).astype(np.int32), mask=rng.random((5, 5)) < 0.2)
+    boolean = rng.random((4, 6)) > 0.5
+    return [f32, f64, u8, i64, obj, masked, boolean]
+
+
+def test_native_prepare_matches_oracle():
+    for a in _raw_cases():
+        got = iqhip.api.prepare_native(a)
+        ref = iqref.prepare(a)
+        assert got.dtype == ref.dtype and np.array_equal(got, ref), a.dtype
+
+
+@pytest.mark.parametrize("shape,pad", [((10, 7), (16, 16)), ((30, 40), (22, 16)), ((3, 4, 2), (13, 5, 9)),
+                                       ((12, 12, 4), (12, 12, 4))])
+def test_native_soft_padding_matches_oracle(shape, pad):
+    rng = np.random.default_rng(31)
+    a = rng.standard_normal(shape)
+    a.flat[::7] = np.nan
+    got = iqhip.api.prepare_soft_native(a, pad)
+    ref = iqref.prepare(iqref.pad_symmetric(a, pad))[tuple(slice(0, p) for p in pad)]
+    assert got.shape == tuple(pad) and np.array_equal(got, ref)
+    u = rng.integers(0, 3, shape).astype(np.uint8)
+    assert np.array_equal(iqhip.api.prepare_soft_native(u, pad),
+                          iqref.prepare(iqref.pad_symmetric(u, pad))[tuple(slice(0, p) for p in pad)])
+
+
+def test_native_finddisabled_matches_oracle():
+    rng = np.random.default_rng(32)
+    for shape, tile in (((20, 18), (5, 4)), ((12, 10, 6), (3, 4, 2)), ((9, 9), (9, 9))):
+        T = rng.standard_normal(shape)
+        T[rng.random(shape) < 0.02] = np.nan
+        geo = iqref.geometry(shape, tile, shape, tuple(0.3 for _ in shape))
+        assert np.array_equal(iqhip.api.finddisabled_native(T, tile).astype(bool), iqref.finddisabled(T, geo))
+    obj = np.array([[1, None, 3, 4], [5, 6, 7, 8], [1, 2, 3, 4]], dtype=object)
+    geo = iqref.geometry(obj.shape, (2, 2), obj.shape, (0.5, 0.5))
+    assert np.array_equal(iqhip.api.finddisabled_native(obj, (2, 2)).astype(bool), iqref.finddisabled(obj, geo))

@@ -290,6 +290,26 @@ def test_soft_trend_ordering():
     assert r[0][:, :10, :].mean() <= r[0][:, 10:, :].mean()
 
 
+def test_soft_data_beyond_the_grid_is_not_read():
+    # AUX is padded to padsize and read through views at grid coordinates
+    # (utils.jl:74-80, iqsim.jl:222-227): voxels past padsize must not matter
+    rng = np.random.default_rng(21)
+    T = rng.random((30, 30))
+    auxti = rng.random((30, 30))
+    aux = rng.random((70, 55))
+    sim = (30, 30)
+    pad = iqref.geometry(T.shape, (10, 10), sim, (1 / 6, 1 / 6))["padsize"]
+    assert pad[0] < 70 and pad[1] < 55
+    other = aux.copy()
+    other[pad[0]:, :] = 1e3
+    other[:, pad[1]:] = -1e3
+    u = rng.random((1, 16))
+    a = iqref.iqsim(T, (10, 10), sim, soft=[(aux, auxti)], uniforms=u)
+    b = iqref.iqsim(T, (10, 10), sim, soft=[(other, auxti)], uniforms=u)
+    c = iqref.iqsim(T, (10, 10), sim, soft=[(aux[:pad[0], :pad[1]], auxti)], uniforms=u)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[0], c[0])
+
+
 def test_hard_data_honoured():
     TI = np.ones((20, 20, 20))
     obs = np.zeros((20, 20, 20))
