@@ -108,6 +108,11 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0, help="only run the copy used to calibrate HBM counters")
     ap.add_argument("--workload", type=int, default=2, choices=sorted(WORKLOADS),
                     help="2 (default): the plain full-scan workload; 1, 3, 4, 5: the other workloads")
+    ap.add_argument("--learned", action="store_true",
+                    help="also time the learned proposal (shortlist + exact rerank) on the same plan and report "
+                         "it as the value, with the full scan beside it")
+    ap.add_argument("--K", type=int, default=4096, help="shortlist size of the learned proposal")
+    ap.add_argument("--check-every", type=int, default=8, help="full-scan recall check every N-th batch")
     args = ap.parse_args()
     if args.copy_mb > 0:
         copy_calibration(args.copy_mb)
@@ -174,7 +179,7 @@ def main():
             "workload": WORKLOADS[args.workload],
             "tile_steps_per_step": int(sum(plan.real_npath)),
             "device_ms_per_step": dev_time / args.steps * 1e3,
-            "learned_proposal": "not built",
+            "learned_proposal": "off",
             "mincut": {"cuts": ncuts, "push_rounds_per_cut": cut_rounds / max(ncuts, 1),
                        "relabel_levels_per_cut": cut_levels / max(ncuts, 1)},
         },
@@ -193,6 +198,38 @@ def main():
             "mfma_bf16_util": (mfma_flop / scan_sec / 1e12 / PEAK_BF16_TFLOPS) if scan_sec > 0 else 0.0,
         },
     }
+    if args.learned:
+        # learned + batched: same plan, same uniforms; the full scan above is the baseline
+        t0 = time.perf_counter()
+        plan.learn(nsamples=8192, iters=16, seed=1)
+        lib.iq_ctx_synchronize(ctx.handle)
+        learn_s = time.perf_counter() - t0
+        plan.set_proposal(True, K=args.K, check_every=args.check_every, min_recall=0.99)
+        plan.profile(False)
+        for i in range(args.warmup):
+            plan.run(U[i])
+        t0 = time.perf_counter()
+        members = hits = queries = full = 0
+        fell = False
+        ldev = 0.0
+        for i in range(args.steps):
+            plan.run(U[args.warmup + i])
+            ldev += plan.last_time()
+            st = plan.proposal_stats()
+            members += st["band_members"]; hits += st["in_shortlist"]
+            queries += st["queries"]; full += st["fully_recalled"]; fell |= st["fell_back"]
+        lib.iq_ctx_synchronize(ctx.handle)
+        lel = time.perf_counter() - t0
+        full_scan_value = res["value"]
+        res["value"] = voxels / lel
+        res["ms_per_step"] = lel / args.steps * 1e3
+        res["config"]["learned_proposal"] = {
+            "K": args.K, "check_every": args.check_every, "min_recall": 0.99,
+            "recall": hits / members if members else None, "checked_queries": queries,
+            "fully_recalled_queries": full, "fell_back": fell, "learn_seconds": learn_s,
+            "device_ms_per_step": ldev / args.steps * 1e3,
+            "full_scan_value": full_scan_value, "speedup_vs_full_scan": res["value"] / full_scan_value,
+        }
     plan.close()
     if args.cpu_sample > 0 and args.workload == 2:
         res["cpu_baseline"] = cpu_baseline(w, args.cpu_sample)

@@ -11,12 +11,14 @@
 #include <cstring>
 #include <map>
 #include <new>
+#include <random>
 #include <string>
 #include <vector>
 
 #include "k_distance.hip"
 #include "k_select.hip"
 #include "k_cut.hip"
+#include "k_learn.hip"
 
 size_t iq_sort_temp_bytes(int n);
 hipError_t iq_sort_pairs(void* tmp, size_t tmp_bytes, const double* keys_in, double* keys_out,
@@ -322,6 +324,14 @@ struct iq_plan {
   IqQuery* dq = nullptr; // query table (all batches of a run)
   int64_t dq_cap = 0;
   size_t cut_lds_max = 0;
+  bool cut_wave = true;    // every slab fits one wavefront's cut
+  std::vector<int32_t> enabled_host;
+  // learned proposal
+  bool lrn_ready = false, lrn_on = false, lrn_fell_back = false;
+  int lrn_check = 8;
+  double lrn_min_recall = 0.99;
+  int64_t lrn_stats[4] = {0, 0, 0, 0};
+  std::vector<int64_t> lrn_wofs, lrn_bofs;
   float last_ms = 0.0f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // scan profiling
@@ -588,6 +598,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
     if (!d->disabled[c]) en.push_back((int32_t)c);
   pl->nen = (int64_t)en.size();
   H.enabled = pl->mem.upload(en.data(), en.size(), s);
+  pl->enabled_host = en;
   H.nen = pl->nen;
   H.steps = pl->mem.upload(pl->steps.data(), pl->steps.size(), s);
   H.tvox = pl->mem.upload(tvox.data(), tvox.size(), s);
@@ -674,6 +685,16 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
         const double ga = 2.0 * kterms * 0x1p-24 / (1.0 - 2.0 * kterms * 0x1p-24);
         pl->mfma_gerr = 1.02 * ga + (pl->mfma_split ? 4.0 * 0x1p-18 : 0.0);
         H.chunks = pl->mem.upload(chunks.data(), chunks.size(), s);
+        std::vector<int32_t> kofs(chunks.size());
+        for (int b = 0; b < H.ktab.nbox; b++) {
+          const IqKBox& kb = H.ktab.box[b];
+          const int RX = 32 + kb.wxr - 1, RY = 16 + kb.wyr - 1;
+          for (int k = 2 * kb.s0; k < 2 * kb.s1; k++) {
+            const IqChunk& c = chunks[k];
+            kofs[k] = ((c.jz - kb.z0) * RX * RY + (c.jy - kb.y0) * RX + (c.jx - kb.x0)) | (c.dir ? (1 << 30) : 0);
+          }
+        }
+        H.kofs = pl->mem.upload(kofs.data(), kofs.size(), s);
         std::vector<uint8_t> mb((size_t)hmasks.size() * g.tilevox);
         for (size_t m = 0; m < hmasks.size(); m++) std::memcpy(&mb[m * g.tilevox], hmasks[m].bits.data(), g.tilevox);
         H.maskbits = pl->mem.upload(mb.data(), mb.size(), s);
@@ -746,6 +767,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
       maxn = std::max(maxn, n);
     }
   const int64_t cutper = maxn > IQ_PR_NLDS ? maxn * CUT_NODE_BYTES : 0;
+  pl->cut_wave = maxn <= IQ_CUT_WAVE_MAX;
   size_t per_slot = (size_t)nmap * g.tilevox * 8 + (size_t)kmax * 4 + 64 + (size_t)P * (4 + 8 + 4 + 4 + 16) + nw64 * 8 +
                     (size_t)IQ_SORT_MAX * 12 + (size_t)(nw64 + 1) * 8 + (size_t)IQ_MAX_SLAB * g.tilevox +
                     (size_t)IQ_MAX_SLAB * cutper;
@@ -823,8 +845,8 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   HC(hipEventCreate(&pl->e0));
   HC(hipEventCreate(&pl->e1));
   if (pl->cut_lds_max > 64 * 1024) {
-    HC(hipFuncSetAttribute((const void*)k_graphcut<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
-    HC(hipFuncSetAttribute((const void*)k_graphcut<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
+    HC(hipFuncSetAttribute((const void*)k_graphcut<float, CB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
+    HC(hipFuncSetAttribute((const void*)k_graphcut<double, CB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
   }
   HC(hipStreamSynchronize(s));
 }
@@ -848,8 +870,12 @@ void launch_scan_batch(iq_plan* pl, const IqQuery* q, int nq, hipStream_t s) {
 }
 
 // One batch of independent queries: distances, selection, cut and paste.
+// Selection of the plain queries: 0 full scan; 1 learned shortlist; 2 both,
+// the full scan deciding and the shortlist's recall measured on its band.
+enum { SEL_SCAN = 0, SEL_LEARNED = 1, SEL_CHECK = 2 };
+
 template <typename T>
-void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool has_plain, bool has_multi) {
+void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool has_plain, bool has_multi, int sel) {
   const IqGeo& g = pl->geo;
   hipStream_t s = pl->ctx->stream;
   const IqDev* D = pl->dd;
@@ -857,7 +883,22 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
   const int nsrc_prep = has_multi ? 2 + pl->naux : 1;
   hipLaunchKernelGGL((k_query_prep<T>), dim3(nq, nsrc_prep), dim3(256), 0, s, D, q);
   HCL();
-  if (has_plain) {
+  if (has_plain && sel != SEL_SCAN) {
+    hipLaunchKernelGGL(k_learn_embed, dim3(nq), dim3(256), 0, s, D, q);
+    HCL();
+    hipLaunchKernelGGL(k_learn_scores, dim3(nblk(P, 256), nblk(nq, LQG)), dim3(256), 0, s, D, q, nq);
+    HCL();
+    hipLaunchKernelGGL(k_learn_kth, dim3(nq), dim3(1024), 0, s, D, q);
+    HCL();
+  }
+  if (has_plain && sel == SEL_LEARNED) {
+    hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q, 1);
+    HCL();
+    hipLaunchKernelGGL(k_band_recheck, dim3(64, nq), dim3(256), 0, s, D, q);
+    HCL();
+    hipLaunchKernelGGL(k_band_finalize, dim3(nq), dim3(FB), 0, s, D, q);
+    HCL();
+  } else if (has_plain) {
     if (pl->fast) {
       if (pl->prof && pl->prof_used + 2 > (int64_t)pl->prof_ev.size()) {
         hipEvent_t a, b;
@@ -873,8 +914,9 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
         dim3 grid(nblk(g.dist[0], 32), nblk(g.dist[1], 16), (unsigned)(g.dist[2] * ntile));
         if (pl->prof) {
           HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));
-          // 4 wavefronts x 4 row blocks x k-steps x (3 or 1) 32x32x16 MFMAs per group
-          pl->prof_mflop += (double)grid.x * grid.y * grid.z * 16.0 * pl->hd.ktab.nstep * (pl->mfma_split ? 3.0 : 1.0) *
+          // one 32-candidate row block per candidate row inside the grid, per
+          // x block and z x query tile: k-steps x (3 or 1) 32x32x16 MFMAs
+          pl->prof_mflop += (double)grid.x * g.dist[1] * grid.z * pl->hd.ktab.nstep * (pl->mfma_split ? 3.0 : 1.0) *
                             (2.0 * 32 * 32 * 16);
         }
         if (pl->mfma_split)
@@ -897,7 +939,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
         pl->batch_flop.push_back(f);
         pl->prof_flop += f;
       }
-      hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q);
+      hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q, 0);
       HCL();
       hipLaunchKernelGGL(k_band_recheck, dim3(64, nq), dim3(256), 0, s, D, q);
       HCL();
@@ -907,6 +949,10 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     }
     hipLaunchKernelGGL(k_band_finalize, dim3(nq), dim3(FB), 0, s, D, q);
     HCL();
+    if (sel == SEL_CHECK) {
+      hipLaunchKernelGGL(k_learn_recall, dim3(nq), dim3(256), 0, s, D, q);
+      HCL();
+    }
   }
   if (has_multi) {
     hipLaunchKernelGGL(k_multi_maps, dim3(nblk(P, 256), nq, 2 + pl->naux), dim3(256), 0, s, D, q);
@@ -925,7 +971,10 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     hipLaunchKernelGGL(k_relax_finalize, dim3(nq), dim3(FB), 0, s, D, q);
     HCL();
   }
-  hipLaunchKernelGGL((k_graphcut<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB), pl->cut_lds_max, s, D, q);
+  if (pl->cut_wave)
+    hipLaunchKernelGGL((k_graphcut<T, 64>), dim3(IQ_MAX_SLAB, nq), dim3(64), pl->cut_lds_max, s, D, q);
+  else
+    hipLaunchKernelGGL((k_graphcut<T, CB>), dim3(IQ_MAX_SLAB, nq), dim3(CB), pl->cut_lds_max, s, D, q);
   HCL();
   hipLaunchKernelGGL((k_paste<T>), dim3(nblk(g.tilevox, 256), nq), dim3(256), 0, s, D, q);
   HCL();
@@ -963,6 +1012,10 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
           wq.push_back(q);
         }
     }
+    if (pl->lrn_ready && pl->lrn_on)  // queries of one mask side by side: one bank row load serves them
+      std::stable_sort(wq.begin(), wq.end(), [&](const IqQuery& a, const IqQuery& b) {
+        return pl->steps[a.step].mask < pl->steps[b.step].mask;
+      });
     for (size_t b0 = 0; b0 < wq.size(); b0 += (size_t)pl->nslots) {
       Batch bt{(int64_t)qs.size(), 0, false, false};
       for (size_t k = b0; k < wq.size() && k < b0 + (size_t)pl->nslots; k++) {
@@ -989,8 +1042,31 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s, H.cuts, pl->nvox, (uint8_t)0);
   hipLaunchKernelGGL((k_fill<int64_t>), dim3(16), dim3(256), 0, s, H.pick, pl->nslots, (int64_t)-1);
   HCL();
-  for (const Batch& b : batches)
-    run_batch<T>(pl, qs.data() + b.off, pl->dq + b.off, b.n, b.plain, b.multi);
+  bool learned = pl->lrn_ready && pl->lrn_on;
+  pl->lrn_fell_back = false;
+  for (int k = 0; k < 4; k++) pl->lrn_stats[k] = 0;
+  if (learned) {
+    hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(64), 0, s, H.lrn.stats, (int64_t)4, 0ull);
+    HCL();
+  }
+  int64_t nplain = 0;
+  for (const Batch& b : batches) {
+    int sel = SEL_SCAN;
+    if (learned && b.plain) sel = (pl->lrn_check > 0 && nplain++ % pl->lrn_check == 0) ? SEL_CHECK : SEL_LEARNED;
+    run_batch<T>(pl, qs.data() + b.off, pl->dq + b.off, b.n, b.plain, b.multi, sel);
+    if (sel == SEL_CHECK) {
+      // the full scan judged this batch: stop using the shortlist once its
+      // cumulative recall on the exact bands falls below the threshold
+      unsigned long long st[4];
+      HC(hipMemcpyAsync(st, H.lrn.stats, sizeof(st), hipMemcpyDeviceToHost, s));
+      HC(hipStreamSynchronize(s));
+      for (int k = 0; k < 4; k++) pl->lrn_stats[k] = (int64_t)st[k];
+      if (st[0] > 0 && (double)st[1] < pl->lrn_min_recall * (double)st[0]) {
+        learned = false;
+        pl->lrn_fell_back = true;
+      }
+    }
+  }
   HC(hipEventRecord(pl->e1, s));
   HC(hipStreamSynchronize(s));
   HC(hipEventElapsedTime(&pl->last_ms, pl->e0, pl->e1));
@@ -1003,7 +1079,130 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   int32_t code = 0;
   HC(hipMemcpy(&code, H.err, sizeof(int32_t), hipMemcpyDeviceToHost));
   HC(hipMemcpy(pl->cut_stats, H.rounds, 3 * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (pl->lrn_ready && pl->lrn_on) {
+    unsigned long long st[4];
+    HC(hipMemcpy(st, H.lrn.stats, sizeof(st), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 4; k++) pl->lrn_stats[k] = (int64_t)st[k];
+  }
   check_kernel_error(code);
+}
+
+void sgemm(hipStream_t s, int M, int N, int K, const float* A, int64_t sam, int64_t sak, const float* B, int64_t sbn,
+           int64_t sbk, float* C, int64_t ldc) {
+  hipLaunchKernelGGL(k_sgemm, dim3(nblk(N, LG_T), nblk(M, LG_T)), dim3(256), 0, s, M, N, K, A, sam, sak, B, sbn, sbk, C,
+                     ldc);
+  HCL();
+}
+
+// Rows of Y (64 x V) made orthonormal into W: W = L^-1 Y with Y Y^T = L L^T
+// (Cholesky in FP64 on the host), applied twice.
+void orthonormalize(hipStream_t s, DMem& tmp, const float* Y, float* W, int V) {
+  float* G = tmp.alloc<float>(IQ_EMB * IQ_EMB);
+  float* Li = tmp.alloc<float>(IQ_EMB * IQ_EMB);
+  float* Y1 = tmp.alloc<float>((size_t)IQ_EMB * V);
+  const float* src = Y;
+  for (int pass = 0; pass < 2; pass++) {
+    float* dst = pass == 0 ? Y1 : W;
+    sgemm(s, IQ_EMB, IQ_EMB, V, src, V, 1, src, V, 1, G, IQ_EMB);
+    std::vector<float> gh(IQ_EMB * IQ_EMB);
+    HC(hipMemcpyAsync(gh.data(), G, gh.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    HC(hipStreamSynchronize(s));
+    std::vector<double> L(IQ_EMB * IQ_EMB, 0.0);
+    double tr = 0.0;
+    for (int i = 0; i < IQ_EMB; i++) tr += gh[i * IQ_EMB + i];
+    const double ridge = 1e-9 * tr / IQ_EMB + 1e-30;
+    for (int i = 0; i < IQ_EMB; i++)
+      for (int j = 0; j <= i; j++) {
+        double v = 0.5 * ((double)gh[i * IQ_EMB + j] + (double)gh[j * IQ_EMB + i]) + (i == j ? ridge : 0.0);
+        for (int k = 0; k < j; k++) v -= L[i * IQ_EMB + k] * L[j * IQ_EMB + k];
+        if (i == j) {
+          if (!(v > 0.0)) bad(IQ_ERR_UNSUPPORTED, "learned proposal: training windows span fewer than 64 directions");
+          L[i * IQ_EMB + i] = std::sqrt(v);
+        } else {
+          L[i * IQ_EMB + j] = v / L[j * IQ_EMB + j];
+        }
+      }
+    std::vector<float> li(IQ_EMB * IQ_EMB, 0.0f);  // inverse of the lower-triangular L
+    for (int c = 0; c < IQ_EMB; c++) {
+      std::vector<double> x(IQ_EMB, 0.0);
+      for (int i = 0; i < IQ_EMB; i++) {
+        double v = i == c ? 1.0 : 0.0;
+        for (int k = 0; k < i; k++) v -= L[i * IQ_EMB + k] * x[k];
+        x[i] = v / L[i * IQ_EMB + i];
+      }
+      for (int i = 0; i < IQ_EMB; i++) li[i * IQ_EMB + c] = (float)x[i];
+    }
+    HC(hipMemcpyAsync(Li, li.data(), li.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    sgemm(s, IQ_EMB, V, IQ_EMB, Li, IQ_EMB, 1, src, 1, V, dst, V);
+    HC(hipStreamSynchronize(s));
+    src = dst;
+  }
+}
+
+// Encoders of every overlap mask from nsamp training-image windows (iters
+// rounds of block subspace iteration Y = X^T X W^T, rows orthonormalized),
+// then the bank of every candidate's encoding.
+void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
+  hipStream_t s = pl->ctx->stream;
+  IqDev& H = pl->hd;
+  const IqGeo& g = pl->geo;
+  const int nm = (int)pl->masks.size();
+  if (nm == 0) bad(IQ_ERR_ARG, "learned proposal: the plan has no overlap mask");
+  if (nsamp < IQ_EMB || iters < 0) bad(IQ_ERR_ARG, "learned proposal: need at least 64 samples");
+  if (pl->enabled_host.empty()) bad(IQ_ERR_NOCAND, "every candidate pattern is disabled");
+  if (!pl->lrn_ready) {
+    std::vector<int64_t> wofs(nm), bofs(nm);
+    size_t wtot = 0;
+    for (int m = 0; m < nm; m++) {
+      wofs[m] = (int64_t)wtot;
+      wtot += (size_t)IQ_EMB * pl->masks[m].V;
+      bofs[m] = (int64_t)m * IQ_EMB * g.P;
+    }
+    H.lrn.W = pl->mem.alloc<float>(wtot);
+    H.lrn.bank = pl->mem.alloc<float>((size_t)nm * IQ_EMB * g.P);
+    H.lrn.wofs = pl->mem.upload(wofs.data(), wofs.size(), s);
+    H.lrn.bofs = pl->mem.upload(bofs.data(), bofs.size(), s);
+    H.lrn.eq = pl->mem.alloc<float>((size_t)pl->nslots * IQ_EMB);
+    H.lrn.lsc = pl->mem.alloc<float>((size_t)pl->nslots * g.P);
+    H.lrn.thr = pl->mem.alloc<uint32_t>(pl->nslots);
+    H.lrn.stats = pl->mem.alloc<unsigned long long>(4);
+    pl->lrn_wofs = wofs;
+    pl->lrn_bofs = bofs;
+  }
+  std::mt19937_64 rng(seed);
+  const int64_t n = nsamp;
+  std::vector<int32_t> cands((size_t)n);
+  std::uniform_int_distribution<size_t> pick(0, pl->enabled_host.size() - 1);
+  for (int64_t i = 0; i < n; i++) cands[i] = pl->enabled_host[pick(rng)];
+  DMem tmp;
+  int32_t* dc = tmp.upload(cands.data(), cands.size(), s);
+  std::normal_distribution<float> nrm(0.0f, 1.0f);
+  for (int m = 0; m < nm; m++) {
+    const int V = pl->masks[m].V;
+    DMem it;
+    float* X = it.alloc<float>((size_t)n * V);
+    float* Z = it.alloc<float>((size_t)n * IQ_EMB);
+    float* Y = it.alloc<float>((size_t)IQ_EMB * V);
+    float* Wm = (float*)H.lrn.W + pl->lrn_wofs[m];
+    hipLaunchKernelGGL(k_learn_gather, dim3((unsigned)n), dim3(256), 0, s, (const IqDev*)pl->dd, m, (const int32_t*)dc, X);
+    HCL();
+    std::vector<float> w0((size_t)IQ_EMB * V);
+    for (float& v : w0) v = nrm(rng);
+    HC(hipMemcpyAsync(Y, w0.data(), w0.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    orthonormalize(s, it, Y, Wm, V);
+    for (int r = 0; r < iters; r++) {
+      sgemm(s, (int)n, IQ_EMB, V, X, V, 1, Wm, V, 1, Z, IQ_EMB);      // Z = X W^T
+      sgemm(s, IQ_EMB, V, (int)n, Z, 1, IQ_EMB, X, 1, V, Y, V);       // Y = Z^T X
+      orthonormalize(s, it, Y, Wm, V);
+    }
+    hipLaunchKernelGGL(k_learn_bank, dim3(nblk(g.P, 256)), dim3(256), 0, s, (const IqDev*)pl->dd, m, (const float*)Wm,
+                       (float*)H.lrn.bank + pl->lrn_bofs[m]);
+    HCL();
+    HC(hipStreamSynchronize(s));
+  }
+  pl->lrn_ready = true;
+  HC(hipMemcpyAsync(pl->dd, &pl->hd, sizeof(IqDev), hipMemcpyHostToDevice, s));
+  HC(hipStreamSynchronize(s));
 }
 
 }  // namespace
@@ -1373,6 +1572,52 @@ int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double
   if (seconds) *seconds = plan->prof_sec;
   if (flop) *flop = plan->prof_flop;
   if (mfma_flop) *mfma_flop = plan->prof_mflop;
+  return IQ_OK;
+}
+
+int iq_plan_learn(iq_ctx* ctx, iq_plan* plan, int64_t nsamples, int iters, uint64_t seed) {
+  if (!ctx || !plan) return fail(ctx, IQ_ERR_ARG, "null context or plan");
+  return guard(ctx, [&] {
+    if (plan->ctx != ctx) bad(IQ_ERR_ARG, "plan belongs to another context");
+    plan_learn(plan, nsamples, iters, seed);
+  });
+}
+
+int iq_plan_set_proposal(iq_plan* plan, int enable, int64_t K, int check_every, double min_recall) {
+  if (!plan) return fail(nullptr, IQ_ERR_ARG, "null plan");
+  return guard(plan->ctx, [&] {
+    if (enable && !plan->lrn_ready) bad(IQ_ERR_ARG, "call iq_plan_learn first");
+    if (K < 1 || check_every < 0 || !(min_recall >= 0.0 && min_recall <= 1.0)) bad(IQ_ERR_ARG, "invalid proposal settings");
+    plan->lrn_on = enable != 0;
+    plan->lrn_check = check_every;
+    plan->lrn_min_recall = min_recall;
+    plan->hd.lrn.K = (int32_t)std::min<int64_t>(K, plan->geo.P);
+    hipStream_t s = plan->ctx->stream;
+    HC(hipMemcpyAsync(plan->dd, &plan->hd, sizeof(IqDev), hipMemcpyHostToDevice, s));
+    HC(hipStreamSynchronize(s));
+  });
+}
+
+int iq_plan_encoder(iq_plan* plan, int mask, float* W, int32_t* tvox, int64_t* V, int* nmasks) {
+  if (!plan) return fail(nullptr, IQ_ERR_ARG, "null plan");
+  return guard(plan->ctx, [&] {
+    if (nmasks) *nmasks = (int)plan->masks.size();
+    if (mask < 0 || mask >= (int)plan->masks.size()) bad(IQ_ERR_ARG, "mask index out of range");
+    if (V) *V = plan->masks[mask].V;
+    if (tvox) HC(hipMemcpy(tvox, plan->hd.tvox + plan->masks[mask].off, (size_t)plan->masks[mask].V * sizeof(int32_t),
+                           hipMemcpyDeviceToHost));
+    if (W) {
+      if (!plan->lrn_ready) bad(IQ_ERR_ARG, "call iq_plan_learn first");
+      HC(hipMemcpy(W, plan->hd.lrn.W + plan->lrn_wofs[mask], (size_t)IQ_EMB * plan->masks[mask].V * sizeof(float),
+                   hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+int iq_plan_proposal_stats(iq_plan* plan, int64_t* out) {
+  if (!plan || !out) return fail(nullptr, IQ_ERR_ARG, "null argument");
+  for (int k = 0; k < 4; k++) out[k] = plan->lrn_stats[k];
+  out[4] = plan->lrn_fell_back ? 1 : 0;
   return IQ_OK;
 }
 

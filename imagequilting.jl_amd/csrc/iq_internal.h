@@ -12,6 +12,7 @@
 #define IQ_PR_LIMIT 200000  // push-relabel round guard (never reached on valid input)
 #define IQ_SORT_MAX 4096    // largest multi-valued band ranked in LDS
 #define IQ_PR_NLDS 1536     // interior min-cut nodes kept in LDS (above: global scratch)
+#define IQ_CUT_WAVE_MAX 512 // slabs up to this many interior nodes are cut by one wavefront
 
 // error codes written by kernels into the per-run flag word (first error wins)
 #define IQ_KERR_NOCAND 1
@@ -79,6 +80,24 @@ struct IqKBox {
 struct IqKTab {
   int32_t nbox, nstep;
   IqKBox box[IQ_MAX_KBOX];
+};
+
+// Learned proposal: per overlap-mask shape a linear encoder e(x) = W x of
+// the masked values (IQ_EMB outputs) and the bank of encoded candidate
+// windows; approximate distances A2 - 2 e(x_c).e(q) + B2 shortlist the K
+// best candidates of a query, which the exact recheck then reranks.
+#define IQ_EMB 64
+struct IqLearn {
+  int32_t on;                 // shortlist from the bank instead of the full scan
+  int32_t K;                  // shortlist size
+  const float* W;             // encoders: IQ_EMB x V of each mask, row-major, at wofs[mask]
+  const int64_t* wofs;        // per mask (-1: no encoder)
+  const float* bank;          // P x IQ_EMB per encoded mask, at bofs[mask]
+  const int64_t* bofs;
+  float* eq;                  // per slot: the query's encoding
+  float* lsc;                 // per slot: P approximate distances
+  uint32_t* thr;              // per slot: order key of the K-th smallest approximate distance
+  unsigned long long* stats;  // recall check: band members, members in the shortlist, queries, fully recalled
 };
 
 // Geometry shared by all kernels of a plan.
@@ -158,10 +177,19 @@ struct IqDev {
   // MFMA scan
   IqKTab ktab;
   const IqChunk* chunks;
+  const int32_t* kofs;                     // per chunk: offset in its box's LDS stage, bit 30: run along y
   const uint8_t* maskbits;                 // per mask: tilevox membership bytes
   uint4* bh;                               // B fragments (hi), [ntile][nstep][64]
   uint4* bl;                               // B fragments (lo)
+  IqLearn lrn;
 };
+
+// Order-preserving 32-bit key of a non-NaN float (-0 and +0 share a key).
+__host__ __device__ inline uint32_t iq_f32_key(float x) {
+  union { float f; uint32_t u; } c;
+  c.f = x == 0.0f ? 0.0f : x;
+  return (c.u >> 31) ? ~c.u : (c.u | 0x80000000u);
+}
 
 __host__ __device__ inline int64_t iq_lin3(const int32_t d[3], int64_t x, int64_t y, int64_t z) {
   return x + (int64_t)d[0] * (y + (int64_t)d[1] * z);

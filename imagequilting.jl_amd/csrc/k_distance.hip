@@ -505,17 +505,32 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
       lw[i] = iq_split_word(v);
     }
     __syncthreads();
+    // B fragments and chunk offsets are loaded one k-step ahead
+    const uint4* __restrict__ bhp = D->bh + (int64_t)tile * nstep * 64 + lane;
+    const uint4* __restrict__ blp = D->bl + (int64_t)tile * nstep * 64 + lane;
+    const int32_t* __restrict__ kofs = D->kofs;
+    uint4 bh_n = {0u, 0u, 0u, 0u}, bl_n = {0u, 0u, 0u, 0u};
+    int ko_n = 0;
+    if (kbx.s0 < kbx.s1) {
+      bh_n = bhp[(int64_t)kbx.s0 * 64];
+      if (SPLIT) bl_n = blp[(int64_t)kbx.s0 * 64];
+      ko_n = kofs[2 * kbx.s0 + kb];
+    }
     for (int step = kbx.s0; step < kbx.s1; step++) {
-      const int64_t o = ((int64_t)tile * nstep + step) * 64 + lane;
-      const uint4 bhv = D->bh[o];
-      iq_bf16x8 bh = __builtin_bit_cast(iq_bf16x8, bhv);
+      const iq_bf16x8 bh = __builtin_bit_cast(iq_bf16x8, bh_n);
       iq_bf16x8 bl;
-      if (SPLIT) bl = __builtin_bit_cast(iq_bf16x8, D->bl[o]);
-      const IqChunk ch = D->chunks[2 * step + kb];
-      const int es = ch.dir == 0 ? 1 : RX;
-      const int base = (ch.jz - kbx.z0) * RX * RY + (ch.jy - kbx.y0 + 4 * w) * RX + (ch.jx - kbx.x0) + row;
+      if (SPLIT) bl = __builtin_bit_cast(iq_bf16x8, bl_n);
+      const int ko = ko_n;
+      if (step + 1 < kbx.s1) {
+        bh_n = bhp[(int64_t)(step + 1) * 64];
+        if (SPLIT) bl_n = blp[(int64_t)(step + 1) * 64];
+        ko_n = kofs[2 * (step + 1) + kb];
+      }
+      const int es = (ko >> 30) ? RX : 1;
+      const int base = (ko & 0x3fffffff) + 4 * w * RX + row;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
+        if (cy0 + 4 * w + i >= g.dist[1]) break;  // candidate rows past the grid (uniform per wavefront)
         const int a0 = base + i * RX;
         uint32_t wv[8];
 #pragma unroll
@@ -602,22 +617,27 @@ IQ_SCAN_INST(4, 8, 16, 2)
 
 // ---------------------------------------------------------------------------
 // Band superset: lo <= T where T = round_up(fl(fl(1+tol) * U)) and U is the
-// least upper bound of the scan, so every exact band member is selected.
+// least upper bound of the scan, so every exact band member is selected
+// (learned != 0: the learned proposal's shortlist instead).
 // One bitmap bit per candidate (ballot words of 64 candidates) and an
 // unordered list of the members.  Grid: (candidate blocks, query).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int learned) {
   const IqQuery q = qs[blockIdx.y];
   const IqStep& st = D->steps[q.step];
   if (q.multi || st.mask < 0) return;
   const IqGeo& g = D->g;
   const int64_t slot = q.slot;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const double U = (double)__uint_as_float(D->gmin_hi[slot]);
-  const float Tf = iq_f32_up((1.0 + D->tol) * U);
   bool sel = false;
-  if (c < g.P) sel = D->lo32[slot * g.P + c] <= Tf;
+  if (learned) {  // shortlist of the learned proposal (its threshold key)
+    if (c < g.P) sel = iq_f32_key(D->lrn.lsc[slot * g.P + c]) <= D->lrn.thr[slot];
+  } else {
+    const double U = (double)__uint_as_float(D->gmin_hi[slot]);
+    const float Tf = iq_f32_up((1.0 + D->tol) * U);
+    if (c < g.P) sel = D->lo32[slot * g.P + c] <= Tf;
+  }
   const unsigned long long bal = __ballot(sel);
   const int lane = threadIdx.x & 63;
   if (lane == 0 && (c >> 6) < D->nw64) D->bitmap[slot * D->nw64 + (c >> 6)] = bal;

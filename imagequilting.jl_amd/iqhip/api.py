@@ -303,6 +303,39 @@ class Plan:
                                                      v[2:].ctypes.data))
         return int(v[0]), int(v[1]), int(v[2])
 
+    # -- learned proposal (no counterpart in the reference)
+    def learn(self, nsamples: int = 8192, iters: int = 16, seed: int = 0) -> None:
+        """Learn the per-mask encoders and encode every candidate (iq_plan_learn)."""
+        self.ctx.check(self.ctx._L.iq_plan_learn(self.ctx.handle, self.handle, int(nsamples), int(iters), int(seed)))
+
+    def set_proposal(self, enable: bool = True, K: int = 4096, check_every: int = 8, min_recall: float = 0.99) -> None:
+        """Shortlist the K best approximate distances instead of the full scan;
+        every check_every-th batch the full scan decides and measures recall."""
+        self.ctx.check(self.ctx._L.iq_plan_set_proposal(self.handle, int(bool(enable)), int(K), int(check_every),
+                                                        float(min_recall)))
+
+    def proposal_stats(self) -> dict:
+        v = np.zeros(5, dtype=np.int64)
+        self.ctx.check(self.ctx._L.iq_plan_proposal_stats(self.handle, v.ctypes.data))
+        members, hits, queries, full, fell = (int(x) for x in v)
+        return dict(band_members=members, in_shortlist=hits, queries=queries, fully_recalled=full,
+                    recall=hits / members if members else float("nan"), fell_back=bool(fell))
+
+    def encoder(self, mask: int):
+        """(W, tvox): encoder of one mask shape (64 x V) and the mask's
+        column-major tile offsets (ascending)."""
+        V = ctypes.c_int64(0)
+        self.ctx.check(self.ctx._L.iq_plan_encoder(self.handle, int(mask), None, None, ctypes.byref(V), None))
+        W = np.zeros((64, V.value), dtype=np.float32)
+        tv = np.zeros(V.value, dtype=np.int32)
+        self.ctx.check(self.ctx._L.iq_plan_encoder(self.handle, int(mask), W.ctypes.data, tv.ctypes.data, None, None))
+        return W, tv
+
+    def nmasks(self) -> int:
+        nm = ctypes.c_int(0)
+        self.ctx._L.iq_plan_encoder(self.handle, 0, None, None, None, ctypes.byref(nm))
+        return nm.value
+
     def results(self, grids=True, chosen=True, cuts=False):
         """Per realization: padded grids, chosen picks (npath), cut grids."""
         vox = [int(np.prod(p)) for p in self.real_padsize]

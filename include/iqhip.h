@@ -194,6 +194,35 @@ int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double
  * kernel; slabs beyond its size limit are not counted).  Diagnostic only. */
 int iq_plan_cut_stats(iq_plan* plan, int64_t* rounds, int64_t* levels, int64_t* cuts);
 
+/* ---- learned proposal (no counterpart in the reference) ----------------
+ * For plans with many candidate patterns: a shortlist of the K best
+ * approximate distances of each plain query replaces the full scan.  The
+ * shortlist is reranked with exact FP64 distances, so a pick always comes
+ * from exact distances, but a band member outside the shortlist is missed.
+ * Sampled batches also run the full scan, which decides them and measures
+ * the recall of the shortlist on the exact band.
+ *
+ * iq_plan_learn: one linear encoder (64 outputs) per overlap-mask shape of
+ * the plan, learned from nsamples training-image windows by iters rounds of
+ * block subspace iteration (seeded; iters = 0 leaves a random orthonormal
+ * encoder), and the bank of every candidate's encoding (256 bytes per
+ * candidate and mask shape).
+ * iq_plan_set_proposal: use the shortlist in iq_plan_run (enable), its size
+ * K, a full-scan check every check_every-th batch (0: never), and return to
+ * the full scan for the rest of a run once the cumulative recall of the
+ * checks falls below min_recall.
+ * iq_plan_proposal_stats: for the last run, out[0] band members of the
+ * checked queries, out[1] those inside the shortlist, out[2] checked
+ * queries, out[3] queries with the whole band inside, out[4] 1 if the run
+ * fell back to the full scan.
+ * iq_plan_encoder: the encoder of mask shape `mask` (64 x V row-major over
+ * the V masked voxels, whose 0-based column-major tile offsets, ascending,
+ * go to tvox); any output may be NULL; *nmasks = number of mask shapes. */
+int iq_plan_learn(iq_ctx* ctx, iq_plan* plan, int64_t nsamples, int iters, uint64_t seed);
+int iq_plan_set_proposal(iq_plan* plan, int enable, int64_t K, int check_every, double min_recall);
+int iq_plan_proposal_stats(iq_plan* plan, int64_t out[5]);
+int iq_plan_encoder(iq_plan* plan, int mask, float* W, int32_t* tvox, int64_t* V, int* nmasks);
+
 /* Convenience: create + run + results + destroy. */
 int iq_simulate(iq_ctx* ctx, const iq_plan_desc* desc, const double* uniforms, const int64_t* forced,
                 void* out, int64_t* chosen, double* cutgrid);
