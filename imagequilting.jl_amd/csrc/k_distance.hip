@@ -632,7 +632,7 @@ k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int l
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool sel = false;
   if (learned) {  // shortlist of the learned proposal (its threshold key)
-    if (c < g.P) sel = iq_f32_key(D->lrn.lsc[slot * g.P + c]) <= D->lrn.thr[slot];
+    if (c < g.P) sel = !D->disabled[c] && iq_f32_key(D->lrn.lsc[slot * g.P + c]) <= D->lrn.thr[slot];
   } else {
     const double U = (double)__uint_as_float(D->gmin_hi[slot]);
     const float Tf = iq_f32_up((1.0 + D->tol) * U);
