@@ -324,7 +324,6 @@ struct iq_plan {
   IqQuery* dq = nullptr; // query table (all batches of a run)
   int64_t dq_cap = 0;
   size_t cut_lds_max = 0;
-  bool cut_wave = true;    // every slab fits one wavefront's cut
   std::vector<int32_t> enabled_host;
   // learned proposal
   bool lrn_ready = false, lrn_on = false, lrn_fell_back = false;
@@ -767,7 +766,6 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
       maxn = std::max(maxn, n);
     }
   const int64_t cutper = maxn > IQ_PR_NLDS ? maxn * CUT_NODE_BYTES : 0;
-  pl->cut_wave = maxn <= IQ_CUT_WAVE_MAX;
   size_t per_slot = (size_t)nmap * g.tilevox * 8 + (size_t)kmax * 4 + 64 + (size_t)P * (4 + 8 + 4 + 4 + 16) + nw64 * 8 +
                     (size_t)IQ_SORT_MAX * 12 + (size_t)(nw64 + 1) * 8 + (size_t)IQ_MAX_SLAB * g.tilevox +
                     (size_t)IQ_MAX_SLAB * cutper;
@@ -845,8 +843,8 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   HC(hipEventCreate(&pl->e0));
   HC(hipEventCreate(&pl->e1));
   if (pl->cut_lds_max > 64 * 1024) {
-    HC(hipFuncSetAttribute((const void*)k_graphcut<float, CB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
-    HC(hipFuncSetAttribute((const void*)k_graphcut<double, CB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
+    HC(hipFuncSetAttribute((const void*)k_graphcut<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
+    HC(hipFuncSetAttribute((const void*)k_graphcut<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
   }
   HC(hipStreamSynchronize(s));
 }
@@ -971,10 +969,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     hipLaunchKernelGGL(k_relax_finalize, dim3(nq), dim3(FB), 0, s, D, q);
     HCL();
   }
-  if (pl->cut_wave)
-    hipLaunchKernelGGL((k_graphcut<T, 64>), dim3(IQ_MAX_SLAB, nq), dim3(64), pl->cut_lds_max, s, D, q);
-  else
-    hipLaunchKernelGGL((k_graphcut<T, CB>), dim3(IQ_MAX_SLAB, nq), dim3(CB), pl->cut_lds_max, s, D, q);
+  hipLaunchKernelGGL((k_graphcut<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB), pl->cut_lds_max, s, D, q);
   HCL();
   hipLaunchKernelGGL((k_paste<T>), dim3(nblk(g.tilevox, 256), nq), dim3(256), 0, s, D, q);
   HCL();

@@ -12,7 +12,6 @@
 #define IQ_PR_LIMIT 200000  // push-relabel round guard (never reached on valid input)
 #define IQ_SORT_MAX 4096    // largest multi-valued band ranked in LDS
 #define IQ_PR_NLDS 1536     // interior min-cut nodes kept in LDS (above: global scratch)
-#define IQ_CUT_WAVE_MAX 512 // slabs up to this many interior nodes are cut by one wavefront
 
 // error codes written by kernels into the per-run flag word (first error wins)
 #define IQ_KERR_NOCAND 1

@@ -286,24 +286,12 @@ struct CutLds {
 
 #define CUT_FAST_NODE_BYTES (8 * 12 + 4 * 2)
 
-// Block-wide OR with the memory ordering of a barrier; a one-wavefront team
-// needs no cross-wavefront reduction.
-template <int TEAM>
-__device__ inline bool team_or(bool x) {
-  if constexpr (TEAM == 64) {
-    __syncthreads();
-    return __ballot(x) != 0ull;
-  } else {
-    return __syncthreads_or(x) != 0;
-  }
-}
-
-template <typename T, int NPT, int TEAM>
+template <typename T, int NPT>
 __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
                               uint8_t* out, char* mem, int32_t* err, int64_t* rounds) {
   const int L = s[dim];
   const int nvox = s[0] * s[1] * s[2];
-  for (int p = threadIdx.x; p < nvox; p += TEAM) {
+  for (int p = threadIdx.x; p < nvox; p += CB) {
     int c[3] = {p % s[0], (p / s[0]) % s[1], p / (s[0] * s[1])};
     uint8_t m = (c[dim] == L - 1) ? 0 : 1;
     out[p] = m ^ side;
@@ -329,7 +317,7 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
   double e[NPT], rt[NPT];
 #pragma unroll
   for (int k = 0; k < NPT; k++) {
-    const int u = threadIdx.x + k * TEAM;
+    const int u = threadIdx.x + k * CB;
     uu[k] = u;
     fl[k] = 0;
     e[k] = 0.0;
@@ -396,13 +384,13 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
         if (hit) { h[u] = lev + 1; ch = 1; }
       }
       levels++;
-      if (!team_or<TEAM>(ch)) break;
+      if (!__syncthreads_or(ch)) break;
     }
     int act = 0;
 #pragma unroll
     for (int k = 0; k < NPT; k++)
       if (uu[k] < n) act |= (e[k] > 0.0 && h[uu[k]] < HMAX);
-    if (!team_or<TEAM>(act)) break;
+    if (!__syncthreads_or(act)) break;
     if (total > IQ_PR_LIMIT) {
       if (threadIdx.x == 0) atomicCAS(err, 0, IQ_KERR_ITER);
       break;
@@ -455,7 +443,7 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
       }
       if (it > 0) {
         total++;
-        if (!team_or<TEAM>(a2)) break;
+        if (!__syncthreads_or(a2)) break;
       } else {
         __syncthreads();
       }
@@ -544,31 +532,18 @@ template <typename T>
 __device__ void cut_dispatch(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
                              uint8_t* out, char* lds, char* gmem, int32_t* err, int64_t* rounds) {
   const int n = s[0] * s[1] * s[2] / s[dim] * (s[dim] > 2 ? s[dim] - 2 : 0);
-  if (n <= CB) cut_slab_fast<T, 1, CB>(A, B, s, dim, side, out, lds, err, rounds);
-  else if (n <= 2 * CB) cut_slab_fast<T, 2, CB>(A, B, s, dim, side, out, lds, err, rounds);
-  else if (n <= 4 * CB) cut_slab_fast<T, 4, CB>(A, B, s, dim, side, out, lds, err, rounds);
+  if (n <= CB) cut_slab_fast<T, 1>(A, B, s, dim, side, out, lds, err, rounds);
+  else if (n <= 2 * CB) cut_slab_fast<T, 2>(A, B, s, dim, side, out, lds, err, rounds);
+  else if (n <= 4 * CB) cut_slab_fast<T, 4>(A, B, s, dim, side, out, lds, err, rounds);
   else if (n <= IQ_PR_NLDS) cut_slab<T>(A, B, s, dim, side, out, lds, err, rounds);
   else cut_slab<T>(A, B, s, dim, side, out, gmem, err, rounds);
 }
 
-// One wavefront per slab of at most IQ_CUT_WAVE_MAX interior nodes: the
-// phase barriers of the synchronous schedule cost no cross-wavefront wait.
-template <typename T>
-__device__ void cut_dispatch_wave(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
-                                  uint8_t* out, char* lds, int32_t* err, int64_t* rounds) {
-  const int n = s[0] * s[1] * s[2] / s[dim] * (s[dim] > 2 ? s[dim] - 2 : 0);
-  if (n <= 64) cut_slab_fast<T, 1, 64>(A, B, s, dim, side, out, lds, err, rounds);
-  else if (n <= 128) cut_slab_fast<T, 2, 64>(A, B, s, dim, side, out, lds, err, rounds);
-  else if (n <= 256) cut_slab_fast<T, 4, 64>(A, B, s, dim, side, out, lds, err, rounds);
-  else cut_slab_fast<T, 8, 64>(A, B, s, dim, side, out, lds, err, rounds);
-}
 
-// Batched cut: grid (slab, query), TEAM threads per slab (64: every slab of
-// the plan has at most IQ_CUT_WAVE_MAX interior nodes).  A = the
-// realization's grid over the tile, B = the picked training-image window
-// (iqsim.jl:244-273).
-template <typename T, int TEAM>
-__global__ void __launch_bounds__(TEAM)
+// Batched cut: grid (slab, query).  A = the realization's grid over the
+// tile, B = the picked training-image window (iqsim.jl:244-273).
+template <typename T>
+__global__ void __launch_bounds__(CB)
 k_graphcut(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   extern __shared__ char cut_lds[];
   const IqQuery q = qs[blockIdx.y];
@@ -591,14 +566,11 @@ k_graphcut(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   B.st[0] = 1; B.st[1] = g.ti[0]; B.st[2] = (int64_t)g.ti[0] * g.ti[1];
   uint8_t* out = D->keep + (slot * IQ_MAX_SLAB + blockIdx.x) * g.tilevox;
   char* gmem = D->cutscratch ? D->cutscratch + (slot * IQ_MAX_SLAB + blockIdx.x) * D->cutscratch_per : nullptr;
-  if constexpr (TEAM == 64) cut_dispatch_wave<T>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, D->err, D->rounds);
-  else cut_dispatch<T>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, gmem, D->err, D->rounds);
+  cut_dispatch<T>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, gmem, D->err, D->rounds);
 }
 
-template __global__ void k_graphcut<float, 64>(const IqDev*, const IqQuery*);
-template __global__ void k_graphcut<double, 64>(const IqDev*, const IqQuery*);
-template __global__ void k_graphcut<float, CB>(const IqDev*, const IqQuery*);
-template __global__ void k_graphcut<double, CB>(const IqDev*, const IqQuery*);
+template __global__ void k_graphcut<float>(const IqDev*, const IqQuery*);
+template __global__ void k_graphcut<double>(const IqDev*, const IqQuery*);
 
 // Standalone cut of one slab pair (iq_graphcut): A and B both column-major of extents s.
 template <typename T>
