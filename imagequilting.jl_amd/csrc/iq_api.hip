@@ -324,6 +324,7 @@ struct iq_plan {
   IqQuery* dq = nullptr; // query table (all batches of a run)
   int64_t dq_cap = 0;
   size_t cut_lds_max = 0;
+  bool cut_big = false;    // largest slab needs the 512-lane register-resident cut
   std::vector<int32_t> enabled_host;
   // learned proposal
   bool lrn_ready = false, lrn_on = false, lrn_fell_back = false;
@@ -762,10 +763,12 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
     for (int q = 0; q < st.nslab; q++) {
       const IqSlab& sl = st.slab[q];
       int64_t n = (int64_t)sl.sz[0] * sl.sz[1] * sl.sz[2] / sl.sz[sl.dim] * std::max(0, sl.sz[sl.dim] - 2);
-      if (n <= IQ_PR_NLDS) pl->cut_lds_max = std::max(pl->cut_lds_max, (size_t)n * CUT_FAST_NODE_BYTES);
+      if (n <= std::max<int64_t>(IQ_PR_NLDS, CUT_BIG_MAX))
+        pl->cut_lds_max = std::max(pl->cut_lds_max, (size_t)n * CUT_FAST_NODE_BYTES);
       maxn = std::max(maxn, n);
     }
-  const int64_t cutper = maxn > IQ_PR_NLDS ? maxn * CUT_NODE_BYTES : 0;
+  pl->cut_big = maxn > 4 * CB && maxn <= CUT_BIG_MAX;
+  const int64_t cutper = (maxn > IQ_PR_NLDS && !pl->cut_big) ? maxn * CUT_NODE_BYTES : 0;
   size_t per_slot = (size_t)nmap * g.tilevox * 8 + (size_t)kmax * 4 + 64 + (size_t)P * (4 + 8 + 4 + 4 + 16) + nw64 * 8 +
                     (size_t)IQ_SORT_MAX * 12 + (size_t)(nw64 + 1) * 8 + (size_t)IQ_MAX_SLAB * g.tilevox +
                     (size_t)IQ_MAX_SLAB * cutper;
@@ -845,6 +848,8 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   if (pl->cut_lds_max > 64 * 1024) {
     HC(hipFuncSetAttribute((const void*)k_graphcut<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
     HC(hipFuncSetAttribute((const void*)k_graphcut<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
+    HC(hipFuncSetAttribute((const void*)k_graphcut_big<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
+    HC(hipFuncSetAttribute((const void*)k_graphcut_big<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
   }
   HC(hipStreamSynchronize(s));
 }
@@ -969,7 +974,10 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     hipLaunchKernelGGL(k_relax_finalize, dim3(nq), dim3(FB), 0, s, D, q);
     HCL();
   }
-  hipLaunchKernelGGL((k_graphcut<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB), pl->cut_lds_max, s, D, q);
+  if (pl->cut_big)
+    hipLaunchKernelGGL((k_graphcut_big<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB_BIG), pl->cut_lds_max, s, D, q);
+  else
+    hipLaunchKernelGGL((k_graphcut<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB), pl->cut_lds_max, s, D, q);
   HCL();
   hipLaunchKernelGGL((k_paste<T>), dim3(nblk(g.tilevox, 256), nq), dim3(256), 0, s, D, q);
   HCL();

@@ -286,12 +286,12 @@ struct CutLds {
 
 #define CUT_FAST_NODE_BYTES (8 * 12 + 4 * 2)
 
-template <typename T, int NPT>
+template <typename T, int NPT, int TEAM>
 __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
                               uint8_t* out, char* mem, int32_t* err, int64_t* rounds) {
   const int L = s[dim];
   const int nvox = s[0] * s[1] * s[2];
-  for (int p = threadIdx.x; p < nvox; p += CB) {
+  for (int p = threadIdx.x; p < nvox; p += TEAM) {
     int c[3] = {p % s[0], (p / s[0]) % s[1], p / (s[0] * s[1])};
     uint8_t m = (c[dim] == L - 1) ? 0 : 1;
     out[p] = m ^ side;
@@ -317,7 +317,7 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
   double e[NPT], rt[NPT];
 #pragma unroll
   for (int k = 0; k < NPT; k++) {
-    const int u = threadIdx.x + k * CB;
+    const int u = threadIdx.x + k * TEAM;
     uu[k] = u;
     fl[k] = 0;
     e[k] = 0.0;
@@ -532,9 +532,9 @@ template <typename T>
 __device__ void cut_dispatch(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
                              uint8_t* out, char* lds, char* gmem, int32_t* err, int64_t* rounds) {
   const int n = s[0] * s[1] * s[2] / s[dim] * (s[dim] > 2 ? s[dim] - 2 : 0);
-  if (n <= CB) cut_slab_fast<T, 1>(A, B, s, dim, side, out, lds, err, rounds);
-  else if (n <= 2 * CB) cut_slab_fast<T, 2>(A, B, s, dim, side, out, lds, err, rounds);
-  else if (n <= 4 * CB) cut_slab_fast<T, 4>(A, B, s, dim, side, out, lds, err, rounds);
+  if (n <= CB) cut_slab_fast<T, 1, CB>(A, B, s, dim, side, out, lds, err, rounds);
+  else if (n <= 2 * CB) cut_slab_fast<T, 2, CB>(A, B, s, dim, side, out, lds, err, rounds);
+  else if (n <= 4 * CB) cut_slab_fast<T, 4, CB>(A, B, s, dim, side, out, lds, err, rounds);
   else if (n <= IQ_PR_NLDS) cut_slab<T>(A, B, s, dim, side, out, lds, err, rounds);
   else cut_slab<T>(A, B, s, dim, side, out, gmem, err, rounds);
 }
@@ -571,6 +571,44 @@ k_graphcut(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 
 template __global__ void k_graphcut<float>(const IqDev*, const IqQuery*);
 template __global__ void k_graphcut<double>(const IqDev*, const IqQuery*);
+
+// Plans whose largest slab has CUT_BIG_MIN..CUT_BIG_MAX interior nodes: 512
+// work-items per slab, three register-resident nodes each (node state of a
+// 1,536-node slab fills the 160 KB of LDS, so one slab per compute unit runs
+// at a time and more lanes per slab shorten each round).
+#define CB_BIG 512
+#define CUT_BIG_MAX 1536
+template <typename T>
+__global__ void __launch_bounds__(CB_BIG)
+k_graphcut_big(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  extern __shared__ char cut_lds[];
+  const IqQuery q = qs[blockIdx.y];
+  const IqStep& st = D->steps[q.step];
+  if ((int)blockIdx.x >= st.nslab) return;
+  const int64_t slot = q.slot;
+  const int64_t pk = D->pick[slot];
+  if (pk < 0) return;
+  const IqGeo& g = D->g;
+  const IqSlab sl = st.slab[blockIdx.x];
+  const int s[3] = {sl.sz[0], sl.sz[1], sl.sz[2]};
+  const int rc[3] = {(int)(pk % g.dist[0]), (int)((pk / g.dist[0]) % g.dist[1]),
+                     (int)(pk / ((int64_t)g.dist[0] * g.dist[1]))};
+  SlabView<T> A, B;
+  const IqReal& R = D->reals[q.real];
+  A.base = (const T*)D->sims + R.sim_off +
+           iq_lin3(R.pad, st.origin[0] + sl.lo[0], st.origin[1] + sl.lo[1], st.origin[2] + sl.lo[2]);
+  A.st[0] = 1; A.st[1] = R.pad[0]; A.st[2] = (int64_t)R.pad[0] * R.pad[1];
+  B.base = (const T*)D->ti + iq_lin3(g.ti, rc[0] + sl.lo[0], rc[1] + sl.lo[1], rc[2] + sl.lo[2]);
+  B.st[0] = 1; B.st[1] = g.ti[0]; B.st[2] = (int64_t)g.ti[0] * g.ti[1];
+  uint8_t* out = D->keep + (slot * IQ_MAX_SLAB + blockIdx.x) * g.tilevox;
+  const int n = s[0] * s[1] * s[2] / s[sl.dim] * (s[sl.dim] > 2 ? s[sl.dim] - 2 : 0);
+  if (n <= CB_BIG) cut_slab_fast<T, 1, CB_BIG>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, D->err, D->rounds);
+  else if (n <= 2 * CB_BIG) cut_slab_fast<T, 2, CB_BIG>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, D->err, D->rounds);
+  else cut_slab_fast<T, 3, CB_BIG>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, D->err, D->rounds);
+}
+
+template __global__ void k_graphcut_big<float>(const IqDev*, const IqQuery*);
+template __global__ void k_graphcut_big<double>(const IqDev*, const IqQuery*);
 
 // Standalone cut of one slab pair (iq_graphcut): A and B both column-major of extents s.
 template <typename T>

@@ -650,18 +650,23 @@ k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int l
 }
 
 // ---------------------------------------------------------------------------
-// Exact recheck of the superset: one wavefront per member.  The masked window
-// values and the query values are staged in LDS 512 at a time by the whole
-// wavefront; every lane then runs the same ascending fma chain (broadcast LDS
-// reads), so the FP64 value is the oracle's bit for bit.  The exact minimum
-// is reduced with an integer atomic on the non-negative double bits.
-// Grid: (groups per query, query); 4 wavefronts per group.
+// Exact recheck of the superset: the FMA chain over the masked voxels in
+// ascending column-major order, so the FP64 value is the oracle's bit for
+// bit.  Large supersets: one work-item per member, the query values and
+// window offsets staged in LDS RC_CHUNK at a time for the whole workgroup.
+// At most RC_SMALL members: one wavefront per member, window and query
+// values staged by the wavefront, every lane running the same chain (short
+// latency when the superset is small).  The exact minimum is reduced with an
+// integer atomic on the non-negative double bits.
+// Grid: (groups per query, query), 256 work-items per group.
 // ---------------------------------------------------------------------------
-#define RC_CHUNK 512
+#define RC_CHUNK 1024
+#define RC_SMALL 512
 __global__ void __launch_bounds__(256)
 k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
-  __shared__ double tl[4][RC_CHUNK];
-  __shared__ double kl[4][RC_CHUNK];
+  __shared__ double kl[RC_CHUNK];
+  __shared__ int32_t ol[RC_CHUNK];
+  __shared__ unsigned long long wbest[4];
   const IqQuery q = qs[blockIdx.y];
   const IqStep& st = D->steps[q.step];
   if (q.multi || st.mask < 0) return;
@@ -673,38 +678,87 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const double* kv = D->kv + slot * D->kv_stride;
   const double* a2map = D->a2maps[m.a2];
   const double b2 = D->b2[slot * (2 + D->naux)];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nwave = gridDim.x * 4;
   const int n = D->nsup[slot];
   double* dex = D->dexact + slot * g.P;
   unsigned long long best = 0x7ff0000000000000ull;
-  for (int e = blockIdx.x * 4 + w; e < n; e += nwave) {
-    const int64_t c = D->sup[slot * g.P + e];
-    int cx = (int)(c % g.dist[0]), cy = (int)((c / g.dist[0]) % g.dist[1]);
-    int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
-    const int64_t base = iq_lin3(g.ti, cx, cy, cz);
+  if (n <= RC_SMALL) {  // few members: a wavefront per member, window values staged for the whole wavefront
+    double* tl = (double*)&kl[(threadIdx.x >> 6) * (RC_CHUNK / 4)];
+    double* ql = tl + RC_CHUNK / 8;
+    const int lane = threadIdx.x & 63;
+    for (int e = blockIdx.x * 4 + (int)(threadIdx.x >> 6); e < n; e += gridDim.x * 4) {
+      const int64_t c = D->sup[slot * g.P + e];
+      const int cx = (int)(c % g.dist[0]), cy = (int)((c / g.dist[0]) % g.dist[1]);
+      const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
+      const int64_t base = iq_lin3(g.ti, cx, cy, cz);
+      double ab = 0.0;
+      for (int j0 = 0; j0 < V; j0 += RC_CHUNK / 8) {
+        const int nj = V - j0 < RC_CHUNK / 8 ? V - j0 : RC_CHUNK / 8;
+        for (int k = lane; k < nj; k += 64) {
+          tl[k] = D->ti_f32 ? (double)((const float*)D->ti)[base + toff[j0 + k]]
+                            : ((const double*)D->ti)[base + toff[j0 + k]];
+          ql[k] = kv[j0 + k];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int k = 0; k < nj; k++) ab = __builtin_fma(tl[k], ql[k], ab);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      const double d = __builtin_fabs((a2map[c] - 2.0 * ab) + b2);
+      if (lane == 0) dex[c] = d;
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(d);
+      best = bits < best ? bits : best;
+    }
+  }
+  for (int e0 = blockIdx.x * 256; n > RC_SMALL && e0 < n; e0 += gridDim.x * 256) {  // uniform over the group
+    const int e = e0 + (int)threadIdx.x;
+    const bool live = e < n;
+    int64_t c = 0, base = 0;
+    if (live) {
+      c = D->sup[slot * g.P + e];
+      const int cx = (int)(c % g.dist[0]), cy = (int)((c / g.dist[0]) % g.dist[1]);
+      const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
+      base = iq_lin3(g.ti, cx, cy, cz);
+    }
     double ab = 0.0;
     for (int j0 = 0; j0 < V; j0 += RC_CHUNK) {
       const int nj = V - j0 < RC_CHUNK ? V - j0 : RC_CHUNK;
-      for (int k = lane; k < nj; k += 64) {
-        tl[w][k] = D->ti_f32 ? (double)((const float*)D->ti)[base + toff[j0 + k]]
-                             : ((const double*)D->ti)[base + toff[j0 + k]];
-        kl[w][k] = kv[j0 + k];
+      __syncthreads();
+      for (int k = threadIdx.x; k < nj; k += 256) {
+        kl[k] = kv[j0 + k];
+        ol[k] = toff[j0 + k];
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      for (int k = 0; k < nj; k++) ab = __builtin_fma(tl[w][k], kl[w][k], ab);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __syncthreads();
+      if (live) {
+        if (D->ti_f32) {
+          const float* ti = (const float*)D->ti + base;
+          for (int k = 0; k < nj; k++) ab = __builtin_fma((double)ti[ol[k]], kl[k], ab);
+        } else {
+          const double* ti = (const double*)D->ti + base;
+          for (int k = 0; k < nj; k++) ab = __builtin_fma(ti[ol[k]], kl[k], ab);
+        }
+      }
     }
-    const double d = __builtin_fabs((a2map[c] - 2.0 * ab) + b2);
-    if (lane == 0) dex[c] = d;
-    unsigned long long bits = (unsigned long long)__double_as_longlong(d);
-    best = bits < best ? bits : best;
+    if (live) {
+      const double d = __builtin_fabs((a2map[c] - 2.0 * ab) + b2);
+      dex[c] = d;
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(d);
+      best = bits < best ? bits : best;
+    }
   }
-  if (lane == 0 && best != 0x7ff0000000000000ull) atomicMin(&D->gmin64[slot], best);
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long other = __shfl_xor(best, o);
+    best = other < best ? other : best;
+  }
+  if ((threadIdx.x & 63) == 0) wbest[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = wbest[0];
+    for (int i = 1; i < 4; i++) b = wbest[i] < b ? wbest[i] : b;
+    if (b != 0x7ff0000000000000ull) atomicMin(&D->gmin64[slot], b);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -198,54 +198,61 @@ k_learn_scores(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int 
   }
 }
 
-// Shortlist threshold of a query: the order key at the upper edge of the
-// 21-bit key prefix bin holding the K-th smallest approximate distance (two
-// 11/10-bit histogram passes over the P scores).  Grid: nq blocks of 1024.
+// Shortlist threshold of a query: an order key at or above its K-th
+// smallest approximate distance.  A strided sample of LKS keys (sorted in
+// LDS) brackets the K-th smallest between the sample minimum and a quantile
+// at four times K/P; one pass over the P keys histograms only that bracket
+// (2048 bins), so few work-items touch the histogram, and the threshold is
+// the upper edge of the bin where the count reaches K (the bracket's top when
+// the sample underestimated it).  Grid: nq blocks of 1024.
+#define LKS 4096
 __global__ void __launch_bounds__(1024)
 k_learn_kth(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  __shared__ uint32_t smp[LKS];
   __shared__ int hist[2048];
-  __shared__ int64_t sel[2];
   const IqQuery q = qs[blockIdx.x];
   const IqStep& st = D->steps[q.step];
   if (!learn_applies(D, q, st)) return;
   const IqGeo& g = D->g;
   const float* s = D->lrn.lsc + (int64_t)q.slot * g.P;
   const int64_t K = D->lrn.K < D->nen ? D->lrn.K : D->nen;
-  // pass 1: top 11 bits of the key
+  for (int i = threadIdx.x; i < LKS; i += blockDim.x) smp[i] = iq_f32_key(s[(int64_t)i * g.P / LKS]);
   for (int i = threadIdx.x; i < 2048; i += blockDim.x) hist[i] = 0;
   __syncthreads();
-  for (int64_t c = threadIdx.x; c < g.P; c += blockDim.x) atomicAdd(&hist[iq_f32_key(s[c]) >> 21], 1);
+  for (int k = 2; k <= LKS; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < LKS; i += blockDim.x) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint32_t a = smp[i], b = smp[ixj];
+          if ((a > b) == ((i & k) == 0)) { smp[i] = b; smp[ixj] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  const uint32_t lo = smp[0];
+  int64_t qi = (4 * K * LKS) / g.P + 1;
+  const uint32_t hi = qi >= LKS ? 0xffffffffu : smp[qi];
+  const uint32_t span = hi - lo;
+  int shift = 0;
+  while (shift < 32 && (span >> shift) >= 2048u) shift++;
+  for (int64_t c = threadIdx.x; c < g.P; c += blockDim.x) {
+    const uint32_t k = iq_f32_key(s[c]);
+    if (k <= hi) atomicAdd(&hist[k <= lo ? 0 : (k - lo) >> shift], 1);
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     int64_t run = 0;
-    int b = 0;
-    for (; b < 2047; b++) {
-      if (run + hist[b] >= K) break;
+    uint32_t thr = hi;
+    for (int b = 0; b < 2048; b++) {
       run += hist[b];
+      if (run >= K) {
+        const uint64_t top = (uint64_t)lo + (((uint64_t)b + 1) << shift) - 1;
+        thr = top < (uint64_t)hi ? (uint32_t)top : hi;
+        break;
+      }
     }
-    sel[0] = b;
-    sel[1] = run;
-  }
-  __syncthreads();
-  const uint32_t b1 = (uint32_t)sel[0];
-  const int64_t below = sel[1];
-  __syncthreads();
-  // pass 2: next 11 bits inside bin b1
-  for (int i = threadIdx.x; i < 2048; i += blockDim.x) hist[i] = 0;
-  __syncthreads();
-  for (int64_t c = threadIdx.x; c < g.P; c += blockDim.x) {
-    const uint32_t k = iq_f32_key(s[c]);
-    if ((k >> 21) == b1) atomicAdd(&hist[(k >> 10) & 0x7ffu], 1);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int64_t run = below;
-    int b = 0;
-    for (; b < 2047; b++) {
-      if (run + hist[b] >= K) break;
-      run += hist[b];
-    }
-    D->lrn.thr[q.slot] = (b1 << 21) | ((uint32_t)b << 10) | 0x3ffu;
+    D->lrn.thr[q.slot] = thr;
   }
 }
 

@@ -202,3 +202,11 @@ def test_sim_batch_soft_and_hard_per_grid():
     hard = [{(3, 4): 0.5, (30, 20): -0.3}, {(10, 50): 0.1}]
     print("batch soft + hard", flush=True)
     _compare_batch(T, (10, 10), sizes, [1, 1], soft=[(aux, aux_ti)], hard=hard, tol=0.2)
+
+
+def test_sim_large_slabs_512_lane_cut():
+    # x/y slabs of 8 x 24 x 8 voxels: 1,152 interior cut nodes (512-lane cut)
+    rng = np.random.default_rng(16)
+    T = _smooth(rng, (44, 44, 12), 5, np.float32)
+    print("large slabs", flush=True)
+    _compare(T, (24, 24, 8), (40, 40, 10), overlap=(1 / 3, 1 / 3, 1 / 3))
