@@ -112,7 +112,7 @@ def main():
                     help="also time the learned proposal (shortlist + exact rerank) on the same plan and report "
                          "it as the value, with the full scan beside it")
     ap.add_argument("--K", type=int, default=4096, help="shortlist size of the learned proposal")
-    ap.add_argument("--check-every", type=int, default=8, help="full-scan recall check every N-th batch")
+    ap.add_argument("--check-every", type=int, default=16, help="full-scan recall check every N-th batch")
     args = ap.parse_args()
     if args.copy_mb > 0:
         copy_calibration(args.copy_mb)

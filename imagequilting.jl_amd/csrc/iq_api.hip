@@ -332,6 +332,7 @@ struct iq_plan {
   double lrn_min_recall = 0.99;
   int64_t lrn_stats[4] = {0, 0, 0, 0};
   std::vector<int64_t> lrn_wofs, lrn_bofs;
+  int lrn_vmax = 0;
   float last_ms = 0.0f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // scan profiling
@@ -895,7 +896,12 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     HCL();
   }
   if (has_plain && sel == SEL_LEARNED) {
+    // shortlist -> FP32 bounds of its members -> band superset -> exact recheck
     hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q, 1);
+    HCL();
+    hipLaunchKernelGGL(k_short_bound, dim3(64, nq), dim3(256), (size_t)pl->lrn_vmax * sizeof(float), s, D, q);
+    HCL();
+    hipLaunchKernelGGL(k_short_select, dim3(16, nq), dim3(256), 0, s, D, q);
     HCL();
     hipLaunchKernelGGL(k_band_recheck, dim3(64, nq), dim3(256), 0, s, D, q);
     HCL();
@@ -1153,6 +1159,7 @@ void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
   if (nm == 0) bad(IQ_ERR_ARG, "learned proposal: the plan has no overlap mask");
   if (nsamp < IQ_EMB || iters < 0) bad(IQ_ERR_ARG, "learned proposal: need at least 64 samples");
   if (pl->enabled_host.empty()) bad(IQ_ERR_NOCAND, "every candidate pattern is disabled");
+  if (!pl->fast) bad(IQ_ERR_UNSUPPORTED, "learned proposal: training image values must be exact in FP32");
   if (!pl->lrn_ready) {
     std::vector<int64_t> wofs(nm), bofs(nm);
     size_t wtot = 0;
@@ -1169,6 +1176,13 @@ void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
     H.lrn.lsc = pl->mem.alloc<float>((size_t)pl->nslots * g.P);
     H.lrn.thr = pl->mem.alloc<uint32_t>(pl->nslots);
     H.lrn.stats = pl->mem.alloc<unsigned long long>(4);
+    H.lrn.slist = pl->mem.alloc<int32_t>((size_t)pl->nslots * g.P);
+    H.lrn.nslist = pl->mem.alloc<int32_t>(pl->nslots);
+    for (int m = 0; m < nm; m++) pl->lrn_vmax = std::max(pl->lrn_vmax, pl->masks[m].V);
+    if ((size_t)pl->lrn_vmax * sizeof(float) > 64 * 1024) {
+      HC(hipFuncSetAttribute((const void*)k_short_bound, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(pl->lrn_vmax * sizeof(float))));
+    }
     pl->lrn_wofs = wofs;
     pl->lrn_bofs = bofs;
   }

@@ -96,6 +96,8 @@ struct IqLearn {
   float* eq;                  // per slot: the query's encoding
   float* lsc;                 // per slot: P approximate distances
   uint32_t* thr;              // per slot: order key of the K-th smallest approximate distance
+  int32_t* slist;             // per slot: the shortlist (P capacity)
+  int32_t* nslist;            // per slot: its length
   unsigned long long* stats;  // recall check: band members, members in the shortlist, queries, fully recalled
 };
 

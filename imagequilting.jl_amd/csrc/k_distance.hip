@@ -631,8 +631,17 @@ k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int l
   const int64_t slot = q.slot;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool sel = false;
-  if (learned) {  // shortlist of the learned proposal (its threshold key)
+  if (learned) {  // the learned proposal's shortlist (its threshold key) into slist
     if (c < g.P) sel = !D->disabled[c] && iq_f32_key(D->lrn.lsc[slot * g.P + c]) <= D->lrn.thr[slot];
+    const unsigned long long bal = __ballot(sel);
+    const int lane = threadIdx.x & 63;
+    if (bal) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&D->lrn.nslist[slot], __popcll(bal));
+      base = __shfl(base, 0);
+      if (sel) D->lrn.slist[slot * g.P + base + __popcll(bal & ((1ull << lane) - 1ull))] = (int32_t)c;
+    }
+    return;
   } else {
     const double U = (double)__uint_as_float(D->gmin_hi[slot]);
     const float Tf = iq_f32_up((1.0 + D->tol) * U);

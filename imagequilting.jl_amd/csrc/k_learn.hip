@@ -241,7 +241,11 @@ k_learn_kth(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     if (k <= hi) atomicAdd(&hist[k <= lo ? 0 : (k - lo) >> shift], 1);
   }
   __syncthreads();
+  // fresh shortlist and band bitmap for this query
+  unsigned long long* bm = D->bitmap + (int64_t)q.slot * D->nw64;
+  for (int64_t w = threadIdx.x; w < D->nw64; w += blockDim.x) bm[w] = 0ull;
   if (threadIdx.x == 0) {
+    D->lrn.nslist[q.slot] = 0;
     int64_t run = 0;
     uint32_t thr = hi;
     for (int b = 0; b < 2048; b++) {
@@ -253,6 +257,94 @@ k_learn_kth(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
       }
     }
     D->lrn.thr[q.slot] = thr;
+  }
+}
+
+// FP32 bounds of the shortlist: one wavefront per member sums a.q over the
+// masked voxels in FP32 (lanes take strided voxels, then a shuffle tree) with
+// the bound |AB32 - AB| <= gamma_V sum|a||b| <= gamma_V sqrt(A2 B2), giving an
+// interval [lo, hi] around the exact FP64 distance as in the full scan's
+// epilogue; lo goes to lo32, the least hi to the query's reduction word.
+// Grid: (groups, query), 4 wavefronts per group.
+__global__ void __launch_bounds__(256)
+k_short_bound(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  extern __shared__ float qv[];  // the query values, V floats
+  const IqQuery q = qs[blockIdx.y];
+  const IqStep& st = D->steps[q.step];
+  if (!learn_applies(D, q, st)) return;
+  const IqGeo& g = D->g;
+  const int64_t slot = q.slot;
+  const IqMask& m = D->masks[st.mask];
+  const int V = m.V;
+  const int32_t* toff = D->toff + m.off;
+  const double* kv = D->kv + slot * D->kv_stride;
+  for (int v = threadIdx.x; v < V; v += blockDim.x) qv[v] = (float)kv[v];
+  __syncthreads();
+  const double* a2map = D->a2maps[m.a2];
+  const double b2 = D->b2[slot * (2 + D->naux)];
+  const double u32 = 0x1p-24, u64 = 0x1p-53;
+  const double gerr = 1.02 * ((double)V * u32 / (1.0 - (double)V * u32));
+  const double g64 = (double)V * u64 / (1.0 - (double)V * u64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int n = D->lrn.nslist[slot];
+  const int32_t* sl = D->lrn.slist + slot * g.P;
+  float* lo_out = D->lo32 + slot * g.P;
+  float lmin = __builtin_inff();
+  for (int e = blockIdx.x * 4 + w; e < n; e += gridDim.x * 4) {
+    const int64_t c = sl[e];
+    const int cx = (int)(c % g.dist[0]), cy = (int)((c / g.dist[0]) % g.dist[1]);
+    const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
+    const int64_t base = iq_lin3(g.ti, cx, cy, cz);
+    float acc = 0.0f;
+    for (int v = lane; v < V; v += 64) {
+      const float t = D->ti_f32 ? ((const float*)D->ti)[base + toff[v]] : (float)((const double*)D->ti)[base + toff[v]];
+      acc = __builtin_fmaf(t, qv[v], acc);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    const double a2 = a2map[c];
+    const double sq = __builtin_sqrt(a2 * b2) * (1.0 + 0x1p-30);
+    const double d = __builtin_fabs((a2 - 2.0 * (double)acc) + b2);
+    const double er = (2.0 * (gerr + g64) * sq + 4.0 * u64 * (a2 + b2 + 2.0 * sq) + 1e3 * V * 0x1p-125) * (1.0 + 0x1p-30) +
+                      4.0 * u64 * d;
+    if (lane == 0) lo_out[c] = iq_f32_down(d - er > 0.0 ? d - er : 0.0);
+    lmin = fminf(lmin, iq_f32_up(d + er));
+  }
+  if (lane == 0 && lmin < __builtin_inff()) atomicMin(&D->gmin_hi[slot], __float_as_uint(lmin));
+}
+
+// Superset of the band inside the shortlist: members with lo <= T, T =
+// round_up(fl(fl(1+tol) * U)), U the least upper bound; into the superset
+// list and the band bitmap.  Grid: (groups, query).
+__global__ void __launch_bounds__(256)
+k_short_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  const IqQuery q = qs[blockIdx.y];
+  const IqStep& st = D->steps[q.step];
+  if (!learn_applies(D, q, st)) return;
+  const IqGeo& g = D->g;
+  const int64_t slot = q.slot;
+  const double U = (double)__uint_as_float(D->gmin_hi[slot]);
+  const float Tf = iq_f32_up((1.0 + D->tol) * U);
+  const int n = D->lrn.nslist[slot];
+  const int32_t* sl = D->lrn.slist + slot * g.P;
+  const int lane = threadIdx.x & 63;
+  for (int e0 = blockIdx.x * blockDim.x; e0 < n; e0 += gridDim.x * blockDim.x) {
+    const int e = e0 + (int)threadIdx.x;
+    int64_t c = 0;
+    bool sel = false;
+    if (e < n) {
+      c = sl[e];
+      sel = D->lo32[slot * g.P + c] <= Tf;
+    }
+    const unsigned long long bal = __ballot(sel);
+    if (bal) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&D->nsup[slot], __popcll(bal));
+      base = __shfl(base, 0);
+      if (sel) {
+        D->sup[slot * g.P + base + __popcll(bal & ((1ull << lane) - 1ull))] = (int32_t)c;
+        atomicOr(&D->bitmap[slot * D->nw64 + (c >> 6)], 1ull << (c & 63));
+      }
+    }
   }
 }
 
