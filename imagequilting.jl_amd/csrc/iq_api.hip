@@ -323,6 +323,8 @@ struct iq_plan {
   IqDev* dd = nullptr;   // device descriptor
   IqQuery* dq = nullptr; // query table (all batches of a run)
   int64_t dq_cap = 0;
+  int4* dt = nullptr;    // shortlist tiles (all batches of a run)
+  int64_t dt_cap = 0;
   size_t cut_lds_max = 0;
   bool cut_big = false;    // largest slab needs the 512-lane register-resident cut
   std::vector<int32_t> enabled_host;
@@ -879,7 +881,8 @@ void launch_scan_batch(iq_plan* pl, const IqQuery* q, int nq, hipStream_t s) {
 enum { SEL_SCAN = 0, SEL_LEARNED = 1, SEL_CHECK = 2 };
 
 template <typename T>
-void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool has_plain, bool has_multi, int sel) {
+void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool has_plain, bool has_multi, int sel,
+               const int4* tiles, int ntiles) {
   const IqGeo& g = pl->geo;
   hipStream_t s = pl->ctx->stream;
   const IqDev* D = pl->dd;
@@ -887,18 +890,22 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
   const int nsrc_prep = has_multi ? 2 + pl->naux : 1;
   hipLaunchKernelGGL((k_query_prep<T>), dim3(nq, nsrc_prep), dim3(256), 0, s, D, q);
   HCL();
-  if (has_plain && sel != SEL_SCAN) {
+  if (has_plain && sel != SEL_SCAN && ntiles > 0) {
+    // shortlist: query encodings -> bracket from a sample -> MFMA scan of the
+    // bank keeping what falls in the bracket -> the K best
     hipLaunchKernelGGL(k_learn_embed, dim3(nq), dim3(256), 0, s, D, q);
     HCL();
-    hipLaunchKernelGGL(k_learn_scores, dim3(nblk(P, 256), nblk(nq, LQG)), dim3(256), 0, s, D, q, nq);
+    hipLaunchKernelGGL(k_learn_bfrag, dim3(ntiles), dim3(64), 0, s, D, q, tiles);
     HCL();
-    hipLaunchKernelGGL(k_learn_kth, dim3(nq), dim3(1024), 0, s, D, q);
+    hipLaunchKernelGGL(k_learn_bracket, dim3(nq), dim3(1024), 0, s, D, q);
+    HCL();
+    hipLaunchKernelGGL(k_learn_mscan, dim3(nblk(P, 128)), dim3(256), 0, s, D, q, tiles, ntiles);
+    HCL();
+    hipLaunchKernelGGL(k_learn_pick, dim3(nq), dim3(1024), 0, s, D, q);
     HCL();
   }
   if (has_plain && sel == SEL_LEARNED) {
     // shortlist -> FP32 bounds of its members -> band superset -> exact recheck
-    hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q, 1);
-    HCL();
     hipLaunchKernelGGL(k_short_bound, dim3(64, nq), dim3(256), (size_t)pl->lrn_vmax * sizeof(float), s, D, q);
     HCL();
     hipLaunchKernelGGL(k_short_select, dim3(16, nq), dim3(256), 0, s, D, q);
@@ -948,7 +955,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
         pl->batch_flop.push_back(f);
         pl->prof_flop += f;
       }
-      hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q, 0);
+      hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q);
       HCL();
       hipLaunchKernelGGL(k_band_recheck, dim3(64, nq), dim3(256), 0, s, D, q);
       HCL();
@@ -1002,7 +1009,8 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   pl->batch_flop.clear();
   // query table of every batch
   std::vector<IqQuery> qs;
-  struct Batch { int64_t off; int n; bool plain, multi; };
+  std::vector<int4> tiles;
+  struct Batch { int64_t off; int n; bool plain, multi; int64_t toff; int nt; };
   std::vector<Batch> batches;
   for (size_t k = 0; k < pl->nwaves; k++) {
     // lockstep: wave k of every grid, all realizations
@@ -1026,7 +1034,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
         return pl->steps[a.step].mask < pl->steps[b.step].mask;
       });
     for (size_t b0 = 0; b0 < wq.size(); b0 += (size_t)pl->nslots) {
-      Batch bt{(int64_t)qs.size(), 0, false, false};
+      Batch bt{(int64_t)qs.size(), 0, false, false, 0, 0};
       for (size_t k = b0; k < wq.size() && k < b0 + (size_t)pl->nslots; k++) {
         IqQuery q = wq[k];
         q.slot = (int32_t)(k - b0);
@@ -1034,6 +1042,21 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
         bt.multi |= q.multi != 0;
         qs.push_back(q);
         bt.n++;
+      }
+      if (pl->lrn_ready && pl->lrn_on) {
+        // tiles of up to 32 consecutive plain queries of one overlap mask
+        bt.toff = (int64_t)tiles.size();
+        for (int i = 0; i < bt.n; i++) {
+          const IqQuery& q = qs[bt.off + i];
+          const int mk = pl->steps[q.step].mask;
+          if (q.multi || mk < 0) continue;
+          if (bt.nt > 0 && tiles.back().x == mk && tiles.back().z < 32 && tiles.back().y + tiles.back().z == i) {
+            tiles.back().z++;
+          } else {
+            tiles.push_back(make_int4(mk, i, 1, 0));
+            bt.nt++;
+          }
+        }
       }
       batches.push_back(bt);
     }
@@ -1043,6 +1066,11 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
     pl->dq_cap = (int64_t)qs.size();
   }
   if (!qs.empty()) HC(hipMemcpyAsync(pl->dq, qs.data(), qs.size() * sizeof(IqQuery), hipMemcpyHostToDevice, s));
+  if ((int64_t)tiles.size() > pl->dt_cap) {
+    pl->dt = pl->mem.alloc<int4>(tiles.size());
+    pl->dt_cap = (int64_t)tiles.size();
+  }
+  if (!tiles.empty()) HC(hipMemcpyAsync(pl->dt, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s));
   HC(hipEventRecord(pl->e0, s));
   hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(64), 0, s, H.err, (int64_t)1, 0);
   hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, H.rounds, (int64_t)3, (int64_t)0);
@@ -1062,7 +1090,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   for (const Batch& b : batches) {
     int sel = SEL_SCAN;
     if (learned && b.plain) sel = (pl->lrn_check > 0 && nplain++ % pl->lrn_check == 0) ? SEL_CHECK : SEL_LEARNED;
-    run_batch<T>(pl, qs.data() + b.off, pl->dq + b.off, b.n, b.plain, b.multi, sel);
+    run_batch<T>(pl, qs.data() + b.off, pl->dq + b.off, b.n, b.plain, b.multi, sel, pl->dt + b.toff, b.nt);
     if (sel == SEL_CHECK) {
       // the full scan judged this batch: stop using the shortlist once its
       // cumulative recall on the exact bands falls below the threshold
@@ -1169,12 +1197,15 @@ void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
       bofs[m] = (int64_t)m * IQ_EMB * g.P;
     }
     H.lrn.W = pl->mem.alloc<float>(wtot);
-    H.lrn.bank = pl->mem.alloc<float>((size_t)nm * IQ_EMB * g.P);
+    H.lrn.bank = pl->mem.alloc<uint16_t>((size_t)nm * IQ_EMB * g.P);
     H.lrn.wofs = pl->mem.upload(wofs.data(), wofs.size(), s);
     H.lrn.bofs = pl->mem.upload(bofs.data(), bofs.size(), s);
     H.lrn.eq = pl->mem.alloc<float>((size_t)pl->nslots * IQ_EMB);
-    H.lrn.lsc = pl->mem.alloc<float>((size_t)pl->nslots * g.P);
-    H.lrn.thr = pl->mem.alloc<uint32_t>(pl->nslots);
+    H.lrn.bq = pl->mem.alloc<uint4>((size_t)pl->nslots * (IQ_EMB / 16) * 64);
+    H.lrn.brk = pl->mem.alloc<uint32_t>((size_t)pl->nslots * 4);
+    H.lrn.hist = pl->mem.alloc<int32_t>((size_t)pl->nslots * 2048);
+    H.lrn.lcnt = pl->mem.alloc<int32_t>(pl->nslots);
+    H.lrn.sbits = pl->mem.alloc<unsigned long long>((size_t)pl->nslots * H.nw64);
     H.lrn.stats = pl->mem.alloc<unsigned long long>(4);
     H.lrn.slist = pl->mem.alloc<int32_t>((size_t)pl->nslots * g.P);
     H.lrn.nslist = pl->mem.alloc<int32_t>(pl->nslots);
@@ -1213,7 +1244,7 @@ void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
       orthonormalize(s, it, Y, Wm, V);
     }
     hipLaunchKernelGGL(k_learn_bank, dim3(nblk(g.P, 256)), dim3(256), 0, s, (const IqDev*)pl->dd, m, (const float*)Wm,
-                       (float*)H.lrn.bank + pl->lrn_bofs[m]);
+                       (uint16_t*)H.lrn.bank + pl->lrn_bofs[m]);
     HCL();
     HC(hipStreamSynchronize(s));
   }
@@ -1609,6 +1640,12 @@ int iq_plan_set_proposal(iq_plan* plan, int enable, int64_t K, int check_every, 
     plan->lrn_check = check_every;
     plan->lrn_min_recall = min_recall;
     plan->hd.lrn.K = (int32_t)std::min<int64_t>(K, plan->geo.P);
+    // candidate lists inside the bracket: room for several times K
+    const int64_t cap = std::min<int64_t>(plan->geo.P, std::max<int64_t>(16 * (int64_t)plan->hd.lrn.K, 65536));
+    if (enable && cap > plan->hd.lrn.lcap) {
+      plan->hd.lrn.list = plan->mem.alloc<int2>((size_t)plan->nslots * cap);
+      plan->hd.lrn.lcap = cap;
+    }
     hipStream_t s = plan->ctx->stream;
     HC(hipMemcpyAsync(plan->dd, &plan->hd, sizeof(IqDev), hipMemcpyHostToDevice, s));
     HC(hipStreamSynchronize(s));

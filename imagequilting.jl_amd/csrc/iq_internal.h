@@ -91,11 +91,16 @@ struct IqLearn {
   int32_t K;                  // shortlist size
   const float* W;             // encoders: IQ_EMB x V of each mask, row-major, at wofs[mask]
   const int64_t* wofs;        // per mask (-1: no encoder)
-  const float* bank;          // P x IQ_EMB per encoded mask, at bofs[mask]
+  const uint16_t* bank;       // P x IQ_EMB bf16 rows per encoded mask, at bofs[mask]
   const int64_t* bofs;
   float* eq;                  // per slot: the query's encoding
-  float* lsc;                 // per slot: P approximate distances
-  uint32_t* thr;              // per slot: order key of the K-th smallest approximate distance
+  uint4* bq;                  // B fragments of the query tiles, [tile][IQ_EMB / 16][64]
+  uint32_t* brk;              // per slot: bracket lo, hi, bin shift
+  int32_t* hist;              // per slot: 2048-bin histogram over the bracket
+  int32_t* lcnt;              // per slot: candidates listed inside the bracket
+  int2* list;                 // per slot: lcap (candidate, order key) pairs
+  int64_t lcap;
+  unsigned long long* sbits;  // per slot: shortlist bitmap (nw64 words)
   int32_t* slist;             // per slot: the shortlist (P capacity)
   int32_t* nslist;            // per slot: its length
   unsigned long long* stats;  // recall check: band members, members in the shortlist, queries, fully recalled

@@ -617,36 +617,22 @@ IQ_SCAN_INST(4, 8, 16, 2)
 
 // ---------------------------------------------------------------------------
 // Band superset: lo <= T where T = round_up(fl(fl(1+tol) * U)) and U is the
-// least upper bound of the scan, so every exact band member is selected
-// (learned != 0: the learned proposal's shortlist instead).
+// least upper bound of the scan, so every exact band member is selected.
 // One bitmap bit per candidate (ballot words of 64 candidates) and an
 // unordered list of the members.  Grid: (candidate blocks, query).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int learned) {
+k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const IqQuery q = qs[blockIdx.y];
   const IqStep& st = D->steps[q.step];
   if (q.multi || st.mask < 0) return;
   const IqGeo& g = D->g;
   const int64_t slot = q.slot;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double U = (double)__uint_as_float(D->gmin_hi[slot]);
+  const float Tf = iq_f32_up((1.0 + D->tol) * U);
   bool sel = false;
-  if (learned) {  // the learned proposal's shortlist (its threshold key) into slist
-    if (c < g.P) sel = !D->disabled[c] && iq_f32_key(D->lrn.lsc[slot * g.P + c]) <= D->lrn.thr[slot];
-    const unsigned long long bal = __ballot(sel);
-    const int lane = threadIdx.x & 63;
-    if (bal) {
-      int base = 0;
-      if (lane == 0) base = atomicAdd(&D->lrn.nslist[slot], __popcll(bal));
-      base = __shfl(base, 0);
-      if (sel) D->lrn.slist[slot * g.P + base + __popcll(bal & ((1ull << lane) - 1ull))] = (int32_t)c;
-    }
-    return;
-  } else {
-    const double U = (double)__uint_as_float(D->gmin_hi[slot]);
-    const float Tf = iq_f32_up((1.0 + D->tol) * U);
-    if (c < g.P) sel = D->lo32[slot * g.P + c] <= Tf;
-  }
+  if (c < g.P) sel = D->lo32[slot * g.P + c] <= Tf;
   const unsigned long long bal = __ballot(sel);
   const int lane = threadIdx.x & 63;
   if (lane == 0 && (c >> 6) < D->nw64) D->bitmap[slot * D->nw64 + (c >> 6)] = bal;

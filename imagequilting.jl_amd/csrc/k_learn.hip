@@ -77,10 +77,11 @@ k_sgemm(int M, int N, int K, const float* __restrict__ A, int64_t sam, int64_t s
     }
 }
 
-// Bank of one mask: bank[j * P + c] = sum_v W[j][v] * TI[window c at voxel v]
-// (one work-item per candidate; W staged through LDS 64 voxels at a time).
+// Bank of one mask: bank[c * IQ_EMB + j] = bf16(sum_v W[j][v] * TI[window c
+// at voxel v]) (one work-item per candidate; W staged through LDS 64 voxels
+// at a time; rows of 64 bf16 are the A operand of the MFMA shortlist scan).
 __global__ void __launch_bounds__(256)
-k_learn_bank(const IqDev* __restrict__ D, int mask, const float* __restrict__ W, float* __restrict__ bank) {
+k_learn_bank(const IqDev* __restrict__ D, int mask, const float* __restrict__ W, uint16_t* __restrict__ bank) {
   __shared__ float wc[64][IQ_EMB];
   const IqGeo& g = D->g;
   const IqMask& m = D->masks[mask];
@@ -113,9 +114,18 @@ k_learn_bank(const IqDev* __restrict__ D, int mask, const float* __restrict__ W,
     }
     __syncthreads();
   }
-  if (live)
+  if (live) {
+    uint4* row = (uint4*)(bank + c * IQ_EMB);
 #pragma unroll
-    for (int j = 0; j < IQ_EMB; j++) bank[(int64_t)j * g.P + c] = acc[j];
+    for (int j = 0; j < IQ_EMB; j += 8) {
+      uint4 w;
+      w.x = iq_bf16_rne(acc[j]) | (iq_bf16_rne(acc[j + 1]) << 16);
+      w.y = iq_bf16_rne(acc[j + 2]) | (iq_bf16_rne(acc[j + 3]) << 16);
+      w.z = iq_bf16_rne(acc[j + 4]) | (iq_bf16_rne(acc[j + 5]) << 16);
+      w.w = iq_bf16_rne(acc[j + 6]) | (iq_bf16_rne(acc[j + 7]) << 16);
+      row[j / 8] = w;
+    }
+  }
 }
 
 __device__ inline bool learn_applies(const IqDev* D, const IqQuery& q, const IqStep& st) {
@@ -140,84 +150,64 @@ k_learn_embed(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   if (p == 0) D->lrn.eq[(int64_t)q.slot * IQ_EMB + j] = (part[0][j] + part[1][j]) + (part[2][j] + part[3][j]);
 }
 
-// Approximate distances of every candidate for a group of LQG queries:
-// lsc[slot][c] = A2[c] - 2 e(x_c).e(q) + B2 (+inf for disabled windows).
-// Grid: (candidate blocks of 256, query groups).
-#define LQG 16
-__global__ void __launch_bounds__(256)
-k_learn_scores(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq) {
-  __shared__ float eqs[LQG][IQ_EMB];
-  __shared__ int qm[LQG];
-  const IqGeo& g = D->g;
-  const int q0 = blockIdx.y * LQG;
-  for (int i = threadIdx.x; i < LQG * IQ_EMB; i += blockDim.x) {
-    const int k = i / IQ_EMB, j = i % IQ_EMB;
-    float v = 0.0f;
-    if (q0 + k < nq) v = D->lrn.eq[(int64_t)qs[q0 + k].slot * IQ_EMB + j];
-    eqs[k][j] = v;
-  }
-  if (threadIdx.x < LQG) {
-    int mk = -1;
-    if (q0 + (int)threadIdx.x < nq) {
-      const IqQuery q = qs[q0 + threadIdx.x];
-      const IqStep& st = D->steps[q.step];
-      if (learn_applies(D, q, st)) mk = st.mask;
+__device__ inline float bf16_val(uint32_t b) { return __uint_as_float(b << 16); }
+
+// B fragments of the query tiles (32 queries of one mask each): lane l of
+// (tile, k-step ks) holds bf16(e_q[16 ks + 8 (l / 32) + e]), e < 8, of query
+// tile.q0 + l % 32.  Grid: ntiles blocks of 64.
+__global__ void __launch_bounds__(64)
+k_learn_bfrag(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const int4* __restrict__ tiles) {
+  const int4 t = tiles[blockIdx.x];
+  const int l = threadIdx.x;
+  const float* eq = (l % 32) < t.z ? D->lrn.eq + (int64_t)qs[t.y + l % 32].slot * IQ_EMB : nullptr;
+  for (int ks = 0; ks < IQ_EMB / 16; ks++) {
+    uint4 w = {0u, 0u, 0u, 0u};
+    if (eq) {
+      const float* e = eq + 16 * ks + 8 * (l / 32);
+      w.x = iq_bf16_rne(e[0]) | (iq_bf16_rne(e[1]) << 16);
+      w.y = iq_bf16_rne(e[2]) | (iq_bf16_rne(e[3]) << 16);
+      w.z = iq_bf16_rne(e[4]) | (iq_bf16_rne(e[5]) << 16);
+      w.w = iq_bf16_rne(e[6]) | (iq_bf16_rne(e[7]) << 16);
     }
-    qm[threadIdx.x] = mk;
-  }
-  __syncthreads();
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.P) return;
-  const bool off = D->disabled[c];
-  float e[IQ_EMB];
-  int cur = -1;
-  double a2 = 0.0;
-  for (int k = 0; k < LQG && q0 + k < nq; k++) {
-    const int mk = qm[k];
-    if (mk < 0) continue;
-    const IqQuery q = qs[q0 + k];
-    float* out = D->lrn.lsc + (int64_t)q.slot * g.P;
-    if (off) { out[c] = __builtin_inff(); continue; }
-    if (mk != cur) {
-      const float* bk = D->lrn.bank + D->lrn.bofs[mk];
-#pragma unroll
-      for (int j = 0; j < IQ_EMB; j++) e[j] = bk[(int64_t)j * g.P + c];
-      a2 = D->a2maps[D->masks[mk].a2][c];
-      cur = mk;
-    }
-    float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f, d3 = 0.0f;
-#pragma unroll
-    for (int j = 0; j < IQ_EMB; j += 4) {
-      d0 = __builtin_fmaf(e[j], eqs[k][j], d0);
-      d1 = __builtin_fmaf(e[j + 1], eqs[k][j + 1], d1);
-      d2 = __builtin_fmaf(e[j + 2], eqs[k][j + 2], d2);
-      d3 = __builtin_fmaf(e[j + 3], eqs[k][j + 3], d3);
-    }
-    const double b2 = D->b2[(int64_t)q.slot * (2 + D->naux)];
-    out[c] = (float)((a2 - 2.0 * (double)((d0 + d1) + (d2 + d3))) + b2);
+    D->lrn.bq[((int64_t)blockIdx.x * (IQ_EMB / 16) + ks) * 64 + l] = w;
   }
 }
 
-// Shortlist threshold of a query: an order key at or above its K-th
-// smallest approximate distance.  A strided sample of LKS keys (sorted in
-// LDS) brackets the K-th smallest between the sample minimum and a quantile
-// at four times K/P; one pass over the P keys histograms only that bracket
-// (2048 bins), so few work-items touch the histogram, and the threshold is
-// the upper edge of the bin where the count reaches K (the bracket's top when
-// the sample underestimated it).  Grid: nq blocks of 1024.
+// Approximate distance of candidate c for a query, as the MFMA scan forms
+// it: A2[c] - 2 sum_j bf16(E[c][j]) bf16(e_q[j]) + B2.
+__device__ inline float learn_score(const IqDev* D, int mask, int64_t c, const float* eq, double b2) {
+  const uint16_t* row = D->lrn.bank + D->lrn.bofs[mask] + c * IQ_EMB;
+  float acc = 0.0f;
+  for (int j = 0; j < IQ_EMB; j++) acc = __builtin_fmaf(bf16_val(row[j]), bf16_val(iq_bf16_rne(eq[j])), acc);
+  return (float)((D->a2maps[D->masks[mask].a2][c] - 2.0 * (double)acc) + b2);
+}
+
+// Per query: a bracket [lo, hi] of order keys around the K-th smallest
+// approximate distance from a strided sample of LKS candidates (sorted in
+// LDS; hi = the sample quantile at three times K/P), the bin width of a
+// 2048-bin histogram over the bracket, and fresh per-query state
+// (histogram, candidate list, shortlist, bitmaps).  Grid: nq blocks of 1024.
 #define LKS 4096
 __global__ void __launch_bounds__(1024)
-k_learn_kth(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   __shared__ uint32_t smp[LKS];
-  __shared__ int hist[2048];
   const IqQuery q = qs[blockIdx.x];
   const IqStep& st = D->steps[q.step];
   if (!learn_applies(D, q, st)) return;
   const IqGeo& g = D->g;
-  const float* s = D->lrn.lsc + (int64_t)q.slot * g.P;
+  const int64_t slot = q.slot;
+  const float* eq = D->lrn.eq + slot * IQ_EMB;
+  const double b2 = D->b2[slot * (2 + D->naux)];
   const int64_t K = D->lrn.K < D->nen ? D->lrn.K : D->nen;
-  for (int i = threadIdx.x; i < LKS; i += blockDim.x) smp[i] = iq_f32_key(s[(int64_t)i * g.P / LKS]);
+  for (int i = threadIdx.x; i < LKS; i += blockDim.x) {
+    const int64_t c = (int64_t)i * g.P / LKS;
+    smp[i] = D->disabled[c] ? 0xffffffffu : iq_f32_key(learn_score(D, st.mask, c, eq, b2));
+  }
+  int32_t* hist = D->lrn.hist + slot * 2048;
   for (int i = threadIdx.x; i < 2048; i += blockDim.x) hist[i] = 0;
+  unsigned long long* bm = D->bitmap + slot * D->nw64;
+  unsigned long long* sb = D->lrn.sbits + slot * D->nw64;
+  for (int64_t w = threadIdx.x; w < D->nw64; w += blockDim.x) { bm[w] = 0ull; sb[w] = 0ull; }
   __syncthreads();
   for (int k = 2; k <= LKS; k <<= 1)
     for (int j = k >> 1; j > 0; j >>= 1) {
@@ -230,33 +220,135 @@ k_learn_kth(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
       }
       __syncthreads();
     }
-  const uint32_t lo = smp[0];
-  int64_t qi = (4 * K * LKS) / g.P + 1;
-  const uint32_t hi = qi >= LKS ? 0xffffffffu : smp[qi];
-  const uint32_t span = hi - lo;
-  int shift = 0;
-  while (shift < 32 && (span >> shift) >= 2048u) shift++;
-  for (int64_t c = threadIdx.x; c < g.P; c += blockDim.x) {
-    const uint32_t k = iq_f32_key(s[c]);
-    if (k <= hi) atomicAdd(&hist[k <= lo ? 0 : (k - lo) >> shift], 1);
-  }
-  __syncthreads();
-  // fresh shortlist and band bitmap for this query
-  unsigned long long* bm = D->bitmap + (int64_t)q.slot * D->nw64;
-  for (int64_t w = threadIdx.x; w < D->nw64; w += blockDim.x) bm[w] = 0ull;
   if (threadIdx.x == 0) {
-    D->lrn.nslist[q.slot] = 0;
+    const uint32_t lo = smp[0];
+    const int64_t qi = (3 * K * LKS) / g.P + 1;
+    const uint32_t hi = qi >= LKS ? 0xfffffffeu : smp[qi];
+    const uint32_t span = hi - lo;
+    uint32_t shift = 0;
+    while (shift < 32 && (span >> shift) >= 2048u) shift++;
+    uint32_t* b = D->lrn.brk + slot * 4;
+    b[0] = lo; b[1] = hi; b[2] = shift;
+    D->lrn.lcnt[slot] = 0;
+    D->lrn.nslist[slot] = 0;
+  }
+}
+
+// Shortlist scan on MFMA (v_mfma_f32_32x32x16_bf16): approximate distances
+// of 32 candidates x 32 queries of a tile per wavefront from the bf16 bank
+// rows (A) and the tile's B fragments; the few inside a query's bracket are
+// counted in its histogram and appended to its candidate list (order key
+// kept).  Each wavefront owns 32 consecutive candidates and walks all tiles
+// (tiles of one mask are adjacent, so bank rows load once per mask).
+// Grid: ceil(P / 128) blocks of 4 wavefronts.
+__global__ void __launch_bounds__(256)
+k_learn_mscan(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const int4* __restrict__ tiles, int ntiles) {
+  const IqGeo& g = D->g;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t c0 = (int64_t)blockIdx.x * 128 + w * 32;
+  const int kb = lane >> 5;
+  iq_bf16x8 a[IQ_EMB / 16];
+  double a2[16];
+  uint32_t offm = 0;  // candidate rows of this lane that are disabled or past P
+  int cur = -1;
+  const int64_t LC = D->lrn.lcap;
+  for (int t = 0; t < ntiles; t++) {
+    const int4 tl = tiles[t];
+    if (tl.x != cur) {
+      cur = tl.x;
+      const uint16_t* bank = D->lrn.bank + D->lrn.bofs[cur];
+      const int64_t ca = c0 + (lane & 31);
+#pragma unroll
+      for (int ks = 0; ks < IQ_EMB / 16; ks++) {
+        uint4 v = {0u, 0u, 0u, 0u};
+        if (ca < g.P) v = *(const uint4*)(bank + ca * IQ_EMB + 16 * ks + 8 * kb);
+        a[ks] = __builtin_bit_cast(iq_bf16x8, v);
+      }
+      const double* a2m = D->a2maps[D->masks[cur].a2];
+      offm = 0;
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const int64_t c = c0 + 8 * (v >> 2) + 4 * kb + (v & 3);
+        const bool ok = c < g.P && !D->disabled[c];
+        a2[v] = ok ? a2m[c] : 0.0;
+        offm |= ok ? 0u : (1u << v);
+      }
+    }
+    iq_f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc[v] = 0.0f;
+    const uint4* bq = D->lrn.bq + (int64_t)t * (IQ_EMB / 16) * 64 + lane;
+#pragma unroll
+    for (int ks = 0; ks < IQ_EMB / 16; ks++)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ks], __builtin_bit_cast(iq_bf16x8, bq[ks * 64]), acc, 0, 0, 0);
+    if ((lane & 31) >= tl.z) continue;
+    const int64_t slot = qs[tl.y + (lane & 31)].slot;
+    const double b2 = D->b2[slot * (2 + D->naux)];
+    const uint32_t* br = D->lrn.brk + slot * 4;
+    const uint32_t lo = br[0], hi = br[1], shift = br[2];
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      if (offm & (1u << v)) continue;
+      const uint32_t k = iq_f32_key((float)((a2[v] - 2.0 * (double)acc[v]) + b2));
+      if (k > hi) continue;
+      atomicAdd(&D->lrn.hist[slot * 2048 + (k <= lo ? 0 : (k - lo) >> shift)], 1);
+      const int idx = atomicAdd(&D->lrn.lcnt[slot], 1);
+      if (idx < LC) D->lrn.list[slot * LC + idx] = make_int2((int)(c0 + 8 * (v >> 2) + 4 * kb + (v & 3)), (int)k);
+    }
+  }
+}
+
+// Per query: the threshold key where the histogram count reaches K (the
+// bracket's top when the sample underestimated it), then the shortlist: the
+// listed candidates at or below it (list and bitmap).  Grid: nq blocks of 1024.
+__global__ void __launch_bounds__(1024)
+k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  __shared__ uint32_t thr;
+  const IqQuery q = qs[blockIdx.x];
+  const IqStep& st = D->steps[q.step];
+  if (!learn_applies(D, q, st)) return;
+  const IqGeo& g = D->g;
+  const int64_t slot = q.slot;
+  const int64_t K = D->lrn.K < D->nen ? D->lrn.K : D->nen;
+  if (threadIdx.x == 0) {
+    const uint32_t* br = D->lrn.brk + slot * 4;
+    const uint32_t lo = br[0], hi = br[1], shift = br[2];
+    const int32_t* hist = D->lrn.hist + slot * 2048;
     int64_t run = 0;
-    uint32_t thr = hi;
+    uint32_t th = hi;
     for (int b = 0; b < 2048; b++) {
       run += hist[b];
       if (run >= K) {
         const uint64_t top = (uint64_t)lo + (((uint64_t)b + 1) << shift) - 1;
-        thr = top < (uint64_t)hi ? (uint32_t)top : hi;
+        th = top < (uint64_t)hi ? (uint32_t)top : hi;
         break;
       }
     }
-    D->lrn.thr[q.slot] = thr;
+    thr = th;
+  }
+  __syncthreads();
+  const int64_t LC = D->lrn.lcap;
+  const int64_t n = D->lrn.lcnt[slot] < LC ? D->lrn.lcnt[slot] : LC;
+  const int2* list = D->lrn.list + slot * LC;
+  const int lane = threadIdx.x & 63;
+  for (int64_t e0 = 0; e0 < n; e0 += blockDim.x) {
+    const int64_t e = e0 + threadIdx.x;
+    int2 it = make_int2(0, 0);
+    bool sel = false;
+    if (e < n) {
+      it = list[e];
+      sel = (uint32_t)it.y <= thr;
+    }
+    const unsigned long long bal = __ballot(sel);
+    if (bal) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&D->lrn.nslist[slot], __popcll(bal));
+      base = __shfl(base, 0);
+      if (sel) {
+        D->lrn.slist[slot * g.P + base + __popcll(bal & ((1ull << lane) - 1ull))] = it.x;
+        atomicOr(&D->lrn.sbits[slot * D->nw64 + (it.x >> 6)], 1ull << (it.x & 63));
+      }
+    }
   }
 }
 
@@ -361,10 +453,9 @@ k_learn_recall(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const int64_t slot = q.slot;
   const int64_t nb = D->nband[slot];
   const int32_t* band = D->band + slot * g.P;
-  const float* s = D->lrn.lsc + slot * g.P;
-  const uint32_t thr = D->lrn.thr[slot];
+  const unsigned long long* sb = D->lrn.sbits + slot * D->nw64;
   int h = 0;
-  for (int64_t i = threadIdx.x; i < nb; i += blockDim.x) h += iq_f32_key(s[band[i]]) <= thr;
+  for (int64_t i = threadIdx.x; i < nb; i += blockDim.x) h += (int)((sb[band[i] >> 6] >> (band[i] & 63)) & 1ull);
   for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
   if ((threadIdx.x & 63) == 0) hits[threadIdx.x >> 6] = h;
   __syncthreads();
