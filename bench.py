@@ -155,6 +155,7 @@ def main():
     lib.iq_ctx_synchronize(ctx.handle)
     elapsed = time.perf_counter() - t0
     cut_rounds, cut_levels, ncuts = plan.cut_stats()
+    nsel, nsuper, nband = plan.select_stats()
     voxels = float(sum(np.prod(z) for z in simsizes)) * args.steps
     achieved = scan_flop / scan_sec / 1e12 if scan_sec > 0 else 0.0
     # training images whose values are exact in bf16 (facies codes, binary)
@@ -182,6 +183,8 @@ def main():
             "learned_proposal": "off",
             "mincut": {"cuts": ncuts, "push_rounds_per_cut": cut_rounds / max(ncuts, 1),
                        "relabel_levels_per_cut": cut_levels / max(ncuts, 1)},
+            "selection": {"queries": nsel, "superset_per_query": nsuper / max(nsel, 1),
+                          "band_per_query": nband / max(nsel, 1)},
         },
         "roofline": {
             "bound": "mfma",

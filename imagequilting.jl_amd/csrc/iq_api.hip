@@ -344,7 +344,7 @@ struct iq_plan {
   double prof_flop = 0.0, prof_sec = 0.0, prof_mflop = 0.0;
   int64_t prof_launches = 0;
   std::vector<double> batch_flop;
-  int64_t cut_stats[3] = {0, 0, 0};  // push rounds, relabel BFS levels, cuts of the last run
+  int64_t cut_stats[6] = {0, 0, 0, 0, 0, 0};  // push rounds, relabel BFS levels, cuts; plain queries, superset and band sizes
   // stable sorts of the multi-source maps
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
@@ -838,7 +838,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
     H.bl = pl->mem.alloc<uint4>(nb);
   }
   H.err = pl->mem.alloc<int32_t>(1);
-  H.rounds = pl->mem.alloc<int64_t>(3);  // push rounds, relabel BFS levels, cuts
+  H.rounds = pl->mem.alloc<int64_t>(6);  // push rounds, BFS levels, cuts; plain queries, superset, band members
 
   // ---- results
   const size_t esz = pl->dtype == IQ_F32 ? 4 : 8;
@@ -1073,7 +1073,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   if (!tiles.empty()) HC(hipMemcpyAsync(pl->dt, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s));
   HC(hipEventRecord(pl->e0, s));
   hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(64), 0, s, H.err, (int64_t)1, 0);
-  hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, H.rounds, (int64_t)3, (int64_t)0);
+  hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, H.rounds, (int64_t)6, (int64_t)0);
   hipLaunchKernelGGL((k_fill<int64_t>), dim3(256), dim3(256), 0, s, H.log, pl->nlog, (int64_t)-1);
   hipLaunchKernelGGL((k_fill<T>), dim3(1024), dim3(256), 0, s, (T*)H.sims, pl->nvox, (T)0);
   hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s, H.cuts, pl->nvox, (uint8_t)0);
@@ -1115,7 +1115,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   }
   int32_t code = 0;
   HC(hipMemcpy(&code, H.err, sizeof(int32_t), hipMemcpyDeviceToHost));
-  HC(hipMemcpy(pl->cut_stats, H.rounds, 3 * sizeof(int64_t), hipMemcpyDeviceToHost));
+  HC(hipMemcpy(pl->cut_stats, H.rounds, 6 * sizeof(int64_t), hipMemcpyDeviceToHost));
   if (pl->lrn_ready && pl->lrn_on) {
     unsigned long long st[4];
     HC(hipMemcpy(st, H.lrn.stats, sizeof(st), hipMemcpyDeviceToHost));
@@ -1672,6 +1672,14 @@ int iq_plan_proposal_stats(iq_plan* plan, int64_t* out) {
   if (!plan || !out) return fail(nullptr, IQ_ERR_ARG, "null argument");
   for (int k = 0; k < 4; k++) out[k] = plan->lrn_stats[k];
   out[4] = plan->lrn_fell_back ? 1 : 0;
+  return IQ_OK;
+}
+
+int iq_plan_select_stats(iq_plan* plan, int64_t* queries, int64_t* superset, int64_t* band) {
+  if (!plan) return fail(nullptr, IQ_ERR_ARG, "null plan");
+  if (queries) *queries = plan->cut_stats[3];
+  if (superset) *superset = plan->cut_stats[4];
+  if (band) *band = plan->cut_stats[5];
   return IQ_OK;
 }
 

@@ -269,7 +269,10 @@ __device__ void cut_slab(const SlabView<T>& A, const SlabView<T>& B, const int s
     uint8_t m = (S.h[u] < HMAX) ? 0 : 1;
     out[pc[0] + s[0] * (pc[1] + s[1] * pc[2])] = m ^ side;
   }
-  if (threadIdx.x == 0 && rounds) atomicAdd((unsigned long long*)rounds, (unsigned long long)total);
+  if (threadIdx.x == 0 && rounds) {
+    atomicAdd((unsigned long long*)rounds, (unsigned long long)total);
+    atomicAdd((unsigned long long*)rounds + 2, 1ull);
+  }
 }
 
 // Register-resident form of cut_slab for slabs of at most NPT * CB interior

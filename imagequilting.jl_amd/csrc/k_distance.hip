@@ -657,10 +657,12 @@ k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 // ---------------------------------------------------------------------------
 #define RC_CHUNK 1024
 #define RC_SMALL 512
+#define RC_WCH 512
 __global__ void __launch_bounds__(256)
 k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   __shared__ double kl[RC_CHUNK];
   __shared__ int32_t ol[RC_CHUNK];
+  __shared__ double wl[4 * 2 * RC_WCH];
   __shared__ unsigned long long wbest[4];
   const IqQuery q = qs[blockIdx.y];
   const IqStep& st = D->steps[q.step];
@@ -676,27 +678,57 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const int n = D->nsup[slot];
   double* dex = D->dexact + slot * g.P;
   unsigned long long best = 0x7ff0000000000000ull;
-  if (n <= RC_SMALL) {  // few members: a wavefront per member, window values staged for the whole wavefront
-    double* tl = (double*)&kl[(threadIdx.x >> 6) * (RC_CHUNK / 4)];
-    double* ql = tl + RC_CHUNK / 8;
+  if (n <= RC_SMALL) {  // few members: a wavefront per member, its chain fed from LDS
+    // the wavefront stages RC_WCH window and query values per chunk and loads
+    // the next chunk into registers while the chain runs over the current one
+    double* tl = wl + (threadIdx.x >> 6) * (2 * RC_WCH);
+    double* ql = tl + RC_WCH;
     const int lane = threadIdx.x & 63;
     for (int e = blockIdx.x * 4 + (int)(threadIdx.x >> 6); e < n; e += gridDim.x * 4) {
       const int64_t c = D->sup[slot * g.P + e];
       const int cx = (int)(c % g.dist[0]), cy = (int)((c / g.dist[0]) % g.dist[1]);
       const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
       const int64_t base = iq_lin3(g.ti, cx, cy, cz);
+      double rt[RC_WCH / 64], rq[RC_WCH / 64];
+#pragma unroll
+      for (int i = 0; i < RC_WCH / 64; i++) {
+        const int k = lane + 64 * i;
+        rt[i] = 0.0;
+        rq[i] = 0.0;
+        if (k < V) {
+          rt[i] = D->ti_f32 ? (double)((const float*)D->ti)[base + toff[k]] : ((const double*)D->ti)[base + toff[k]];
+          rq[i] = kv[k];
+        }
+      }
       double ab = 0.0;
-      for (int j0 = 0; j0 < V; j0 += RC_CHUNK / 8) {
-        const int nj = V - j0 < RC_CHUNK / 8 ? V - j0 : RC_CHUNK / 8;
-        for (int k = lane; k < nj; k += 64) {
-          tl[k] = D->ti_f32 ? (double)((const float*)D->ti)[base + toff[j0 + k]]
-                            : ((const double*)D->ti)[base + toff[j0 + k]];
-          ql[k] = kv[j0 + k];
+      for (int j0 = 0; j0 < V; j0 += RC_WCH) {
+        const int nj = V - j0 < RC_WCH ? V - j0 : RC_WCH;
+#pragma unroll
+        for (int i = 0; i < RC_WCH / 64; i++) {
+          tl[lane + 64 * i] = rt[i];
+          ql[lane + 64 * i] = rq[i];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int k = 0; k < nj; k++) ab = __builtin_fma(tl[k], ql[k], ab);
+        const int j1 = j0 + RC_WCH;
+#pragma unroll
+        for (int i = 0; i < RC_WCH / 64; i++) {  // next chunk in flight during the chain
+          const int k = j1 + lane + 64 * i;
+          if (k < V) {
+            rt[i] = D->ti_f32 ? (double)((const float*)D->ti)[base + toff[k]] : ((const double*)D->ti)[base + toff[k]];
+            rq[i] = kv[k];
+          }
+        }
+        int k = 0;
+        for (; k + 8 <= nj; k += 8) {  // eight LDS pairs in flight, then the ordered fmas
+          double t[8], u[8];
+#pragma unroll
+          for (int i = 0; i < 8; i++) { t[i] = tl[k + i]; u[i] = ql[k + i]; }
+#pragma unroll
+          for (int i = 0; i < 8; i++) ab = __builtin_fma(t[i], u[i], ab);
+        }
+        for (; k < nj; k++) ab = __builtin_fma(tl[k], ql[k], ab);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -726,13 +758,28 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
         ol[k] = toff[j0 + k];
       }
       __syncthreads();
-      if (live) {
+      if (live) {  // 16 window loads in flight, then the ordered fmas
+        int k = 0;
         if (D->ti_f32) {
           const float* ti = (const float*)D->ti + base;
-          for (int k = 0; k < nj; k++) ab = __builtin_fma((double)ti[ol[k]], kl[k], ab);
+          for (; k + 16 <= nj; k += 16) {
+            float t[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) t[i] = ti[ol[k + i]];
+#pragma unroll
+            for (int i = 0; i < 16; i++) ab = __builtin_fma((double)t[i], kl[k + i], ab);
+          }
+          for (; k < nj; k++) ab = __builtin_fma((double)ti[ol[k]], kl[k], ab);
         } else {
           const double* ti = (const double*)D->ti + base;
-          for (int k = 0; k < nj; k++) ab = __builtin_fma(ti[ol[k]], kl[k], ab);
+          for (; k + 16 <= nj; k += 16) {
+            double t[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) t[i] = ti[ol[k + i]];
+#pragma unroll
+            for (int i = 0; i < 16; i++) ab = __builtin_fma(t[i], kl[k + i], ab);
+          }
+          for (; k < nj; k++) ab = __builtin_fma(ti[ol[k]], kl[k], ab);
         }
       }
     }

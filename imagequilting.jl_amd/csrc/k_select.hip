@@ -479,6 +479,11 @@ k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   int64_t nb;
   int64_t off = blk_excl_scan(cnt, &nb, sh);
   const double dmax = blk_max_f64(lmax, sh);
+  if (threadIdx.x == 0 && D->rounds) {  // work counters: superset and band sizes
+    atomicAdd((unsigned long long*)D->rounds + 3, 1ull);
+    atomicAdd((unsigned long long*)D->rounds + 4, (unsigned long long)D->nsup[slot]);
+    atomicAdd((unsigned long long*)D->rounds + 5, (unsigned long long)nb);
+  }
   const bool small = nb <= IQ_SORT_MAX;
   for (int64_t w = w0; w < w1; w++) {
     unsigned long long b = bitmap[w];

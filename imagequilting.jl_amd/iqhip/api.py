@@ -296,6 +296,13 @@ class Plan:
                                                       t[2:].ctypes.data))
         return int(n[0]), float(t[0]), float(t[1]), float(t[2])
 
+    def select_stats(self):
+        """(plain queries, summed FP64-rechecked superset sizes, summed band sizes) of the last run."""
+        v = np.zeros(3, dtype=np.int64)
+        self.ctx.check(self.ctx._L.iq_plan_select_stats(self.handle, v[0:].ctypes.data, v[1:].ctypes.data,
+                                                        v[2:].ctypes.data))
+        return int(v[0]), int(v[1]), int(v[2])
+
     def cut_stats(self):
         """(push rounds, relabel BFS levels, cuts) of the last run."""
         v = np.zeros(3, dtype=np.int64)

@@ -194,6 +194,11 @@ int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double
  * kernel; slabs beyond its size limit are not counted).  Diagnostic only. */
 int iq_plan_cut_stats(iq_plan* plan, int64_t* rounds, int64_t* levels, int64_t* cuts);
 
+/* Selection work of the last iq_plan_run (plain queries with an overlap):
+ * number of queries, summed sizes of the band supersets that were rechecked
+ * in FP64, and summed band sizes (iqsim.jl:237).  Diagnostic only. */
+int iq_plan_select_stats(iq_plan* plan, int64_t* queries, int64_t* superset, int64_t* band);
+
 /* ---- learned proposal (no counterpart in the reference) ----------------
  * For plans with many candidate patterns: a shortlist of the K best
  * approximate distances of each plain query replaces the full scan.  The
