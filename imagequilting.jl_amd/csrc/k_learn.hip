@@ -286,14 +286,22 @@ k_learn_mscan(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const
     const double b2 = D->b2[slot * (2 + D->naux)];
     const uint32_t* br = D->lrn.brk + slot * 4;
     const uint32_t lo = br[0], hi = br[1], shift = br[2];
+    uint32_t keys[16];
+    uint32_t in = 0;
 #pragma unroll
     for (int v = 0; v < 16; v++) {
-      if (offm & (1u << v)) continue;
-      const uint32_t k = iq_f32_key((float)((a2[v] - 2.0 * (double)acc[v]) + b2));
-      if (k > hi) continue;
+      keys[v] = iq_f32_key((float)((a2[v] - 2.0 * (double)acc[v]) + b2));
+      in |= (!(offm & (1u << v)) && keys[v] <= hi) ? (1u << v) : 0u;
+    }
+    if (!in) continue;
+    int idx = atomicAdd(&D->lrn.lcnt[slot], __builtin_popcount(in));  // one reservation per lane
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      if (!(in & (1u << v))) continue;
+      const uint32_t k = keys[v];
       atomicAdd(&D->lrn.hist[slot * 2048 + (k <= lo ? 0 : (k - lo) >> shift)], 1);
-      const int idx = atomicAdd(&D->lrn.lcnt[slot], 1);
       if (idx < LC) D->lrn.list[slot * LC + idx] = make_int2((int)(c0 + 8 * (v >> 2) + 4 * kb + (v & 3)), (int)k);
+      idx++;
     }
   }
 }
