@@ -21,6 +21,10 @@
 #include "k_learn.hip"
 
 size_t iq_sort_temp_bytes(int n);
+size_t iq_segsort_temp_bytes(int n, int nseg);
+hipError_t iq_segsort_pairs(void* tmp, size_t tmp_bytes, const double* keys_in, double* keys_out,
+                            const int32_t* vals_in, int32_t* vals_out, int n, int nseg, const int64_t* begin,
+                            const int64_t* end, hipStream_t s);
 hipError_t iq_sort_pairs(void* tmp, size_t tmp_bytes, const double* keys_in, double* keys_out,
                          const int32_t* vals_in, int32_t* vals_out, int n, hipStream_t s);
 
@@ -326,6 +330,7 @@ struct iq_plan {
   int4* dt = nullptr;    // shortlist tiles (all batches of a run)
   int64_t dt_cap = 0;
   size_t cut_lds_max = 0;
+  size_t multi_lds = 0;    // LDS of the tiled multi-source maps (0: gathering kernel)
   bool cut_big = false;    // largest slab needs the 512-lane register-resident cut
   std::vector<int32_t> enabled_host;
   // learned proposal
@@ -350,6 +355,14 @@ struct iq_plan {
   size_t sort_tmp_bytes = 0;
   double* sort_keys = nullptr;
   int32_t* sort_iota = nullptr;
+  // one segmented sort per batch (all maps of all multi-source queries)
+  bool segsort = false;
+  void* seg_tmp = nullptr;
+  size_t seg_tmp_bytes = 0;
+  double* seg_keys = nullptr;
+  int32_t* seg_iota = nullptr;
+  int64_t* seg_begin = nullptr;
+  int64_t* seg_end = nullptr;
 };
 
 namespace {
@@ -775,7 +788,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   size_t per_slot = (size_t)nmap * g.tilevox * 8 + (size_t)kmax * 4 + 64 + (size_t)P * (4 + 8 + 4 + 4 + 16) + nw64 * 8 +
                     (size_t)IQ_SORT_MAX * 12 + (size_t)(nw64 + 1) * 8 + (size_t)IQ_MAX_SLAB * g.tilevox +
                     (size_t)IQ_MAX_SLAB * cutper;
-  if (pl->any_multi) per_slot += (size_t)nmap * P * 12 + (size_t)nmap * nw32 * 4 + (size_t)P * 12;
+  if (pl->any_multi) per_slot += (size_t)nmap * P * (12 + 12) + (size_t)nmap * nw32 * 4 + (size_t)P * 12;  // + segmented-sort keys and values
   int64_t widest = 1;
   for (size_t k = 0; k < pl->nwaves; k++) {
     int64_t width = 0;  // lockstep wave k over every grid and realization
@@ -819,6 +832,15 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   H.nmap = nmap;
   H.nw32 = nw32;
   if (pl->any_multi) {
+    // tiled exact maps when the candidate block's training-image region, the
+    // query values and their region offsets fit in LDS
+    const size_t nreg = (size_t)(MT_X + g.tile[0] - 1) * (MT_Y + g.tile[1] - 1) * g.tile[2];
+    const size_t need = nreg * 8 + (size_t)g.tilevox * 12;
+    if (need <= 150 * 1024) {
+      pl->multi_lds = need;
+      if (need > 64 * 1024)
+        HC(hipFuncSetAttribute((const void*)k_multi_maps_tiled, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    }
     H.maps = pl->mem.alloc<double>((size_t)NS * nmap * P);
     H.sorted = pl->mem.alloc<int32_t>((size_t)NS * nmap * P);
     H.bits = pl->mem.alloc<uint32_t>((size_t)NS * nmap * nw32);
@@ -830,6 +852,18 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
     HCL();
     pl->sort_tmp_bytes = iq_sort_temp_bytes((int)P);
     pl->sort_tmp = pl->mem.alloc<char>(pl->sort_tmp_bytes);
+    const int64_t nall = NS * nmap * P;
+    if (nall < (1ll << 31)) {
+      pl->segsort = true;
+      pl->seg_keys = pl->mem.alloc<double>((size_t)nall);
+      pl->seg_iota = pl->mem.alloc<int32_t>((size_t)nall);
+      hipLaunchKernelGGL(k_iota_mod, dim3(nblk(nall, 256)), dim3(256), 0, s, pl->seg_iota, nall, P);
+      HCL();
+      pl->seg_begin = pl->mem.alloc<int64_t>((size_t)NS * nmap);
+      pl->seg_end = pl->mem.alloc<int64_t>((size_t)NS * nmap);
+      pl->seg_tmp_bytes = iq_segsort_temp_bytes((int)nall, (int)(NS * nmap));
+      pl->seg_tmp = pl->mem.alloc<char>(pl->seg_tmp_bytes);
+    }
   }
   if (pl->mfma) {
     pl->mfma_ntile_max = (NS + 31) / 32;
@@ -971,10 +1005,23 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     }
   }
   if (has_multi) {
-    hipLaunchKernelGGL(k_multi_maps, dim3(nblk(P, 256), nq, 2 + pl->naux), dim3(256), 0, s, D, q);
+    if (pl->multi_lds > 0) {
+      const unsigned nb = nblk(g.dist[0], MT_X) * nblk(g.dist[1], MT_Y) * (unsigned)g.dist[2];
+      hipLaunchKernelGGL(k_multi_maps_tiled, dim3(nb, nq, 2 + pl->naux), dim3(256), pl->multi_lds, s, D, q);
+    } else {
+      hipLaunchKernelGGL(k_multi_maps, dim3(nblk(P, 256), nq, 2 + pl->naux), dim3(256), 0, s, D, q);
+    }
     HCL();
     const IqDev& H = pl->hd;
-    for (int i = 0; i < nq; i++) {
+    if (pl->segsort) {
+      // segments (query i, map k) = [(slot nmap + k) P, + P), empty when unused
+      hipLaunchKernelGGL(k_seg_offsets, dim3(nblk((int64_t)nq * H.nmap, 256)), dim3(256), 0, s, D, q, nq, pl->seg_begin,
+                         pl->seg_end);
+      HCL();
+      HC(iq_segsort_pairs(pl->seg_tmp, pl->seg_tmp_bytes, H.maps, pl->seg_keys, pl->seg_iota, H.sorted,
+                          (int)((int64_t)pl->nslots * H.nmap * P), nq * H.nmap, pl->seg_begin, pl->seg_end, s));
+    }
+    for (int i = 0; i < nq && !pl->segsort; i++) {
       if (!hq[i].multi) continue;
       const IqStep& st = pl->steps[hq[i].step];
       for (int k = 0; k < H.nmap; k++) {

@@ -863,6 +863,104 @@ k_multi_maps(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   out[c] = __builtin_fabs((a2 - 2.0 * ab) + b2);
 }
 
+// Tiled form of k_multi_maps: a workgroup owns MT_X x MT_Y candidates at one
+// z; the training-image (or auxiliary-TI) region their windows cover is
+// staged in LDS as FP64, with the query values and the region offsets of the
+// source's voxels, so every work-item's chain (same ascending order, same
+// operations as k_multi_maps) reads LDS instead of gathering from HBM.
+// Grid: (x blocks * y blocks * dist z, query, source); LDS = region + 12 B per
+// source voxel (the host checks it fits).
+#define MT_X 16
+#define MT_Y 16
+__global__ void __launch_bounds__(256)
+k_multi_maps_tiled(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  extern __shared__ double mreg[];
+  const IqQuery q = qs[blockIdx.y];
+  if (!q.multi) return;
+  const int src = blockIdx.z;
+  const IqStep& st = D->steps[q.step];
+  if (src == 1 && !st.hard) return;
+  const IqGeo& g = D->g;
+  const int nbx = (g.dist[0] + MT_X - 1) / MT_X, nby = (g.dist[1] + MT_Y - 1) / MT_Y;
+  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, cz = blockIdx.x / (nbx * nby);
+  const int64_t slot = q.slot;
+  double* out = D->maps + (slot * D->nmap + src) * g.P;
+  int V;
+  const int32_t* tv;
+  const double* a2map = nullptr;
+  const double* img64 = nullptr;
+  if (src == 0) {
+    if (st.mask < 0) {
+      const int cx = bx * MT_X + (threadIdx.x % MT_X), cy = by * MT_Y + (threadIdx.x / MT_X);
+      if (cx < g.dist[0] && cy < g.dist[1]) {
+        const int64_t c = iq_lin3(g.dist, cx, cy, cz);
+        out[c] = D->disabled[c] ? __builtin_inf() : 0.0;
+      }
+      return;
+    }
+    const IqMask& m = D->masks[st.mask];
+    V = m.V; tv = D->tvox + m.off; a2map = D->a2maps[m.a2];
+  } else if (src == 1) {
+    V = st.hcnt; tv = D->htvox + st.hoff;
+  } else {
+    V = g.tilevox; tv = D->tvox + D->full_off; a2map = D->a2soft[src - 2]; img64 = D->auxti[src - 2];
+  }
+  const int RX = MT_X + g.tile[0] - 1, RY = MT_Y + g.tile[1] - 1, RZ = g.tile[2];
+  const int nreg = RX * RY * RZ;
+  double* reg = mreg;
+  double* kq = mreg + nreg;
+  int32_t* ro = (int32_t*)(kq + V);
+  const int gx0 = bx * MT_X, gy0 = by * MT_Y;
+  for (int i = threadIdx.x; i < nreg; i += blockDim.x) {
+    const int x = i % RX, y = (i / RX) % RY, z = i / (RX * RY);
+    const int gx = gx0 + x, gy = gy0 + y, gz = cz + z;
+    double v = 0.0;
+    if (gx < g.ti[0] && gy < g.ti[1]) {
+      const int64_t o = iq_lin3(g.ti, gx, gy, gz);
+      v = img64 ? img64[o] : (D->ti_f32 ? (double)((const float*)D->ti)[o] : ((const double*)D->ti)[o]);
+    }
+    reg[i] = v;
+  }
+  const double* kv = D->kv + slot * D->kv_stride + (int64_t)src * g.tilevox;
+  for (int j = threadIdx.x; j < V; j += blockDim.x) {
+    const int t = tv[j];
+    const int x = t % g.tile[0], y = (t / g.tile[0]) % g.tile[1], z = t / (g.tile[0] * g.tile[1]);
+    kq[j] = kv[j];
+    ro[j] = x + RX * (y + RY * z);
+  }
+  __syncthreads();
+  const int lx = threadIdx.x % MT_X, ly = threadIdx.x / MT_X;
+  const int cx = gx0 + lx, cy = gy0 + ly;
+  if (cx >= g.dist[0] || cy >= g.dist[1]) return;
+  const int64_t c = iq_lin3(g.dist, cx, cy, cz);
+  if (D->disabled[c]) { out[c] = __builtin_inf(); return; }
+  const double b2 = D->b2[slot * (2 + D->naux) + src];
+  const double* r0 = reg + lx + RX * ly;
+  double a2 = 0.0, ab = 0.0;
+  int j = 0;
+  if (!a2map) {
+    for (; j + 8 <= V; j += 8) {
+      double t[8], u[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) { t[i] = r0[ro[j + i]]; u[i] = kq[j + i]; }
+#pragma unroll
+      for (int i = 0; i < 8; i++) { a2 = __builtin_fma(t[i], t[i], a2); ab = __builtin_fma(t[i], u[i], ab); }
+    }
+    for (; j < V; j++) { const double t = r0[ro[j]]; a2 = __builtin_fma(t, t, a2); ab = __builtin_fma(t, kq[j], ab); }
+  } else {
+    for (; j + 8 <= V; j += 8) {
+      double t[8], u[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) { t[i] = r0[ro[j + i]]; u[i] = kq[j + i]; }
+#pragma unroll
+      for (int i = 0; i < 8; i++) ab = __builtin_fma(t[i], u[i], ab);
+    }
+    for (; j < V; j++) ab = __builtin_fma(r0[ro[j]], kq[j], ab);
+    a2 = a2map[c];
+  }
+  out[c] = __builtin_fabs((a2 - 2.0 * ab) + b2);
+}
+
 // Plain queries whose training image is not exactly representable in FP32:
 // the overlap map is computed exactly for every candidate, all finite
 // candidates form the superset.
