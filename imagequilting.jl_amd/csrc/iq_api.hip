@@ -964,9 +964,11 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
         dim3 grid(nblk(g.dist[0], 32), nblk(g.dist[1], 16), (unsigned)(g.dist[2] * ntile));
         if (pl->prof) {
           HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));
-          // one 32-candidate row block per candidate row inside the grid, per
-          // x block and z x query tile: k-steps x (3 or 1) 32x32x16 MFMAs
-          pl->prof_mflop += (double)grid.x * g.dist[1] * grid.z * pl->hd.ktab.nstep * (pl->mfma_split ? 3.0 : 1.0) *
+          // four 32-candidate row blocks per wavefront with a row inside the
+          // grid, per x block and z x query tile: k-steps x (3 or 1) MFMAs
+          double rows = 0.0;
+          for (int y0 = 0; y0 < g.dist[1]; y0 += 16) rows += 4.0 * ((std::min(16, g.dist[1] - y0) + 3) / 4);
+          pl->prof_mflop += (double)grid.x * rows * grid.z * pl->hd.ktab.nstep * (pl->mfma_split ? 3.0 : 1.0) *
                             (2.0 * 32 * 32 * 16);
         }
         if (pl->mfma_split)

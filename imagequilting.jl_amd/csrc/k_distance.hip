@@ -485,6 +485,7 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
   const int tile = blockIdx.z / g.dist[2];
   const int cz = blockIdx.z % g.dist[2];
   const int cx0 = blockIdx.x * 32, cy0 = blockIdx.y * 16;
+  const bool wlive = cy0 + 4 * w < g.dist[1];  // uniform per wavefront
   const int nstep = D->ktab.nstep;
   const float* __restrict__ ti = D->ti32;
   iq_f32x16 acc[4];
@@ -528,27 +529,38 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
       }
       const int es = (ko >> 30) ? RX : 1;
       const int base = (ko & 0x3fffffff) + 4 * w * RX + row;
+      if (!wlive) continue;  // every candidate row of this wavefront is past the grid
+      // all 32 LDS words of the four row blocks first, then the MFMAs in an
+      // order where consecutive ones use different accumulators
+      uint32_t wv[4][8];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) wv[i][e] = lw[base + i * RX + e * es];
+      iq_bf16x8 ah[4], al[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        if (cy0 + 4 * w + i >= g.dist[1]) break;  // candidate rows past the grid (uniform per wavefront)
-        const int a0 = base + i * RX;
-        uint32_t wv[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) wv[e] = lw[a0 + e * es];
-        uint4 ah, al;
-        ah.x = __builtin_amdgcn_perm(wv[1], wv[0], 0x05040100u);
-        ah.y = __builtin_amdgcn_perm(wv[3], wv[2], 0x05040100u);
-        ah.z = __builtin_amdgcn_perm(wv[5], wv[4], 0x05040100u);
-        ah.w = __builtin_amdgcn_perm(wv[7], wv[6], 0x05040100u);
-        acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(iq_bf16x8, ah), bh, acc[i], 0, 0, 0);
+        uint4 h4, l4;
+        h4.x = __builtin_amdgcn_perm(wv[i][1], wv[i][0], 0x05040100u);
+        h4.y = __builtin_amdgcn_perm(wv[i][3], wv[i][2], 0x05040100u);
+        h4.z = __builtin_amdgcn_perm(wv[i][5], wv[i][4], 0x05040100u);
+        h4.w = __builtin_amdgcn_perm(wv[i][7], wv[i][6], 0x05040100u);
+        ah[i] = __builtin_bit_cast(iq_bf16x8, h4);
         if (SPLIT) {
-          al.x = __builtin_amdgcn_perm(wv[1], wv[0], 0x07060302u);
-          al.y = __builtin_amdgcn_perm(wv[3], wv[2], 0x07060302u);
-          al.z = __builtin_amdgcn_perm(wv[5], wv[4], 0x07060302u);
-          al.w = __builtin_amdgcn_perm(wv[7], wv[6], 0x07060302u);
-          acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(iq_bf16x8, ah), bl, acc[i], 0, 0, 0);
-          acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(iq_bf16x8, al), bh, acc[i], 0, 0, 0);
+          l4.x = __builtin_amdgcn_perm(wv[i][1], wv[i][0], 0x07060302u);
+          l4.y = __builtin_amdgcn_perm(wv[i][3], wv[i][2], 0x07060302u);
+          l4.z = __builtin_amdgcn_perm(wv[i][5], wv[i][4], 0x07060302u);
+          l4.w = __builtin_amdgcn_perm(wv[i][7], wv[i][6], 0x07060302u);
+          al[i] = __builtin_bit_cast(iq_bf16x8, l4);
         }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh, acc[i], 0, 0, 0);
+      if (SPLIT) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl, acc[i], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh, acc[i], 0, 0, 0);
       }
     }
     __syncthreads();
