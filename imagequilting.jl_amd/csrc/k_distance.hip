@@ -498,12 +498,19 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
     const int RX = 32 + kbx.wxr - 1, RY = 16 + kbx.wyr - 1;
     const int n = RX * RY * kbx.wz;
     const int gx0 = cx0 + kbx.x0, gy0 = cy0 + kbx.y0, gz0 = cz + kbx.z0;
+    // element i = x + RX (y + RY z); coordinates advanced incrementally (256 per pass)
+    int x = (int)threadIdx.x % RX, y = ((int)threadIdx.x / RX) % RY, z = (int)threadIdx.x / (RX * RY);
+    const int sx = 256 % RX, sy = (256 / RX) % RY, sz = 256 / (RX * RY);
     for (int i = threadIdx.x; i < n; i += 256) {
-      int x = i % RX, y = (i / RX) % RY, z = i / (RX * RY);
-      int gx = gx0 + x, gy = gy0 + y, gz = gz0 + z;
+      const int gx = gx0 + x, gy = gy0 + y, gz = gz0 + z;
       float v = 0.0f;
       if (gx < g.ti[0] && gy < g.ti[1] && gz < g.ti[2]) v = ti[iq_lin3(g.ti, gx, gy, gz)];
       lw[i] = iq_split_word(v);
+      x += sx;
+      y += sy;
+      z += sz;
+      if (x >= RX) { x -= RX; y++; }
+      if (y >= RY) { y -= RY; z++; }
     }
     __syncthreads();
     // B fragments and chunk offsets are loaded one k-step ahead
@@ -576,12 +583,22 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
     live = !q.multi && st.mask >= 0;
     slot = q.slot;
     if (live) {
+      // Interval around the oracle's FP64 distance, evaluated in FP32:
+      // |AB32 - AB| <= gerr sum|a||b| and sum|a||b| <= S = sqrt(A2) sqrt(B2)
+      // (rounded up by 2^-19); 8 u32 (A2 + B2 + 2S) covers the FP32
+      // conversions and the evaluation of d as well as the oracle's own FP64
+      // rounding.  The main term carries a 2^-10 margin for its own FP32
+      // evaluation, the small terms are doubled to cover the rounding of
+      // d -/+ e, and lo / hi are pushed outward by 2^-20.
       const IqMask& m = D->masks[st.mask];
       const double* a2map = D->a2maps[m.a2];
-      const double b2 = D->b2[slot * (2 + D->naux)];
+      const float B = (float)D->b2[slot * (2 + D->naux)];
+      const float sB = __builtin_sqrtf(B) * (1.0f + 0x1p-19f);
       const double V = (double)m.V;
-      const double u64 = 0x1p-53;
-      const double g64 = V * u64 / (1.0 - V * u64);
+      const double g64 = V * 0x1p-53 / (1.0 - V * 0x1p-53);
+      const float gs = (float)((gerr + g64) * (1.0 + 0x1p-10));
+      const float u32 = 0x1p-24f;
+      const float tiny = (float)(1e3 * V * 0x1p-125);
       float* lo_out = D->lo32 + slot * g.P;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
@@ -593,13 +610,12 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
             const int64_t c = iq_lin3(g.dist, cx, cy, cz);
             float lo = __builtin_inff(), hi = __builtin_inff();
             if (!D->disabled[c]) {
-              const double a2 = a2map[c];
-              const double sq = __builtin_sqrt(a2 * b2) * (1.0 + 0x1p-30);
-              const double d = __builtin_fabs((a2 - 2.0 * (double)acc[i][v]) + b2);
-              const double e = (2.0 * (gerr + g64) * sq + 4.0 * u64 * (a2 + b2 + 2.0 * sq) + 1e3 * V * 0x1p-125) *
-                               (1.0 + 0x1p-30) + 4.0 * u64 * d;
-              lo = iq_f32_down(d - e > 0.0 ? d - e : 0.0);
-              hi = iq_f32_up(d + e);
+              const float A = (float)a2map[c];
+              const float d = __builtin_fabsf((A - 2.0f * acc[i][v]) + B);
+              const float S = __builtin_sqrtf(A) * sB;
+              const float e = (2.0f * gs * S) * (1.0f + 0x1p-10f) + 2.0f * (8.0f * u32 * ((A + B) + 2.0f * S) + tiny);
+              lo = d - e > 0.0f ? (d - e) * (1.0f - 0x1p-20f) : 0.0f;
+              hi = (d + e) * (1.0f + 0x1p-20f);
             }
             lo_out[c] = lo;
             lmin = fminf(lmin, hi);
