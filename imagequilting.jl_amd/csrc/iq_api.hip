@@ -625,6 +625,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
 
   // ---- exact A2 maps of the overlap masks (A2 chains over the TI)
   std::vector<const double*> a2ptr;
+  std::vector<const float*> a2fptr;
   for (size_t mi = 0; mi < pl->masks.size(); mi++) {
     IqMask& m = pl->masks[mi];
     double* a2 = pl->mem.alloc<double>(g.P);
@@ -635,11 +636,17 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
       hipLaunchKernelGGL((k_exact_map<double, 2>), dim3(nblk(g.P, 256)), dim3(256), 0, s, (const double*)H.ti, g,
                          H.toff + m.off, (const double*)nullptr, m.V, (const double*)nullptr, a2);
     HCL();
+    float* a2f = pl->mem.alloc<float>(g.P);
+    hipLaunchKernelGGL(k_a2_f32, dim3(std::min<int64_t>(nblk(g.P, 256), 4096)), dim3(256), 0, s, (const double*)a2,
+                       (const uint8_t*)H.disabled, g.P, a2f);
+    HCL();
     m.a2 = (int32_t)a2ptr.size();
     a2ptr.push_back(a2);
+    a2fptr.push_back(a2f);
   }
   H.masks = pl->mem.upload(pl->masks.data(), pl->masks.size(), s);
   H.a2maps = (const double* const*)pl->mem.upload(a2ptr.data(), a2ptr.size(), s);
+  H.a2f = (const float* const*)pl->mem.upload(a2fptr.data(), a2fptr.size(), s);
   // one scan configuration for the plan: every mask must fit its LDS, the
   // most used mask decides the tiling; 3-D grids prefer the 2-D register block
   if (pl->fast && !hmasks.empty()) {

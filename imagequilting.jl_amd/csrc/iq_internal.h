@@ -146,6 +146,7 @@ struct IqDev {
   const int32_t* htoff;
   int32_t full_off;
   const double* const* a2maps;
+  const float* const* a2f;  // FP32 copies of a2maps, NaN where the window is disabled
   const IqReal* reals;
   const double* const* auxti;
   const double* const* a2soft;
@@ -195,6 +196,17 @@ __host__ __device__ inline uint32_t iq_f32_key(float x) {
   union { float f; uint32_t u; } c;
   c.f = x == 0.0f ? 0.0f : x;
   return (c.u >> 31) ? ~c.u : (c.u | 0x80000000u);
+}
+
+// global-address-space view of a device pointer: loads through it are
+// global_load (counted by vmcnt only), not flat loads that also hold lgkmcnt
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T* iq_glob(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T* iq_globw(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
 }
 
 __host__ __device__ inline int64_t iq_lin3(const int32_t d[3], int64_t x, int64_t y, int64_t z) {

@@ -427,6 +427,7 @@ template __global__ void k_scan2_f32<4, 4, 16, 4>(const IqDev*, const IqQuery*, 
 // ---------------------------------------------------------------------------
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 iq_bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float iq_f32x16;
+typedef unsigned int iq_u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline uint32_t iq_bf16_rne(float f) {  // round-to-nearest-even bf16 bits of a finite float
   uint32_t u = __float_as_uint(f);
@@ -474,8 +475,16 @@ k_mfma_bprep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq
   D->bl[o] = make_uint4(l[0], l[1], l[2], l[3]);
 }
 
-template <bool SPLIT>
+// FP32 copy of an A2 map for the scan's interval bound; NaN marks a disabled
+// window (an A2 that overflows FP32 becomes +inf and stays a candidate)
 __global__ void __launch_bounds__(256)
+k_a2_f32(const double* __restrict__ a2, const uint8_t* __restrict__ disabled, int64_t P, float* __restrict__ out) {
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < P; c += (int64_t)gridDim.x * 256)
+    out[c] = disabled[c] ? __builtin_nanf("") : (float)a2[c];
+}
+
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq, double gerr) {
   extern __shared__ uint32_t lw[];
   __shared__ float red[4][32];
@@ -487,7 +496,10 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
   const int cx0 = blockIdx.x * 32, cy0 = blockIdx.y * 16;
   const bool wlive = cy0 + 4 * w < g.dist[1];  // uniform per wavefront
   const int nstep = D->ktab.nstep;
-  const float* __restrict__ ti = D->ti32;
+  const int t0 = g.ti[0], t1 = g.ti[1], t2 = g.ti[2];
+  const int d0 = g.dist[0], d1 = g.dist[1];
+  const auto ti = iq_glob(D->ti32);
+  constexpr int SU = 8;
   iq_f32x16 acc[4];
 #pragma unroll
   for (int i = 0; i < 4; i++)
@@ -498,26 +510,35 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
     const int RX = 32 + kbx.wxr - 1, RY = 16 + kbx.wyr - 1;
     const int n = RX * RY * kbx.wz;
     const int gx0 = cx0 + kbx.x0, gy0 = cy0 + kbx.y0, gz0 = cz + kbx.z0;
-    // element i = x + RX (y + RY z); coordinates advanced incrementally (256 per pass)
+    // element i = x + RX (y + RY z); coordinates advanced incrementally (256
+    // per element, SU elements per pass with all their loads issued first);
+    // windows past the training image read as zero
     int x = (int)threadIdx.x % RX, y = ((int)threadIdx.x / RX) % RY, z = (int)threadIdx.x / (RX * RY);
     const int sx = 256 % RX, sy = (256 / RX) % RY, sz = 256 / (RX * RY);
-    for (int i = threadIdx.x; i < n; i += 256) {
-      const int gx = gx0 + x, gy = gy0 + y, gz = gz0 + z;
-      float v = 0.0f;
-      if (gx < g.ti[0] && gy < g.ti[1] && gz < g.ti[2]) v = ti[iq_lin3(g.ti, gx, gy, gz)];
-      lw[i] = iq_split_word(v);
-      x += sx;
-      y += sy;
-      z += sz;
-      if (x >= RX) { x -= RX; y++; }
-      if (y >= RY) { y -= RY; z++; }
+    for (int i0 = threadIdx.x; i0 < n; i0 += 256 * SU) {
+      float v[SU];
+      bool in[SU];
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        const int gx = gx0 + x, gy = gy0 + y, gz = gz0 + z;
+        in[u] = gx < t0 && gy < t1 && gz < t2;
+        v[u] = ti[in[u] ? gx + (int64_t)t0 * (gy + (int64_t)t1 * gz) : 0];
+        x += sx;
+        y += sy;
+        z += sz;
+        if (x >= RX) { x -= RX; y++; }
+        if (y >= RY) { y -= RY; z++; }
+      }
+#pragma unroll
+      for (int u = 0; u < SU; u++)
+        if (i0 + 256 * u < n) lw[i0 + 256 * u] = iq_split_word(in[u] ? v[u] : 0.0f);
     }
     __syncthreads();
     // B fragments and chunk offsets are loaded one k-step ahead
-    const uint4* __restrict__ bhp = D->bh + (int64_t)tile * nstep * 64 + lane;
-    const uint4* __restrict__ blp = D->bl + (int64_t)tile * nstep * 64 + lane;
-    const int32_t* __restrict__ kofs = D->kofs;
-    uint4 bh_n = {0u, 0u, 0u, 0u}, bl_n = {0u, 0u, 0u, 0u};
+    const auto bhp = iq_glob((const iq_u32x4*)(D->bh + (int64_t)tile * nstep * 64 + lane));
+    const auto blp = iq_glob((const iq_u32x4*)(D->bl + (int64_t)tile * nstep * 64 + lane));
+    const auto kofs = iq_glob(D->kofs);
+    iq_u32x4 bh_n = {0u, 0u, 0u, 0u}, bl_n = {0u, 0u, 0u, 0u};
     int ko_n = 0;
     if (kbx.s0 < kbx.s1) {
       bh_n = bhp[(int64_t)kbx.s0 * 64];
@@ -585,39 +606,46 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
     if (live) {
       // Interval around the oracle's FP64 distance, evaluated in FP32:
       // |AB32 - AB| <= gerr sum|a||b| and sum|a||b| <= S = sqrt(A2) sqrt(B2)
-      // (rounded up by 2^-19); 8 u32 (A2 + B2 + 2S) covers the FP32
-      // conversions and the evaluation of d as well as the oracle's own FP64
-      // rounding.  The main term carries a 2^-10 margin for its own FP32
-      // evaluation, the small terms are doubled to cover the rounding of
-      // d -/+ e, and lo / hi are pushed outward by 2^-20.
+      // (square roots of operands clamped up to 2^-100, rounded up by
+      // 2^-19); 8 u32 (A2 + B2 + 2S) covers the FP32 conversions and the
+      // evaluation of d as well as the oracle's own FP64 rounding.  The main
+      // term carries a 2^-10 margin for its own FP32 evaluation, the small
+      // terms are doubled to cover the rounding of d -/+ e, and lo / hi are
+      // pushed outward by 2^-20.  A2 = NaN marks a disabled window.
       const IqMask& m = D->masks[st.mask];
-      const double* a2map = D->a2maps[m.a2];
+      const auto a2f = iq_glob(D->a2f[m.a2]);
       const float B = (float)D->b2[slot * (2 + D->naux)];
-      const float sB = __builtin_sqrtf(B) * (1.0f + 0x1p-19f);
+      const float sB = __builtin_amdgcn_sqrtf(fmaxf(B, 0x1p-100f)) * (1.0f + 0x1p-19f);
       const double V = (double)m.V;
       const double g64 = V * 0x1p-53 / (1.0 - V * 0x1p-53);
-      const float gs = (float)((gerr + g64) * (1.0 + 0x1p-10));
+      const float gs2 = (float)((gerr + g64) * (1.0 + 0x1p-10)) * 2.0f * (1.0f + 0x1p-10f);
       const float u32 = 0x1p-24f;
-      const float tiny = (float)(1e3 * V * 0x1p-125);
-      float* lo_out = D->lo32 + slot * g.P;
+      const float tiny2 = 2.0f * (float)(1e3 * V * 0x1p-125);
+      const auto lo_out = iq_globw(D->lo32 + slot * g.P);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int cy = cy0 + 4 * w + i;
+        if (cy >= d1) break;  // uniform per wavefront
+        const int64_t rb = (int64_t)d0 * (cy + (int64_t)d1 * cz) + cx0 + 4 * kb;
+        const int xl = d0 - cx0 - 4 * kb;  // candidates of this row left to the lane
+        float A[16];
 #pragma unroll
         for (int v = 0; v < 16; v++) {
-          const int cx = cx0 + 8 * (v >> 2) + 4 * kb + (v & 3);
-          if (cx < g.dist[0] && cy < g.dist[1]) {
-            const int64_t c = iq_lin3(g.dist, cx, cy, cz);
-            float lo = __builtin_inff(), hi = __builtin_inff();
-            if (!D->disabled[c]) {
-              const float A = (float)a2map[c];
-              const float d = __builtin_fabsf((A - 2.0f * acc[i][v]) + B);
-              const float S = __builtin_sqrtf(A) * sB;
-              const float e = (2.0f * gs * S) * (1.0f + 0x1p-10f) + 2.0f * (8.0f * u32 * ((A + B) + 2.0f * S) + tiny);
-              lo = d - e > 0.0f ? (d - e) * (1.0f - 0x1p-20f) : 0.0f;
-              hi = (d + e) * (1.0f + 0x1p-20f);
-            }
-            lo_out[c] = lo;
+          const int ox = 8 * (v >> 2) + (v & 3);
+          A[v] = a2f[ox < xl ? rb + ox : rb - 4 * kb];
+        }
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+          const int ox = 8 * (v >> 2) + (v & 3);
+          const float a = A[v];
+          const float d = __builtin_fabsf((a - 2.0f * acc[i][v]) + B);
+          const float S = __builtin_amdgcn_sqrtf(fmaxf(a, 0x1p-100f)) * sB;
+          const float e = gs2 * S + 2.0f * (8.0f * u32 * ((a + B) + 2.0f * S)) + tiny2;
+          float lo = d - e > 0.0f ? (d - e) * (1.0f - 0x1p-20f) : 0.0f;
+          float hi = (d + e) * (1.0f + 0x1p-20f);
+          if (a != a) lo = hi = __builtin_inff();
+          if (ox < xl) {
+            lo_out[rb + ox] = lo;
             lmin = fminf(lmin, hi);
           }
         }
