@@ -115,6 +115,7 @@ void check_kernel_error(int32_t code) {
     case IQ_KERR_NOTINDB: bad(IQ_ERR_NOTINDB, "a forced pick is not in the pattern database");
     case IQ_KERR_ITER: bad(IQ_ERR_ITER, "min-cut push-relabel iteration guard reached");
     case IQ_KERR_SORTMAX: bad(IQ_ERR_UNSUPPORTED, "pattern database with more than " + std::to_string(IQ_SORT_MAX) + " entries and several distinct distances");
+    case IQ_KERR_EMPTYBAND: bad(IQ_ERR_HIP, "internal error: empty pattern band");
     default: bad(IQ_ERR_HIP, "unknown kernel error " + std::to_string(code));
   }
 }
@@ -332,6 +333,7 @@ struct iq_plan {
   size_t cut_lds_max = 0;
   size_t multi_lds = 0;    // LDS of the tiled multi-source maps (0: gathering kernel)
   bool cut_big = false;    // largest slab needs the 512-lane register-resident cut
+  bool cut_wave = false;   // every slab fits the one-wavefront cut
   std::vector<int32_t> enabled_host;
   // learned proposal
   bool lrn_ready = false, lrn_on = false, lrn_fell_back = false;
@@ -791,6 +793,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
       maxn = std::max(maxn, n);
     }
   pl->cut_big = maxn > 4 * CB && maxn <= CUT_BIG_MAX;
+  pl->cut_wave = maxn <= CUT_WAVE_MAX;
   const int64_t cutper = (maxn > IQ_PR_NLDS && !pl->cut_big) ? maxn * CUT_NODE_BYTES : 0;
   size_t per_slot = (size_t)nmap * g.tilevox * 8 + (size_t)kmax * 4 + 64 + (size_t)P * (4 + 8 + 4 + 4 + 16) + nw64 * 8 +
                     (size_t)IQ_SORT_MAX * 12 + (size_t)(nw64 + 1) * 8 + (size_t)IQ_MAX_SLAB * g.tilevox +
@@ -953,7 +956,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     HCL();
     hipLaunchKernelGGL(k_band_recheck, dim3(64, nq), dim3(256), 0, s, D, q);
     HCL();
-    hipLaunchKernelGGL(k_band_finalize, dim3(nq), dim3(FB), 0, s, D, q);
+    hipLaunchKernelGGL(k_band_finalize, dim3(nq), dim3(FB), 0, s, D, q, 1);
     HCL();
   } else if (has_plain) {
     if (pl->fast) {
@@ -998,7 +1001,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
         pl->batch_flop.push_back(f);
         pl->prof_flop += f;
       }
-      hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q);
+      hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256 * SEL_PT), nq), dim3(256), 0, s, D, q);
       HCL();
       hipLaunchKernelGGL(k_band_recheck, dim3(64, nq), dim3(256), 0, s, D, q);
       HCL();
@@ -1006,7 +1009,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
       hipLaunchKernelGGL(k_band_exact_all, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q);
       HCL();
     }
-    hipLaunchKernelGGL(k_band_finalize, dim3(nq), dim3(FB), 0, s, D, q);
+    hipLaunchKernelGGL(k_band_finalize, dim3(nq), dim3(FB), 0, s, D, q, pl->fast ? 1 : 0);
     HCL();
     if (sel == SEL_CHECK) {
       hipLaunchKernelGGL(k_learn_recall, dim3(nq), dim3(256), 0, s, D, q);
@@ -1043,7 +1046,9 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     hipLaunchKernelGGL(k_relax_finalize, dim3(nq), dim3(FB), 0, s, D, q);
     HCL();
   }
-  if (pl->cut_big)
+  if (pl->cut_wave)
+    hipLaunchKernelGGL((k_graphcut_wave<T>), dim3(IQ_MAX_SLAB, nq), dim3(64), pl->cut_lds_max, s, D, q);
+  else if (pl->cut_big)
     hipLaunchKernelGGL((k_graphcut_big<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB_BIG), pl->cut_lds_max, s, D, q);
   else
     hipLaunchKernelGGL((k_graphcut<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB), pl->cut_lds_max, s, D, q);
@@ -1580,7 +1585,13 @@ int iq_graphcut(iq_ctx* ctx, const void* A, const void* B, int dtype, const int6
       HC(hipFuncSetAttribute((const void*)k_graphcut_host<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       HC(hipFuncSetAttribute((const void*)k_graphcut_host<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
-    if (dtype == IQ_F32)
+    if (n <= CUT_WAVE_MAX && dtype == IQ_F32)
+      hipLaunchKernelGGL((k_graphcut_host_wave<float>), dim3(1), dim3(64), lds, s, (const float*)dA, (const float*)dB,
+                         (int)dims[0], (int)dims[1], (int)dims[2], dim, keep, err);
+    else if (n <= CUT_WAVE_MAX)
+      hipLaunchKernelGGL((k_graphcut_host_wave<double>), dim3(1), dim3(64), lds, s, (const double*)dA,
+                         (const double*)dB, (int)dims[0], (int)dims[1], (int)dims[2], dim, keep, err);
+    else if (dtype == IQ_F32)
       hipLaunchKernelGGL((k_graphcut_host<float>), dim3(1), dim3(CB), lds, s, (const float*)dA, (const float*)dB,
                          (int)dims[0], (int)dims[1], (int)dims[2], dim, keep, scratch, err);
     else

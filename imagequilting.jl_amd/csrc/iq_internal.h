@@ -18,6 +18,7 @@
 #define IQ_KERR_NOTINDB 2
 #define IQ_KERR_ITER 3
 #define IQ_KERR_SORTMAX 4
+#define IQ_KERR_EMPTYBAND 5  // internal: a selection produced no pattern (never expected)
 
 // One axis-aligned box inside the tile, [x0, x0+wx) x [y0, y0+wy) x [z0, z0+wz).
 struct IqBox {

@@ -282,6 +282,42 @@ __device__ void cut_slab(const SlabView<T>& A, const SlabView<T>& B, const int s
 // buffered) and the edge residual / push records live in LDS.  The gather of
 // round k runs at the start of round k+1 (push records double buffered), so a
 // round costs two barriers.
+// Team synchronisation of the register-resident cut: a 64-wide team is one
+// wavefront (its LDS accesses complete in order; the fences only stop the
+// compiler from moving them across the step), wider teams use the
+// workgroup barrier.
+template <int TEAM>
+__device__ inline void team_sync() {
+  if constexpr (TEAM == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// OR of a predicate over the team: wavefront ballots, then (wider teams)
+// one flag per wavefront in LDS behind a single barrier; the flags alternate
+// between two buffers so the next call cannot overwrite a flag still unread.
+template <int TEAM>
+__device__ inline bool team_or(int p, int* tflag, int& par) {
+  if constexpr (TEAM == 64) {
+    team_sync<64>();
+    return __any(p);
+  } else {
+    const bool w = __any(p);
+    int* f = tflag + par * (TEAM / 64);
+    if ((threadIdx.x & 63) == 0) f[threadIdx.x >> 6] = w ? 1 : 0;
+    __syncthreads();
+    int r = 0;
+#pragma unroll
+    for (int i = 0; i < TEAM / 64; i++) r |= f[i];
+    par ^= 1;
+    return r != 0;
+  }
+}
+
 struct CutLds {
   double *rf, *rb, *pr0, *pr1;
   int32_t *h0, *h1;
@@ -290,7 +326,7 @@ struct CutLds {
 #define CUT_FAST_NODE_BYTES (8 * 12 + 4 * 2)
 
 template <typename T, int NPT, int TEAM>
-__device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
+__device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
                               uint8_t* out, char* mem, int32_t* err, int64_t* rounds) {
   const int L = s[dim];
   const int nvox = s[0] * s[1] * s[2];
@@ -305,6 +341,8 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
   const int n = si[0] * si[1] * si[2];
   const int HMAX = n + 1;
   const int ist[3] = {1, si[0], si[0] * si[1]};
+  __shared__ int tflag[2 * (TEAM / 64)];
+  int par = 0;
   CutLds S;
   S.rf = (double*)mem;
   S.rb = S.rf + 3 * n;
@@ -350,7 +388,7 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
       fl[k] = f;
     }
   }
-  __syncthreads();
+  team_sync<TEAM>();
 
   int64_t total = 0, levels = 0;
   int32_t* h = S.h0;   // committed heights
@@ -360,7 +398,7 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
 #pragma unroll
     for (int k = 0; k < NPT; k++)
       if (uu[k] < n) h[uu[k]] = ((fl[k] & 8) && rt[k] > 0.0) ? 1 : HMAX;
-    __syncthreads();
+    team_sync<TEAM>();
     // residual reachability towards the sink does not change during the BFS: load it once
     bool canm[NPT][3], canp[NPT][3];
 #pragma unroll
@@ -387,13 +425,13 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
         if (hit) { h[u] = lev + 1; ch = 1; }
       }
       levels++;
-      if (!__syncthreads_or(ch)) break;
+      if (!team_or<TEAM>(ch, tflag, par)) break;
     }
     int act = 0;
 #pragma unroll
     for (int k = 0; k < NPT; k++)
       if (uu[k] < n) act |= (e[k] > 0.0 && h[uu[k]] < HMAX);
-    if (!__syncthreads_or(act)) break;
+    if (!team_or<TEAM>(act, tflag, par)) break;
     if (total > IQ_PR_LIMIT) {
       if (threadIdx.x == 0) atomicCAS(err, 0, IQ_KERR_ITER);
       break;
@@ -446,9 +484,9 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
       }
       if (it > 0) {
         total++;
-        if (!__syncthreads_or(a2)) break;
+        if (!team_or<TEAM>(a2, tflag, par)) break;
       } else {
-        __syncthreads();
+        team_sync<TEAM>();
       }
       if (it == IQ_GR_FREQ) break;
       { int32_t* t = h; h = hn; hn = t; }
@@ -502,7 +540,7 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
         }
         e[k] = ex;
       }
-      __syncthreads();
+      team_sync<TEAM>();
       { double* t = prc; prc = pro; pro = t; }
     }
     // records of the last gathered round are cleared before the next cycle
@@ -512,7 +550,7 @@ __device__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const 
       if (u < n) { pro[3 * u] = 0.0; pro[3 * u + 1] = 0.0; pro[3 * u + 2] = 0.0;
                    prc[3 * u] = 0.0; prc[3 * u + 1] = 0.0; prc[3 * u + 2] = 0.0; }
     }
-    __syncthreads();
+    team_sync<TEAM>();
   }
 #pragma unroll
   for (int k = 0; k < NPT; k++) {
@@ -575,6 +613,49 @@ k_graphcut(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 template __global__ void k_graphcut<float>(const IqDev*, const IqQuery*);
 template __global__ void k_graphcut<double>(const IqDev*, const IqQuery*);
 
+// Plans whose slabs all have at most CUT_WAVE_MAX interior nodes: one
+// wavefront per slab, up to two register-resident nodes per lane, no
+// workgroup barriers in the push-relabel rounds.  (A round costs about the
+// per-node work times the nodes per lane, so beyond two nodes per lane the
+// 256-wide team with one node per lane is faster despite its barriers.)
+#define CUT_WAVE_MAX (2 * 64)
+template <typename T>
+__device__ __forceinline__ void cut_dispatch_wave(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
+                                  uint8_t* out, char* lds, int32_t* err, int64_t* rounds) {
+  const int n = s[0] * s[1] * s[2] / s[dim] * (s[dim] > 2 ? s[dim] - 2 : 0);
+  if (n <= 64) cut_slab_fast<T, 1, 64>(A, B, s, dim, side, out, lds, err, rounds);
+  else cut_slab_fast<T, 2, 64>(A, B, s, dim, side, out, lds, err, rounds);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_graphcut_wave(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  extern __shared__ char cut_lds[];
+  const IqQuery q = qs[blockIdx.y];
+  const IqStep& st = D->steps[q.step];
+  if ((int)blockIdx.x >= st.nslab) return;
+  const int64_t slot = q.slot;
+  const int64_t pk = D->pick[slot];
+  if (pk < 0) return;
+  const IqGeo& g = D->g;
+  const IqSlab sl = st.slab[blockIdx.x];
+  const int s[3] = {sl.sz[0], sl.sz[1], sl.sz[2]};
+  const int rc[3] = {(int)(pk % g.dist[0]), (int)((pk / g.dist[0]) % g.dist[1]),
+                     (int)(pk / ((int64_t)g.dist[0] * g.dist[1]))};
+  SlabView<T> A, B;
+  const IqReal& R = D->reals[q.real];
+  A.base = (const T*)D->sims + R.sim_off +
+           iq_lin3(R.pad, st.origin[0] + sl.lo[0], st.origin[1] + sl.lo[1], st.origin[2] + sl.lo[2]);
+  A.st[0] = 1; A.st[1] = R.pad[0]; A.st[2] = (int64_t)R.pad[0] * R.pad[1];
+  B.base = (const T*)D->ti + iq_lin3(g.ti, rc[0] + sl.lo[0], rc[1] + sl.lo[1], rc[2] + sl.lo[2]);
+  B.st[0] = 1; B.st[1] = g.ti[0]; B.st[2] = (int64_t)g.ti[0] * g.ti[1];
+  uint8_t* out = D->keep + (slot * IQ_MAX_SLAB + blockIdx.x) * g.tilevox;
+  cut_dispatch_wave<T>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, D->err, D->rounds);
+}
+
+template __global__ void k_graphcut_wave<float>(const IqDev*, const IqQuery*);
+template __global__ void k_graphcut_wave<double>(const IqDev*, const IqQuery*);
+
 // Plans whose largest slab has CUT_BIG_MIN..CUT_BIG_MAX interior nodes: 512
 // work-items per slab, three register-resident nodes each (node state of a
 // 1,536-node slab fills the 160 KB of LDS, so one slab per compute unit runs
@@ -628,6 +709,21 @@ k_graphcut_host(const T* __restrict__ a, const T* __restrict__ b, int s0, int s1
 
 template __global__ void k_graphcut_host<float>(const float*, const float*, int, int, int, int, uint8_t*, char*, int32_t*);
 template __global__ void k_graphcut_host<double>(const double*, const double*, int, int, int, int, uint8_t*, char*, int32_t*);
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_graphcut_host_wave(const T* __restrict__ a, const T* __restrict__ b, int s0, int s1, int s2, int dim,
+                     uint8_t* __restrict__ out, int32_t* __restrict__ err) {
+  extern __shared__ char cut_lds[];
+  const int s[3] = {s0, s1, s2};
+  SlabView<T> A, B;
+  A.base = a; A.st[0] = 1; A.st[1] = s0; A.st[2] = (int64_t)s0 * s1;
+  B.base = b; B.st[0] = 1; B.st[1] = s0; B.st[2] = (int64_t)s0 * s1;
+  cut_dispatch_wave<T>(A, B, s, dim, (uint8_t)0, out, cut_lds, err, nullptr);
+}
+
+template __global__ void k_graphcut_host_wave<float>(const float*, const float*, int, int, int, int, uint8_t*, int32_t*);
+template __global__ void k_graphcut_host_wave<double>(const double*, const double*, int, int, int, int, uint8_t*, int32_t*);
 
 // Paste (iqsim.jl:278): voxels whose OR-ed keep flag is 0 take the picked
 // training-image window; the keep flags are the debug cut grid (iqsim.jl:281).

@@ -677,26 +677,38 @@ IQ_SCAN_INST(4, 8, 16, 2)
 // One bitmap bit per candidate (ballot words of 64 candidates) and an
 // unordered list of the members.  Grid: (candidate blocks, query).
 // ---------------------------------------------------------------------------
+#define SEL_PT 8
 __global__ void __launch_bounds__(256)
 k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const IqQuery q = qs[blockIdx.y];
   const IqStep& st = D->steps[q.step];
   if (q.multi || st.mask < 0) return;
-  const IqGeo& g = D->g;
+  const int64_t P = D->g.P;
   const int64_t slot = q.slot;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const double U = (double)__uint_as_float(D->gmin_hi[slot]);
   const float Tf = iq_f32_up((1.0 + D->tol) * U);
-  bool sel = false;
-  if (c < g.P) sel = D->lo32[slot * g.P + c] <= Tf;
-  const unsigned long long bal = __ballot(sel);
+  const auto lo = iq_glob(D->lo32 + slot * P);
   const int lane = threadIdx.x & 63;
-  if (lane == 0 && (c >> 6) < D->nw64) D->bitmap[slot * D->nw64 + (c >> 6)] = bal;
-  if (bal) {
-    int base = 0;
-    if (lane == 0) base = atomicAdd(&D->nsup[slot], __popcll(bal));
-    base = __shfl(base, 0);
-    if (sel) D->sup[slot * g.P + base + __popcll(bal & ((1ull << lane) - 1ull))] = (int32_t)c;
+  // a wavefront covers SEL_PT words of 64 consecutive candidates
+  const int64_t c0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64 * SEL_PT + lane;
+  float v[SEL_PT];
+#pragma unroll
+  for (int j = 0; j < SEL_PT; j++) {
+    const int64_t c = c0 + 64 * j;
+    v[j] = lo[c < P ? c : 0];
+  }
+#pragma unroll
+  for (int j = 0; j < SEL_PT; j++) {
+    const int64_t c = c0 + 64 * j;
+    const bool sel = c < P && v[j] <= Tf;
+    const unsigned long long bal = __ballot(sel);
+    if (lane == 0 && (c >> 6) < D->nw64) D->bitmap[slot * D->nw64 + (c >> 6)] = bal;
+    if (bal) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&D->nsup[slot], __popcll(bal));
+      base = __shfl(base, 0);
+      if (sel) D->sup[slot * P + base + __popcll(bal & ((1ull << lane) - 1ull))] = (int32_t)c;
+    }
   }
 }
 
@@ -740,22 +752,42 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     double* tl = wl + (threadIdx.x >> 6) * (2 * RC_WCH);
     double* ql = tl + RC_WCH;
     const int lane = threadIdx.x & 63;
+    const auto toffg = iq_glob(toff);
+    const auto kvg = iq_glob(kv);
+    const bool f32 = D->ti_f32;
+    const auto ti32 = iq_glob((const float*)D->ti);
+    const auto ti64 = iq_glob((const double*)D->ti);
+    const int d0 = g.dist[0], d1 = g.dist[1];
     for (int e = blockIdx.x * 4 + (int)(threadIdx.x >> 6); e < n; e += gridDim.x * 4) {
-      const int64_t c = D->sup[slot * g.P + e];
-      const int cx = (int)(c % g.dist[0]), cy = (int)((c / g.dist[0]) % g.dist[1]);
-      const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
+      const int32_t c = D->sup[slot * g.P + e];  // candidate indices fit 31 bits
+      const int cx = c % d0, cy = (c / d0) % d1, cz = c / (d0 * d1);
       const int64_t base = iq_lin3(g.ti, cx, cy, cz);
       double rt[RC_WCH / 64], rq[RC_WCH / 64];
+      // all offsets of a chunk first, then all window values (two latencies)
+      auto fetch = [&](int j0) {
+        int32_t o[RC_WCH / 64];
 #pragma unroll
-      for (int i = 0; i < RC_WCH / 64; i++) {
-        const int k = lane + 64 * i;
-        rt[i] = 0.0;
-        rq[i] = 0.0;
-        if (k < V) {
-          rt[i] = D->ti_f32 ? (double)((const float*)D->ti)[base + toff[k]] : ((const double*)D->ti)[base + toff[k]];
-          rq[i] = kv[k];
+        for (int i = 0; i < RC_WCH / 64; i++) {
+          const int k = j0 + lane + 64 * i;
+          const int kc = k < V ? k : 0;
+          o[i] = toffg[kc];
+          rq[i] = kvg[kc];
         }
-      }
+        if (f32) {  // uniform: the training image is read in its own element type
+#pragma unroll
+          for (int i = 0; i < RC_WCH / 64; i++) rt[i] = (double)ti32[base + o[i]];
+        } else {
+#pragma unroll
+          for (int i = 0; i < RC_WCH / 64; i++) rt[i] = ti64[base + o[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < RC_WCH / 64; i++) {
+          const bool ok = j0 + lane + 64 * i < V;
+          rt[i] = ok ? rt[i] : 0.0;
+          rq[i] = ok ? rq[i] : 0.0;
+        }
+      };
+      fetch(0);
       double ab = 0.0;
       for (int j0 = 0; j0 < V; j0 += RC_WCH) {
         const int nj = V - j0 < RC_WCH ? V - j0 : RC_WCH;
@@ -767,22 +799,24 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int j1 = j0 + RC_WCH;
-#pragma unroll
-        for (int i = 0; i < RC_WCH / 64; i++) {  // next chunk in flight during the chain
-          const int k = j1 + lane + 64 * i;
-          if (k < V) {
-            rt[i] = D->ti_f32 ? (double)((const float*)D->ti)[base + toff[k]] : ((const double*)D->ti)[base + toff[k]];
-            rq[i] = kv[k];
-          }
-        }
+        if (j0 + RC_WCH < V) fetch(j0 + RC_WCH);  // next chunk in flight during the chain
+        // eight LDS pairs of the next group in flight while the ordered fmas
+        // of the current group run
         int k = 0;
-        for (; k + 8 <= nj; k += 8) {  // eight LDS pairs in flight, then the ordered fmas
-          double t[8], u[8];
+        double t[8], u[8];
+        if (nj >= 8) {
 #pragma unroll
-          for (int i = 0; i < 8; i++) { t[i] = tl[k + i]; u[i] = ql[k + i]; }
+          for (int i = 0; i < 8; i++) { t[i] = tl[i]; u[i] = ql[i]; }
+        }
+        for (; k + 8 <= nj; k += 8) {
+          double tn[8], un[8];
+          const int kn = k + 16 <= nj ? k + 8 : k;
+#pragma unroll
+          for (int i = 0; i < 8; i++) { tn[i] = tl[kn + i]; un[i] = ql[kn + i]; }
 #pragma unroll
           for (int i = 0; i < 8; i++) ab = __builtin_fma(t[i], u[i], ab);
+#pragma unroll
+          for (int i = 0; i < 8; i++) { t[i] = tn[i]; u[i] = un[i]; }
         }
         for (; k < nj; k++) ab = __builtin_fma(tl[k], ql[k], ab);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -428,7 +428,7 @@ __device__ bool tau_weights(int64_t n, int nsrc, F src_vals, double* vals, int32
 // band is the enabled list and every rank is 1.
 // ---------------------------------------------------------------------------
 extern "C" __global__ void __launch_bounds__(FB)
-k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int use_list) {
   __shared__ SelScratch sh;
   __shared__ double sv[IQ_SORT_MAX];
   __shared__ int32_t dp[IQ_SORT_MAX];
@@ -459,45 +459,77 @@ k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     return;
   }
   const double thr = (1.0 + D->tol) * m;  // iqsim.jl:237
-  const unsigned long long* bitmap = D->bitmap + slot * D->nw64;
-  const double* dexact = D->dexact + slot * g.P;
   int32_t* band = D->band + slot * g.P;
-  const int64_t nw = D->nw64;
-  const int64_t per = (nw + FB - 1) / FB;
-  const int64_t w0 = threadIdx.x * per, w1 = (w0 + per < nw) ? w0 + per : nw;
-  int64_t cnt = 0;
-  double lmax = -__builtin_inf();
-  for (int64_t w = w0; w < w1; w++) {
-    unsigned long long b = bitmap[w];
-    while (b) {
-      int bit = __builtin_ctzll(b);
-      b &= b - 1;
-      double d = dexact[w * 64 + bit];
-      if (d <= thr) { cnt++; lmax = fmax(lmax, d); }
-    }
-  }
+  const int64_t nsup = D->nsup[slot];
   int64_t nb;
-  int64_t off = blk_excl_scan(cnt, &nb, sh);
-  const double dmax = blk_max_f64(lmax, sh);
-  if (threadIdx.x == 0 && D->rounds) {  // work counters: superset and band sizes
-    atomicAdd((unsigned long long*)D->rounds + 3, 1ull);
-    atomicAdd((unsigned long long*)D->rounds + 4, (unsigned long long)D->nsup[slot]);
-    atomicAdd((unsigned long long*)D->rounds + 5, (unsigned long long)nb);
-  }
-  const bool small = nb <= IQ_SORT_MAX;
-  for (int64_t w = w0; w < w1; w++) {
-    unsigned long long b = bitmap[w];
-    while (b) {
-      int bit = __builtin_ctzll(b);
-      b &= b - 1;
-      int64_t c = w * 64 + bit;
-      double d = dexact[c];
-      if (d <= thr) {
-        band[off] = (int32_t)c;
-        if (small) sd[off] = d;
-        off++;
+  double dmax;
+  bool small;
+  if (use_list && nsup <= FB) {
+    // usual case: the superset list (unordered) holds every candidate of the
+    // band; each member's place in ascending candidate order is the number
+    // of band members with a smaller index.  (use_list = 0: the bitmap was
+    // filled without a list, by the exhaustive exact map.)
+    const auto dexact = iq_glob(D->dexact + slot * g.P);
+    int32_t c = 0;
+    double d = __builtin_inf();
+    if ((int64_t)threadIdx.x < nsup) {
+      c = D->sup[slot * g.P + threadIdx.x];
+      d = dexact[c];
+    }
+    const bool inb = (int64_t)threadIdx.x < nsup && d <= thr;
+    dp[threadIdx.x] = inb ? c : 0x7fffffff;
+    blk_excl_scan(inb ? 1 : 0, &nb, sh);
+    dmax = blk_max_f64(inb ? d : -__builtin_inf(), sh);
+    small = true;
+    if (inb) {
+      int r = 0;
+      for (int j = 0; j < (int)nsup; j++) r += dp[j] < c;
+      band[r] = c;
+      sd[r] = d;
+    }
+  } else {
+    const unsigned long long* bitmap = D->bitmap + slot * D->nw64;
+    const double* dexact = D->dexact + slot * g.P;
+    const int64_t nw = D->nw64;
+    const int64_t per = (nw + FB - 1) / FB;
+    const int64_t w0 = threadIdx.x * per, w1 = (w0 + per < nw) ? w0 + per : nw;
+    int64_t cnt = 0;
+    double lmax = -__builtin_inf();
+    for (int64_t w = w0; w < w1; w++) {
+      unsigned long long b = bitmap[w];
+      while (b) {
+        int bit = __builtin_ctzll(b);
+        b &= b - 1;
+        double d = dexact[w * 64 + bit];
+        if (d <= thr) { cnt++; lmax = fmax(lmax, d); }
       }
     }
+    int64_t off = blk_excl_scan(cnt, &nb, sh);
+    dmax = blk_max_f64(lmax, sh);
+    small = nb <= IQ_SORT_MAX;
+    for (int64_t w = w0; w < w1; w++) {
+      unsigned long long b = bitmap[w];
+      while (b) {
+        int bit = __builtin_ctzll(b);
+        b &= b - 1;
+        int64_t c = w * 64 + bit;
+        double d = dexact[c];
+        if (d <= thr) {
+          band[off] = (int32_t)c;
+          if (small) sd[off] = d;
+          off++;
+        }
+      }
+    }
+  }
+  if (threadIdx.x == 0 && D->rounds) {  // work counters: superset and band sizes
+    atomicAdd((unsigned long long*)D->rounds + 3, 1ull);
+    atomicAdd((unsigned long long*)D->rounds + 4, (unsigned long long)nsup);
+    atomicAdd((unsigned long long*)D->rounds + 5, (unsigned long long)nb);
+  }
+  if (nb == 0) {  // the exact minimum is always in the band; never sample an empty list
+    if (threadIdx.x == 0) { set_err(D->err, IQ_KERR_EMPTYBAND); *o.pick = -2; }
+    return;
   }
   __syncthreads();
   if (nb == 1) {
@@ -517,6 +549,7 @@ k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     // large band: radix-sorted ranks; ranks go to the (consumed) superset
     // list, weights over the key scratch once the ranks are known
     unsigned long long* keys = D->rkey + slot * 2 * g.P;
+    const double* dexact = D->dexact + slot * g.P;
     rk = D->sup + slot * g.P;
     blk_dense_rank_big([&](int64_t i) { return dexact[band[i]]; }, nb, rk, keys, (int32_t*)sv, sh);
     p = (double*)keys;
