@@ -721,7 +721,7 @@ k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 // values staged by the wavefront, every lane running the same chain (short
 // latency when the superset is small).  The exact minimum is reduced with an
 // integer atomic on the non-negative double bits.
-// Grid: (groups per query, query), 256 work-items per group.
+// Grid: (query, groups per query), 256 work-items per group.
 // ---------------------------------------------------------------------------
 #define RC_CHUNK 1024
 #define RC_SMALL 512
@@ -732,7 +732,12 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   __shared__ int32_t ol[RC_CHUNK];
   __shared__ double wl[4 * 2 * RC_WCH];
   __shared__ unsigned long long wbest[4];
-  const IqQuery q = qs[blockIdx.y];
+  // grid (query, member group): the groups holding members of every query
+  // are dispatched first, and groups past a query's superset leave at once
+  const IqQuery q = qs[blockIdx.x];
+  const int grp = blockIdx.y, ngrp = gridDim.y;
+  const int n = D->nsup[q.slot];
+  if (grp * (n <= RC_SMALL ? 4 : 256) >= n) return;
   const IqStep& st = D->steps[q.step];
   if (q.multi || st.mask < 0) return;
   const IqGeo& g = D->g;
@@ -743,7 +748,6 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const double* kv = D->kv + slot * D->kv_stride;
   const double* a2map = D->a2maps[m.a2];
   const double b2 = D->b2[slot * (2 + D->naux)];
-  const int n = D->nsup[slot];
   double* dex = D->dexact + slot * g.P;
   unsigned long long best = 0x7ff0000000000000ull;
   if (n <= RC_SMALL) {  // few members: a wavefront per member, its chain fed from LDS
@@ -758,7 +762,7 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     const auto ti32 = iq_glob((const float*)D->ti);
     const auto ti64 = iq_glob((const double*)D->ti);
     const int d0 = g.dist[0], d1 = g.dist[1];
-    for (int e = blockIdx.x * 4 + (int)(threadIdx.x >> 6); e < n; e += gridDim.x * 4) {
+    for (int e = grp * 4 + (int)(threadIdx.x >> 6); e < n; e += ngrp * 4) {
       const int32_t c = D->sup[slot * g.P + e];  // candidate indices fit 31 bits
       const int cx = c % d0, cy = (c / d0) % d1, cz = c / (d0 * d1);
       const int64_t base = iq_lin3(g.ti, cx, cy, cz);
@@ -829,7 +833,7 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
       best = bits < best ? bits : best;
     }
   }
-  for (int e0 = blockIdx.x * 256; n > RC_SMALL && e0 < n; e0 += gridDim.x * 256) {  // uniform over the group
+  for (int e0 = grp * 256; n > RC_SMALL && e0 < n; e0 += ngrp * 256) {  // uniform over the group
     const int e = e0 + (int)threadIdx.x;
     const bool live = e < n;
     int64_t c = 0, base = 0;

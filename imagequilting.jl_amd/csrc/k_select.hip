@@ -463,7 +463,13 @@ k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int
   const int64_t nsup = D->nsup[slot];
   int64_t nb;
   double dmax;
-  bool small;
+  bool small, inlds = false;
+  // band, distances, ranks and weights of a list-built band stay in LDS
+  // (upper halves of sv / dp; the LDS ranker uses at most their first FB)
+  static_assert(FB <= IQ_SORT_MAX / 4, "LDS band regions overlap the ranker's");
+  static_assert((FB + IQ_SB - 1) / IQ_SB <= S2MAX / 2, "draw scratch of a list-built band");
+  int32_t* lband = dp + IQ_SORT_MAX / 2;
+  double* ldist = sv + IQ_SORT_MAX / 2;
   if (use_list && nsup <= FB) {
     // usual case: the superset list (unordered) holds every candidate of the
     // band; each member's place in ascending candidate order is the number
@@ -481,11 +487,13 @@ k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int
     blk_excl_scan(inb ? 1 : 0, &nb, sh);
     dmax = blk_max_f64(inb ? d : -__builtin_inf(), sh);
     small = true;
+    inlds = true;
     if (inb) {
       int r = 0;
       for (int j = 0; j < (int)nsup; j++) r += dp[j] < c;
       band[r] = c;
-      sd[r] = d;
+      lband[r] = c;
+      ldist[r] = d;
     }
   } else {
     const unsigned long long* bitmap = D->bitmap + slot * D->nw64;
@@ -532,19 +540,21 @@ k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int
     return;
   }
   __syncthreads();
+  const int32_t* db = inlds ? lband : band;
+  double* ps1 = inlds ? s2 + S2MAX / 2 : s1;  // the top level of the draw needs few entries of s2
   if (nb == 1) {
-    publish_pick(band, 1, nullptr, 1.0, q.u, q.forced, s1, s2, o, sh);  // taumodel.jl:9
+    publish_pick(db, 1, nullptr, 1.0, q.u, q.forced, ps1, s2, o, sh);  // taumodel.jl:9
     return;
   }
   if (dmax == m) {  // every band distance equal: all ranks 1
-    publish_pick(band, nb, nullptr, tau_constant_weight(nb), q.u, q.forced, s1, s2, o, sh);
+    publish_pick(db, nb, nullptr, tau_constant_weight(nb), q.u, q.forced, ps1, s2, o, sh);
     return;
   }
-  double* p = sd;
-  int32_t* rk = si;
+  double* p = inlds ? sv + 3 * (IQ_SORT_MAX / 4) : sd;
+  int32_t* rk = inlds ? dp + 3 * (IQ_SORT_MAX / 4) : si;
   if (small) {
     bool c;
-    blk_dense_rank(sd, nb, si, &c, sv, dp, sh);
+    blk_dense_rank(inlds ? ldist : sd, nb, rk, &c, sv, dp, sh);
   } else {
     // large band: radix-sorted ranks; ranks go to the (consumed) superset
     // list, weights over the key scratch once the ranks are known
@@ -558,7 +568,7 @@ k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int
   __syncthreads();
   tau_accumulate(nb, rk, false, p, sh);
   tau_finish(nb, p);
-  publish_pick(band, nb, p, 0.0, q.u, q.forced, s1, s2, o, sh);
+  publish_pick(db, nb, p, 0.0, q.u, q.forced, ps1, s2, o, sh);
 }
 
 // ---------------------------------------------------------------------------
