@@ -318,6 +318,10 @@ struct iq_plan {
   bool scan_perz = false;
   bool mfma = false;        // multi-query MFMA scan
   bool mfma_split = true;   // FP32 data: three bf16 products
+  int mfma_nw = 4;          // wavefronts per scan group (4 candidate rows each), forced by IQHIP_SCAN_NW
+  bool mfma_nw_forced = false;
+  const int32_t* kofs_nw[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // chunk offsets per group size
+  size_t lds_nw[5] = {0, 0, 0, 0, 0};
   size_t mfma_lds = 0;
   double mfma_gerr = 0.0;
   int64_t mfma_ntile_max = 0;
@@ -697,12 +701,32 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
     }
     std::vector<IqChunk> chunks;
     if (want && pl->fast && !hmasks.empty() && build_ktab(ubits, t3, brk, H.ktab, chunks)) {
-      size_t lmax = 0;
-      for (int b = 0; b < H.ktab.nbox; b++) {
-        const IqKBox& kb = H.ktab.box[b];
-        lmax = std::max(lmax, (size_t)(32 + kb.wxr - 1) * (16 + kb.wyr - 1) * kb.wz * 4);
+      // Group size: a group of nw wavefronts covers 4 nw candidate rows.
+      // Launches of at most about four 4-wavefront groups per compute unit
+      // (few queries, a small candidate grid) run faster with 2-wavefront
+      // groups, which spread the same rows over twice as many groups; larger
+      // launches keep 4 wavefronts, whose staged block serves 16 rows.  The
+      // choice is made per launch (run_batch); both layouts are prepared.
+      auto lds_of = [&](int nw) {
+        size_t l = 0;
+        for (int b = 0; b < H.ktab.nbox; b++) {
+          const IqKBox& kb = H.ktab.box[b];
+          l = std::max(l, (size_t)(32 + kb.wxr - 1) * (4 * nw + kb.wyr - 1) * kb.wz * 4);
+        }
+        return l;
+      };
+      if (const char* envnw = std::getenv("IQHIP_SCAN_NW")) {
+        pl->mfma_nw = std::atoi(envnw) >= 4 ? 4 : 2;
+        pl->mfma_nw_forced = true;
       }
-      if (lmax <= 150 * 1024) {
+      size_t lmax = 0;
+      for (int nw : {2, 4}) {
+        if (pl->mfma_nw_forced && nw != pl->mfma_nw) continue;
+        if (lds_of(nw) > 150 * 1024) continue;
+        pl->lds_nw[nw] = lds_of(nw);
+        lmax = std::max(lmax, pl->lds_nw[nw]);
+      }
+      if (pl->lds_nw[2] || pl->lds_nw[4]) {
         pl->mfma = true;
         pl->mfma_lds = lmax;
         pl->mfma_split = !bf16_exact(H.ti32, tivox);
@@ -710,16 +734,19 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
         const double ga = 2.0 * kterms * 0x1p-24 / (1.0 - 2.0 * kterms * 0x1p-24);
         pl->mfma_gerr = 1.02 * ga + (pl->mfma_split ? 4.0 * 0x1p-18 : 0.0);
         H.chunks = pl->mem.upload(chunks.data(), chunks.size(), s);
-        std::vector<int32_t> kofs(chunks.size());
-        for (int b = 0; b < H.ktab.nbox; b++) {
-          const IqKBox& kb = H.ktab.box[b];
-          const int RX = 32 + kb.wxr - 1, RY = 16 + kb.wyr - 1;
-          for (int k = 2 * kb.s0; k < 2 * kb.s1; k++) {
-            const IqChunk& c = chunks[k];
-            kofs[k] = ((c.jz - kb.z0) * RX * RY + (c.jy - kb.y0) * RX + (c.jx - kb.x0)) | (c.dir ? (1 << 30) : 0);
+        for (int nw : {2, 4}) {
+          if (!pl->lds_nw[nw]) continue;
+          std::vector<int32_t> kofs(chunks.size());
+          for (int b = 0; b < H.ktab.nbox; b++) {
+            const IqKBox& kb = H.ktab.box[b];
+            const int RX = 32 + kb.wxr - 1, RY = 4 * nw + kb.wyr - 1;
+            for (int k = 2 * kb.s0; k < 2 * kb.s1; k++) {
+              const IqChunk& c = chunks[k];
+              kofs[k] = ((c.jz - kb.z0) * RX * RY + (c.jy - kb.y0) * RX + (c.jx - kb.x0)) | (c.dir ? (1 << 30) : 0);
+            }
           }
+          pl->kofs_nw[nw] = pl->mem.upload(kofs.data(), kofs.size(), s);
         }
-        H.kofs = pl->mem.upload(kofs.data(), kofs.size(), s);
         std::vector<uint8_t> mb((size_t)hmasks.size() * g.tilevox);
         for (size_t m = 0; m < hmasks.size(); m++) std::memcpy(&mb[m * g.tilevox], hmasks[m].bits.data(), g.tilevox);
         H.maskbits = pl->mem.upload(mb.data(), mb.size(), s);
@@ -817,6 +844,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   H.kv_stride = (int64_t)nmap * g.tilevox;
   H.kv32 = pl->mem.alloc<float>((size_t)NS * kmax);
   H.kv32_stride = kmax;
+  H.kv32_on = pl->fast && !pl->mfma ? 1 : 0;
   H.b2 = pl->mem.alloc<double>((size_t)NS * nmap);
   H.gmin_hi = pl->mem.alloc<uint32_t>(NS);
   H.gmin64 = pl->mem.alloc<unsigned long long>(NS);
@@ -971,20 +999,27 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
         const int ntile = (nq + 31) / 32;
         hipLaunchKernelGGL(k_mfma_bprep, dim3(pl->hd.ktab.nstep, ntile), dim3(64), 0, s, D, q, nq);
         HCL();
-        dim3 grid(nblk(g.dist[0], 32), nblk(g.dist[1], 16), (unsigned)(g.dist[2] * ntile));
+        int nw = pl->lds_nw[4] ? 4 : 2;
+        if (!pl->mfma_nw_forced && pl->lds_nw[2] &&
+            (double)nblk(g.dist[0], 32) * nblk(g.dist[1], 16) * g.dist[2] * ntile <= 4.0 * 256.0)
+          nw = 2;
+        const int YB = 4 * nw;
+        dim3 grid(nblk(g.dist[0], 32), nblk(g.dist[1], YB), (unsigned)(g.dist[2] * ntile));
         if (pl->prof) {
           HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));
           // four 32-candidate row blocks per wavefront with a row inside the
           // grid, per x block and z x query tile: k-steps x (3 or 1) MFMAs
           double rows = 0.0;
-          for (int y0 = 0; y0 < g.dist[1]; y0 += 16) rows += 4.0 * ((std::min(16, g.dist[1] - y0) + 3) / 4);
+          for (int y0 = 0; y0 < g.dist[1]; y0 += YB) rows += 4.0 * ((std::min(YB, g.dist[1] - y0) + 3) / 4);
           pl->prof_mflop += (double)grid.x * rows * grid.z * pl->hd.ktab.nstep * (pl->mfma_split ? 3.0 : 1.0) *
                             (2.0 * 32 * 32 * 16);
         }
         if (pl->mfma_split)
-          hipLaunchKernelGGL((k_scan_mfma<true>), grid, dim3(256), pl->mfma_lds, s, D, q, nq, pl->mfma_gerr);
+          hipLaunchKernelGGL((k_scan_mfma<true>), grid, dim3(64 * nw), pl->lds_nw[nw], s, D, q, nq, pl->mfma_gerr,
+                             pl->kofs_nw[nw]);
         else
-          hipLaunchKernelGGL((k_scan_mfma<false>), grid, dim3(256), pl->mfma_lds, s, D, q, nq, pl->mfma_gerr);
+          hipLaunchKernelGGL((k_scan_mfma<false>), grid, dim3(64 * nw), pl->lds_nw[nw], s, D, q, nq, pl->mfma_gerr,
+                             pl->kofs_nw[nw]);
         HCL();
       } else {
         if (pl->prof) HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));

@@ -158,7 +158,7 @@ struct IqDev {
   int64_t* rounds;
   // per-slot scratch
   double* kv;        int64_t kv_stride;    // (2 + naux) query vectors of tilevox
-  float* kv32;       int64_t kv32_stride;
+  float* kv32;       int64_t kv32_stride;  int32_t kv32_on;  // FP32 query boxes (VALU scan only)
   double* b2;                              // (2 + naux) per slot
   uint32_t* gmin_hi;
   unsigned long long* gmin64;
@@ -185,7 +185,6 @@ struct IqDev {
   // MFMA scan
   IqKTab ktab;
   const IqChunk* chunks;
-  const int32_t* kofs;                     // per chunk: offset in its box's LDS stage, bit 30: run along y
   const uint8_t* maskbits;                 // per mask: tilevox membership bytes
   uint4* bh;                               // B fragments (hi), [ntile][nstep][64]
   uint4* bl;                               // B fragments (lo)

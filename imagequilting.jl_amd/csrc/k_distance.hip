@@ -128,15 +128,40 @@ k_query_prep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const IqReal& R = D->reals[q.real];
   const T* sim = (const T*)D->sims + R.sim_off;
   const double* dgrid = src == 1 ? R.hard : (src >= 2 ? R.aux[src - 2] : nullptr);
-  for (int j = threadIdx.x; j < V; j += blockDim.x) {
-    int t = tv[j];
-    int x = t % g.tile[0], y = (t / g.tile[0]) % g.tile[1], z = t / (g.tile[0] * g.tile[1]);
-    int64_t gi = iq_lin3(R.pad, ox + x, oy + y, oz + z);
-    double v = src == 0 ? (double)sim[gi] : dgrid[gi];
-    kv[j] = v;
-    if (j < PREP_LDS) kl[j] = v;
+  // four voxels per pass: offsets, then grid values, all loads in flight
+  const auto tvg = iq_glob(tv);
+  const int t0 = g.tile[0], t1 = g.tile[1];
+  for (int j0 = threadIdx.x; j0 < V; j0 += 4 * blockDim.x) {
+    int t[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int j = j0 + u * (int)blockDim.x;
+      t[u] = tvg[j < V ? j : 0];
+    }
+    int64_t gi[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int x = t[u] % t0, y = (t[u] / t0) % t1, z = t[u] / (t0 * t1);
+      gi[u] = iq_lin3(R.pad, ox + x, oy + y, oz + z);
+    }
+    double v[4];
+    if (src == 0) {  // uniform per group
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = (double)sim[gi[u]];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = dgrid[gi[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int j = j0 + u * (int)blockDim.x;
+      if (j < V) {
+        kv[j] = v[u];
+        if (j < PREP_LDS) kl[j] = v[u];
+      }
+    }
   }
-  if (src == 0 && mask && !q.multi && D->ti32) {
+  if (src == 0 && mask && !q.multi && D->ti32 && D->kv32_on) {
     float* kv32 = D->kv32 + slot * D->kv32_stride;
     for (int b = 0; b < mask->nbox; b++) {
       const IqBox bx = mask->box[b];
@@ -420,10 +445,11 @@ template __global__ void k_scan2_f32<4, 4, 16, 4>(const IqDev*, const IqQuery*, 
 // data is split t = hi + lo (two bf16) and three products are accumulated
 // (hi*hi, hi*lo, lo*hi); bf16-exact data (small integer codes) needs one.
 // Each wavefront owns four 32-candidate rows (consecutive y) of one z, the
-// group (four wavefronts) sixteen rows; the training-image block of each box
+// group (two to four wavefronts, chosen per plan so that the groups of a
+// launch pack the compute units evenly) 8 to 16 rows; the training-image block of each box
 // is staged in LDS as words (hi | lo << 16), and A fragments are gathered
 // from it along the chunk runs, so any run alignment works.
-// Grid: x blocks (32), y blocks (16), z = candidate z + dist_z * query tile.
+// Grid: x blocks (32), y blocks (4 x wavefronts), z = candidate z + dist_z * query tile.
 // ---------------------------------------------------------------------------
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 iq_bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float iq_f32x16;
@@ -456,17 +482,36 @@ k_mfma_bprep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq
       const IqChunk ch = D->chunks[2 * step + (lane >> 5)];
       const uint8_t* mb = D->maskbits + (int64_t)st.mask * g.tilevox;
       const IqReal& R = D->reals[q.real];
-      const float* sim32 = D->ti_f32 ? (const float*)D->sims + R.sim_off : nullptr;
-      const double* sim64 = D->ti_f32 ? nullptr : (const double*)D->sims + R.sim_off;
-      for (int e = 0; e < ch.n; e++) {
-        const int x = ch.jx + (ch.dir == 0 ? e : 0), y = ch.jy + (ch.dir == 1 ? e : 0), z = ch.jz;
-        const int tv = x + g.tile[0] * (y + g.tile[1] * z);
-        if (!mb[tv]) continue;
-        const int64_t gi = iq_lin3(R.pad, st.origin[0] + x, st.origin[1] + y, st.origin[2] + z);
-        const float v = sim32 ? sim32[gi] : (float)sim64[gi];
-        const uint32_t wv = iq_split_word(v);
-        h[e >> 1] |= (wv & 0xffffu) << (16 * (e & 1));
-        l[e >> 1] |= (wv >> 16) << (16 * (e & 1));
+      if (ch.n > 0) {
+        // mask bytes, then grid values of the chunk's (up to) eight voxels,
+        // each batch of loads in flight together; positions past the chunk
+        // repeat its last voxel and are dropped
+        int64_t gi[8];
+        uint8_t on[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const int ee = e < ch.n ? e : ch.n - 1;
+          const int x = ch.jx + (ch.dir == 0 ? ee : 0), y = ch.jy + (ch.dir == 1 ? ee : 0), z = ch.jz;
+          on[e] = mb[x + g.tile[0] * (y + g.tile[1] * z)];
+          gi[e] = iq_lin3(R.pad, st.origin[0] + x, st.origin[1] + y, st.origin[2] + z);
+        }
+        float v[8];
+        if (D->ti_f32) {
+          const float* sim32 = (const float*)D->sims + R.sim_off;
+#pragma unroll
+          for (int e = 0; e < 8; e++) v[e] = sim32[gi[e]];
+        } else {
+          const double* sim64 = (const double*)D->sims + R.sim_off;
+#pragma unroll
+          for (int e = 0; e < 8; e++) v[e] = (float)sim64[gi[e]];
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          if (e >= ch.n || !on[e]) continue;
+          const uint32_t wv = iq_split_word(v[e]);
+          h[e >> 1] |= (wv & 0xffffu) << (16 * (e & 1));
+          l[e >> 1] |= (wv >> 16) << (16 * (e & 1));
+        }
       }
     }
   }
@@ -485,7 +530,8 @@ k_a2_f32(const double* __restrict__ a2, const uint8_t* __restrict__ disabled, in
 
 template <bool SPLIT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq, double gerr) {
+k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq, double gerr,
+            const int32_t* __restrict__ kofs_tab) {
   extern __shared__ uint32_t lw[];
   __shared__ float red[4][32];
   const IqGeo& g = D->g;
@@ -493,7 +539,8 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
   const int row = lane & 31, kb = lane >> 5;
   const int tile = blockIdx.z / g.dist[2];
   const int cz = blockIdx.z % g.dist[2];
-  const int cx0 = blockIdx.x * 32, cy0 = blockIdx.y * 16;
+  const int nw = (int)(blockDim.x >> 6), NT = (int)blockDim.x;  // 2 to 4 wavefronts, 4 rows each
+  const int cx0 = blockIdx.x * 32, cy0 = blockIdx.y * 4 * nw;
   const bool wlive = cy0 + 4 * w < g.dist[1];  // uniform per wavefront
   const int nstep = D->ktab.nstep;
   const int t0 = g.ti[0], t1 = g.ti[1], t2 = g.ti[2];
@@ -507,15 +554,15 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
     for (int v = 0; v < 16; v++) acc[i][v] = 0.0f;
   for (int b = 0; b < D->ktab.nbox; b++) {
     const IqKBox kbx = D->ktab.box[b];
-    const int RX = 32 + kbx.wxr - 1, RY = 16 + kbx.wyr - 1;
+    const int RX = 32 + kbx.wxr - 1, RY = 4 * nw + kbx.wyr - 1;
     const int n = RX * RY * kbx.wz;
     const int gx0 = cx0 + kbx.x0, gy0 = cy0 + kbx.y0, gz0 = cz + kbx.z0;
-    // element i = x + RX (y + RY z); coordinates advanced incrementally (256
+    // element i = x + RX (y + RY z); coordinates advanced incrementally (NT
     // per element, SU elements per pass with all their loads issued first);
     // windows past the training image read as zero
     int x = (int)threadIdx.x % RX, y = ((int)threadIdx.x / RX) % RY, z = (int)threadIdx.x / (RX * RY);
-    const int sx = 256 % RX, sy = (256 / RX) % RY, sz = 256 / (RX * RY);
-    for (int i0 = threadIdx.x; i0 < n; i0 += 256 * SU) {
+    const int sx = NT % RX, sy = (NT / RX) % RY, sz = NT / (RX * RY);
+    for (int i0 = threadIdx.x; i0 < n; i0 += NT * SU) {
       float v[SU];
       bool in[SU];
 #pragma unroll
@@ -531,13 +578,13 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
       }
 #pragma unroll
       for (int u = 0; u < SU; u++)
-        if (i0 + 256 * u < n) lw[i0 + 256 * u] = iq_split_word(in[u] ? v[u] : 0.0f);
+        if (i0 + NT * u < n) lw[i0 + NT * u] = iq_split_word(in[u] ? v[u] : 0.0f);
     }
     __syncthreads();
     // B fragments and chunk offsets are loaded one k-step ahead
     const auto bhp = iq_glob((const iq_u32x4*)(D->bh + (int64_t)tile * nstep * 64 + lane));
     const auto blp = iq_glob((const iq_u32x4*)(D->bl + (int64_t)tile * nstep * 64 + lane));
-    const auto kofs = iq_glob(D->kofs);
+    const auto kofs = iq_glob(kofs_tab);
     iq_u32x4 bh_n = {0u, 0u, 0u, 0u}, bl_n = {0u, 0u, 0u, 0u};
     int ko_n = 0;
     if (kbx.s0 < kbx.s1) {
@@ -656,13 +703,14 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
   if (kb == 0) red[w][row] = lmin;
   __syncthreads();
   if (w == 0 && kb == 0 && live) {
-    const float m = fminf(fminf(red[0][row], red[1][row]), fminf(red[2][row], red[3][row]));
+    float m = red[0][row];
+    for (int i = 1; i < nw; i++) m = fminf(m, red[i][row]);
     atomicMin(&D->gmin_hi[slot], __float_as_uint(m));
   }
 }
 
-template __global__ void k_scan_mfma<true>(const IqDev*, const IqQuery*, int, double);
-template __global__ void k_scan_mfma<false>(const IqDev*, const IqQuery*, int, double);
+template __global__ void k_scan_mfma<true>(const IqDev*, const IqQuery*, int, double, const int32_t*);
+template __global__ void k_scan_mfma<false>(const IqDev*, const IqQuery*, int, double, const int32_t*);
 
 #define IQ_SCAN_INST(R, LX, LY, LZ) \
   template __global__ void k_scan_f32<R, LX, LY, LZ>(const IqDev*, const IqQuery*, int);
