@@ -156,6 +156,7 @@ def main():
     elapsed = time.perf_counter() - t0
     cut_rounds, cut_levels, ncuts = plan.cut_stats()
     nsel, nsuper, nband = plan.select_stats()
+    tune_nw, tune_us = plan.scan_tuning()
     voxels = float(sum(np.prod(z) for z in simsizes)) * args.steps
     achieved = scan_flop / scan_sec / 1e12 if scan_sec > 0 else 0.0
     # training images whose values are exact in bf16 (facies codes, binary)
@@ -185,6 +186,9 @@ def main():
                        "relabel_levels_per_cut": cut_levels / max(ncuts, 1)},
             "selection": {"queries": nsel, "superset_per_query": nsuper / max(nsel, 1),
                           "band_per_query": nband / max(nsel, 1)},
+            "scan_tuning": {"wavefronts_per_group": {"1_tile": tune_nw[0], "2+_tiles": tune_nw[1]},
+                            "us_per_launch": {f"{t}_tile_{wv}_wavefronts": round(v, 1)
+                                              for (t, wv), v in sorted(tune_us.items())}},
         },
         "roofline": {
             "bound": "mfma",

@@ -320,6 +320,8 @@ struct iq_plan {
   bool mfma_split = true;   // FP32 data: three bf16 products
   int mfma_nw = 4;          // wavefronts per scan group (4 candidate rows each), forced by IQHIP_SCAN_NW
   bool mfma_nw_forced = false;
+  int nw_tiles[3] = {4, 4, 4};  // scan group size for launches of 1 / 2+ query tiles (tuned at plan creation)
+  float tune_us[3][5] = {};     // measured scan microseconds [tiles][wavefronts]
   const int32_t* kofs_nw[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // chunk offsets per group size
   size_t lds_nw[5] = {0, 0, 0, 0, 0};
   size_t mfma_lds = 0;
@@ -410,6 +412,76 @@ std::vector<std::vector<int64_t>> build_waves(const HostGrid& G, const IqStep* s
     waves[w].push_back(i);
   }
   return waves;
+}
+
+// Scan autotuner: the MFMA scan's group size (2 or 4 wavefronts) is measured
+// on the plan itself.  A batch of the plan's own masked steps (32 queries,
+// then 64 when the slots allow) is prepared once; each group size runs one
+// warm-up and three timed launches; the faster is kept per tile count.  The
+// choice changes speed only: both sizes compute the same bounds.
+template <typename T>
+void tune_scan(iq_plan* pl) {
+  const int nwa[2] = {2, 4};
+  for (int t = 1; t <= 2; t++) pl->nw_tiles[t] = pl->lds_nw[4] ? 4 : 2;
+  if (!pl->mfma || pl->mfma_nw_forced || !pl->lds_nw[2] || !pl->lds_nw[4]) return;
+  const IqGeo& g = pl->geo;
+  hipStream_t s = pl->ctx->stream;
+  std::vector<IqQuery> hq;
+  const int64_t want = std::min<int64_t>(pl->nslots, 64);
+  for (const HostGrid& G : pl->grids) {
+    for (int64_t i = 0; i < G.npath && (int64_t)hq.size() < want; i++) {
+      const IqStep& st = pl->steps[G.step0 + i];
+      if (st.mask < 0 || st.hard || pl->naux > 0) continue;
+      IqQuery q{};
+      q.step = (int32_t)(G.step0 + i);
+      q.real = (int32_t)G.real0;
+      q.slot = (int32_t)hq.size();
+      q.multi = 0;
+      q.u = 0.5;
+      q.forced = -1;
+      hq.push_back(q);
+    }
+  }
+  if (hq.size() < 32) return;
+  const IqDev* D = pl->dd;
+  const IqQuery* dq = pl->mem.upload(hq.data(), hq.size(), s);
+  hipLaunchKernelGGL((k_fill<T>), dim3(1024), dim3(256), 0, s, (T*)pl->hd.sims, pl->nvox, (T)0);
+  HCL();
+  hipEvent_t a, b;
+  HC(hipEventCreate(&a));
+  HC(hipEventCreate(&b));
+  for (int ntile = 1; ntile <= 2; ntile++) {
+    const int nq = 32 * ntile;
+    if ((int)hq.size() < nq) break;
+    hipLaunchKernelGGL((k_query_prep<T>), dim3(nq, 1), dim3(256), 0, s, D, dq);
+    HCL();
+    hipLaunchKernelGGL(k_mfma_bprep, dim3(pl->hd.ktab.nstep, ntile), dim3(64), 0, s, D, dq, nq);
+    HCL();
+    float best = 1e30f;
+    for (int nw : nwa) {
+      dim3 grid(nblk(g.dist[0], 32), nblk(g.dist[1], 4 * nw), (unsigned)(g.dist[2] * ntile));
+      float tot = 0.0f;
+      for (int rep = 0; rep < 4; rep++) {
+        if (rep == 1) HC(hipEventRecord(a, s));
+        if (pl->mfma_split)
+          hipLaunchKernelGGL((k_scan_mfma<true>), grid, dim3(64 * nw), pl->lds_nw[nw], s, D, dq, nq, pl->mfma_gerr,
+                             pl->kofs_nw[nw]);
+        else
+          hipLaunchKernelGGL((k_scan_mfma<false>), grid, dim3(64 * nw), pl->lds_nw[nw], s, D, dq, nq, pl->mfma_gerr,
+                             pl->kofs_nw[nw]);
+        HCL();
+      }
+      HC(hipEventRecord(b, s));
+      HC(hipEventSynchronize(b));
+      HC(hipEventElapsedTime(&tot, a, b));
+      const float us = tot * 1e3f / 3.0f;
+      pl->tune_us[ntile][nw] = us;
+      if (us < best) { best = us; pl->nw_tiles[ntile] = nw; }
+    }
+  }
+  if ((int)hq.size() < 64) pl->nw_tiles[2] = pl->nw_tiles[1];
+  HC(hipEventDestroy(a));
+  HC(hipEventDestroy(b));
 }
 
 void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) {
@@ -926,6 +998,8 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
     HC(hipFuncSetAttribute((const void*)k_graphcut_big<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
     HC(hipFuncSetAttribute((const void*)k_graphcut_big<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
   }
+  if (pl->dtype == IQ_F32) tune_scan<float>(pl);
+  else tune_scan<double>(pl);
   HC(hipStreamSynchronize(s));
 }
 
@@ -999,10 +1073,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
         const int ntile = (nq + 31) / 32;
         hipLaunchKernelGGL(k_mfma_bprep, dim3(pl->hd.ktab.nstep, ntile), dim3(64), 0, s, D, q, nq);
         HCL();
-        int nw = pl->lds_nw[4] ? 4 : 2;
-        if (!pl->mfma_nw_forced && pl->lds_nw[2] &&
-            (double)nblk(g.dist[0], 32) * nblk(g.dist[1], 16) * g.dist[2] * ntile <= 4.0 * 256.0)
-          nw = 2;
+        const int nw = pl->mfma_nw_forced ? pl->mfma_nw : pl->nw_tiles[ntile > 1 ? 2 : 1];
         const int YB = 4 * nw;
         dim3 grid(nblk(g.dist[0], 32), nblk(g.dist[1], YB), (unsigned)(g.dist[2] * ntile));
         if (pl->prof) {
@@ -1713,6 +1784,17 @@ int iq_plan_last_time(iq_plan* plan, double* seconds) {
 int iq_plan_profile(iq_plan* plan, int enable) {
   if (!plan) return fail(nullptr, IQ_ERR_ARG, "null plan");
   plan->prof = enable != 0;
+  return IQ_OK;
+}
+
+int iq_plan_scan_tuning(iq_plan* plan, int nw[2], float us[4]) {
+  if (!plan || !nw || !us) return fail(nullptr, IQ_ERR_ARG, "null argument");
+  nw[0] = plan->mfma ? (plan->mfma_nw_forced ? plan->mfma_nw : plan->nw_tiles[1]) : 0;
+  nw[1] = plan->mfma ? (plan->mfma_nw_forced ? plan->mfma_nw : plan->nw_tiles[2]) : 0;
+  us[0] = plan->tune_us[1][2];
+  us[1] = plan->tune_us[1][4];
+  us[2] = plan->tune_us[2][2];
+  us[3] = plan->tune_us[2][4];
   return IQ_OK;
 }
 

@@ -296,6 +296,16 @@ class Plan:
                                                       t[2:].ctypes.data))
         return int(n[0]), float(t[0]), float(t[1]), float(t[2])
 
+    def scan_tuning(self):
+        """Wavefronts per MFMA-scan workgroup chosen at plan creation for
+        launches of one and of several 32-query tiles, and the measured
+        microseconds per launch {(tiles, wavefronts): us} (diagnostic)."""
+        nw = np.zeros(2, dtype=np.int32)
+        us = np.zeros(4, dtype=np.float32)
+        self.ctx.check(self.ctx._L.iq_plan_scan_tuning(self.handle, _ptr(nw), _ptr(us)))
+        meas = {(t, w): float(us[2 * (t - 1) + (0 if w == 2 else 1)]) for t in (1, 2) for w in (2, 4)}
+        return (int(nw[0]), int(nw[1])), {k: v for k, v in meas.items() if v > 0}
+
     def select_stats(self):
         """(plain queries, summed FP64-rechecked superset sizes, summed band sizes) of the last run."""
         v = np.zeros(3, dtype=np.int64)

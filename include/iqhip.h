@@ -189,6 +189,13 @@ int iq_plan_last_time(iq_plan* plan, double* seconds);
 int iq_plan_profile(iq_plan* plan, int enable);
 int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double* flop, double* mfma_flop);
 
+/* Scan autotuning result of iq_plan_create (no counterpart in the reference):
+ * wavefronts per MFMA-scan workgroup chosen for launches of one and of two or
+ * more 32-query tiles (nw[0], nw[1]; 0 when the plan has no MFMA scan), and
+ * the measured microseconds per launch us[tiles-1][0: 2 wavefronts, 1: 4]
+ * (0 where not measured).  Diagnostic only. */
+int iq_plan_scan_tuning(iq_plan* plan, int nw[2], float us[4]);
+
 /* Min-cut work of the last iq_plan_run (graphcut.jl:73 replacement):
  * push-relabel rounds, global-relabel BFS levels and cuts (register-resident
  * kernel; slabs beyond its size limit are not counted).  Diagnostic only. */

@@ -128,6 +128,26 @@ def test_sim_valu_scan_path(monkeypatch):
     _compare(T, (10, 10, 4), (40, 40, 12), overlap=(0.25, 0.25, 0.25))
 
 
+@pytest.mark.parametrize("nw", ["2", "4"])
+def test_sim_scan_group_sizes(monkeypatch, nw):
+    # the MFMA scan with 2- and with 4-wavefront groups (forced with
+    # IQHIP_SCAN_NW; the default is measured per plan) against the oracle
+    monkeypatch.setenv("IQHIP_SCAN_NW", nw)
+    rng = np.random.default_rng(20)
+    T = _smooth(rng, (44, 40, 14), 5, np.float32)
+    print("scan groups of", nw, "wavefronts", flush=True)
+    _compare(T, (10, 10, 4), (40, 40, 14), overlap=(0.25, 0.25, 0.25))
+
+
+def test_scan_tuning_reports_a_group_size():
+    rng = np.random.default_rng(21)
+    T = _smooth(rng, (40, 40, 12), 5, np.float32)
+    plan, _ = iqhip.make_plan(T, (10, 10, 4), (60, 60, 16), overlap=(0.25, 0.25, 0.25), nreal=2)
+    (n1, n2), us = plan.scan_tuning()
+    assert n1 in (2, 4) and n2 in (2, 4)
+    assert all(v > 0 for v in us.values())
+
+
 def test_sim_large_tied_band():
     # binary noise, random path (overlaps on several sides from unrelated
     # windows): bands of thousands of candidates over a few distinct integer
