@@ -944,9 +944,22 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   if (pl->any_multi) {
     // tiled exact maps when the candidate block's training-image region, the
     // query values and their region offsets fit in LDS
-    const size_t nreg = (size_t)(MT_X + g.tile[0] - 1) * (MT_Y + g.tile[1] - 1) * g.tile[2];
-    const size_t need = nreg * 8 + (size_t)g.tilevox * 12;
+    // (voxel lists as offsets into the staged region; read as scalar loads)
+    const int RX = MT_X + g.tile[0] - 1, RY = MT_Y + g.tile[1] - 1;
+    const size_t nreg = (size_t)RX * RY * g.tile[2];
+    const size_t need = nreg * 8;
     if (need <= 150 * 1024) {
+      auto region = [&](const std::vector<int32_t>& tv) {
+        std::vector<int32_t> r(tv.size());
+        for (size_t i = 0; i < tv.size(); i++) {
+          const int t = tv[i], x = t % g.tile[0], y = (t / g.tile[0]) % g.tile[1], z = t / (g.tile[0] * g.tile[1]);
+          r[i] = x + RX * (y + RY * z);
+        }
+        return r;
+      };
+      const std::vector<int32_t> rt = region(tvox), rh = region(htvox);
+      H.rtvox = pl->mem.upload(rt.data(), rt.size(), s);
+      H.rhtvox = rh.empty() ? nullptr : pl->mem.upload(rh.data(), rh.size(), s);
       pl->multi_lds = need;
       if (need > 64 * 1024)
         HC(hipFuncSetAttribute((const void*)k_multi_maps_tiled, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));

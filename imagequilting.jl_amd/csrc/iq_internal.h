@@ -145,6 +145,8 @@ struct IqDev {
   const int32_t* toff;   // matching training-image offsets
   const int32_t* htvox;  // hard voxels per step
   const int32_t* htoff;
+  const int32_t* rtvox;   // tvox entries as offsets into the tiled multi-map's staged region
+  const int32_t* rhtvox;  // htvox entries likewise
   int32_t full_off;
   const double* const* a2maps;
   const float* const* a2f;  // FP32 copies of a2maps, NaN where the window is disabled
@@ -203,6 +205,11 @@ __host__ __device__ inline uint32_t iq_f32_key(float x) {
 template <typename T>
 __device__ inline const __attribute__((address_space(1))) T* iq_glob(const T* p) {
   return (const __attribute__((address_space(1))) T*)p;
+}
+// constant-address-space view: uniform loads through it become scalar loads
+template <typename T>
+__device__ inline const __attribute__((address_space(4))) T* iq_const(const T* p) {
+  return (const __attribute__((address_space(4))) T*)p;
 }
 template <typename T>
 __device__ inline __attribute__((address_space(1))) T* iq_globw(T* p) {

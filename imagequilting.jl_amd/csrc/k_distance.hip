@@ -1007,11 +1007,12 @@ k_multi_maps(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 
 // Tiled form of k_multi_maps: a workgroup owns MT_X x MT_Y candidates at one
 // z; the training-image (or auxiliary-TI) region their windows cover is
-// staged in LDS as FP64, with the query values and the region offsets of the
-// source's voxels, so every work-item's chain (same ascending order, same
-// operations as k_multi_maps) reads LDS instead of gathering from HBM.
-// Grid: (x blocks * y blocks * dist z, query, source); LDS = region + 12 B per
-// source voxel (the host checks it fits).
+// staged in LDS as FP64, so every work-item's chain (same ascending order,
+// same operations as k_multi_maps) reads LDS instead of gathering from HBM.
+// The query values and the voxels' region offsets (prepared per plan) are
+// the same for every work-item and come through scalar loads.  Grid: (x
+// blocks * y blocks * dist z, query, source); LDS = the region (the host
+// checks it fits).
 #define MT_X 16
 #define MT_Y 16
 __global__ void __launch_bounds__(256)
@@ -1028,7 +1029,7 @@ k_multi_maps_tiled(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) 
   const int64_t slot = q.slot;
   double* out = D->maps + (slot * D->nmap + src) * g.P;
   int V;
-  const int32_t* tv;
+  const int32_t* rtv;
   const double* a2map = nullptr;
   const double* img64 = nullptr;
   if (src == 0) {
@@ -1041,17 +1042,15 @@ k_multi_maps_tiled(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) 
       return;
     }
     const IqMask& m = D->masks[st.mask];
-    V = m.V; tv = D->tvox + m.off; a2map = D->a2maps[m.a2];
+    V = m.V; rtv = D->rtvox + m.off; a2map = D->a2maps[m.a2];
   } else if (src == 1) {
-    V = st.hcnt; tv = D->htvox + st.hoff;
+    V = st.hcnt; rtv = D->rhtvox + st.hoff;
   } else {
-    V = g.tilevox; tv = D->tvox + D->full_off; a2map = D->a2soft[src - 2]; img64 = D->auxti[src - 2];
+    V = g.tilevox; rtv = D->rtvox + D->full_off; a2map = D->a2soft[src - 2]; img64 = D->auxti[src - 2];
   }
   const int RX = MT_X + g.tile[0] - 1, RY = MT_Y + g.tile[1] - 1, RZ = g.tile[2];
   const int nreg = RX * RY * RZ;
   double* reg = mreg;
-  double* kq = mreg + nreg;
-  int32_t* ro = (int32_t*)(kq + V);
   const int gx0 = bx * MT_X, gy0 = by * MT_Y;
   for (int i = threadIdx.x; i < nreg; i += blockDim.x) {
     const int x = i % RX, y = (i / RX) % RY, z = i / (RX * RY);
@@ -1063,13 +1062,10 @@ k_multi_maps_tiled(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) 
     }
     reg[i] = v;
   }
-  const double* kv = D->kv + slot * D->kv_stride + (int64_t)src * g.tilevox;
-  for (int j = threadIdx.x; j < V; j += blockDim.x) {
-    const int t = tv[j];
-    const int x = t % g.tile[0], y = (t / g.tile[0]) % g.tile[1], z = t / (g.tile[0] * g.tile[1]);
-    kq[j] = kv[j];
-    ro[j] = x + RX * (y + RY * z);
-  }
+  // query values and region offsets are the same for every work-item:
+  // constant-address loads (scalar unit, no LDS traffic)
+  const auto kq = iq_const(D->kv + slot * D->kv_stride + (int64_t)src * g.tilevox);
+  const auto ro = iq_const(rtv);
   __syncthreads();
   const int lx = threadIdx.x % MT_X, ly = threadIdx.x / MT_X;
   const int cx = gx0 + lx, cy = gy0 + ly;
