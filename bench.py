@@ -118,7 +118,7 @@ def main():
         copy_calibration(args.copy_mb)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU design is described, not built "
+        print("bench.py: runs on one GPU only; the multi-GPU layout is a design, not implemented "
               "(realizations shard across GPUs with no communication).", file=sys.stderr)
         sys.exit(2)
 

@@ -1272,7 +1272,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
     if (learned && b.plain) sel = (pl->lrn_check > 0 && nplain++ % pl->lrn_check == 0) ? SEL_CHECK : SEL_LEARNED;
     run_batch<T>(pl, qs.data() + b.off, pl->dq + b.off, b.n, b.plain, b.multi, sel, pl->dt + b.toff, b.nt);
     if (sel == SEL_CHECK) {
-      // the full scan judged this batch: stop using the shortlist once its
+      // the full scan decided this batch: stop using the shortlist once its
       // cumulative recall on the exact bands falls below the threshold
       unsigned long long st[4];
       HC(hipMemcpyAsync(st, H.lrn.stats, sizeof(st), hipMemcpyDeviceToHost, s));

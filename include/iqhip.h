@@ -36,7 +36,7 @@ typedef enum iq_status {
   IQ_ERR_NOCAND = -3,       /* every candidate pattern is disabled (iqsim.jl:237 / relaxation.jl:20-36) */
   IQ_ERR_NOTINDB = -4,      /* a forced pick is not in the pattern database */
   IQ_ERR_ITER = -5,         /* min-cut iteration guard reached */
-  IQ_ERR_UNSUPPORTED = -6,  /* size beyond what this build handles (message names the limit) */
+  IQ_ERR_UNSUPPORTED = -6,  /* size beyond what the library handles (message names the limit) */
   IQ_ERR_NOMEM = -7
 } iq_status;
 
