@@ -814,7 +814,9 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
             const int RX = 32 + kb.wxr - 1, RY = 4 * nw + kb.wyr - 1;
             for (int k = 2 * kb.s0; k < 2 * kb.s1; k++) {
               const IqChunk& c = chunks[k];
-              kofs[k] = ((c.jz - kb.z0) * RX * RY + (c.jy - kb.y0) * RX + (c.jx - kb.x0)) | (c.dir ? (1 << 30) : 0);
+              // x-runs: block layout x + RX (y + RY z); y-runs: y + RY (x + RX z)
+              kofs[k] = c.dir ? (((c.jz - kb.z0) * RX + (c.jx - kb.x0)) * RY + (c.jy - kb.y0)) | (1 << 30)
+                              : ((c.jz - kb.z0) * RY + (c.jy - kb.y0)) * RX + (c.jx - kb.x0);
             }
           }
           pl->kofs_nw[nw] = pl->mem.upload(kofs.data(), kofs.size(), s);

@@ -557,19 +557,27 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
     const int RX = 32 + kbx.wxr - 1, RY = 4 * nw + kbx.wyr - 1;
     const int n = RX * RY * kbx.wz;
     const int gx0 = cx0 + kbx.x0, gy0 = cy0 + kbx.y0, gz0 = cz + kbx.z0;
-    // element i = x + RX (y + RY z); coordinates advanced incrementally (NT
-    // per element, SU elements per pass with all their loads issued first);
-    // windows past the training image read as zero
+    // A box's chunks all run along x or all along y.  The block is staged so
+    // that a run is contiguous: x-runs in the layout x + RX (y + RY z),
+    // y-runs in y + RY (x + RX z); a wavefront's four rows then sit at a
+    // uniform stride (RX, or 1) and the eight words of a run at immediate
+    // offsets.  Element i = x + RX (y + RY z) of the walk; coordinates
+    // advance incrementally (NT per element, SU elements per pass with all
+    // their loads issued first); windows past the training image read as zero.
+    const auto kofs = iq_glob(kofs_tab);
+    const bool ydir = kbx.s0 < kbx.s1 && (kofs[2 * kbx.s0] >> 30) != 0;
     int x = (int)threadIdx.x % RX, y = ((int)threadIdx.x / RX) % RY, z = (int)threadIdx.x / (RX * RY);
     const int sx = NT % RX, sy = (NT / RX) % RY, sz = NT / (RX * RY);
     for (int i0 = threadIdx.x; i0 < n; i0 += NT * SU) {
       float v[SU];
       bool in[SU];
+      int at[SU];
 #pragma unroll
       for (int u = 0; u < SU; u++) {
         const int gx = gx0 + x, gy = gy0 + y, gz = gz0 + z;
         in[u] = gx < t0 && gy < t1 && gz < t2;
         v[u] = ti[in[u] ? gx + (int64_t)t0 * (gy + (int64_t)t1 * gz) : 0];
+        at[u] = ydir ? y + RY * (x + RX * z) : i0 + NT * u;
         x += sx;
         y += sy;
         z += sz;
@@ -578,13 +586,14 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
       }
 #pragma unroll
       for (int u = 0; u < SU; u++)
-        if (i0 + NT * u < n) lw[i0 + NT * u] = iq_split_word(in[u] ? v[u] : 0.0f);
+        if (i0 + NT * u < n) lw[at[u]] = iq_split_word(in[u] ? v[u] : 0.0f);
     }
     __syncthreads();
     // B fragments and chunk offsets are loaded one k-step ahead
     const auto bhp = iq_glob((const iq_u32x4*)(D->bh + (int64_t)tile * nstep * 64 + lane));
     const auto blp = iq_glob((const iq_u32x4*)(D->bl + (int64_t)tile * nstep * 64 + lane));
-    const auto kofs = iq_glob(kofs_tab);
+    const int lane_off = ydir ? 4 * w + RY * row : 4 * w * RX + row;
+    const int istr = ydir ? 1 : RX;
     iq_u32x4 bh_n = {0u, 0u, 0u, 0u}, bl_n = {0u, 0u, 0u, 0u};
     int ko_n = 0;
     if (kbx.s0 < kbx.s1) {
@@ -602,16 +611,17 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
         if (SPLIT) bl_n = blp[(int64_t)(step + 1) * 64];
         ko_n = kofs[2 * (step + 1) + kb];
       }
-      const int es = (ko >> 30) ? RX : 1;
-      const int base = (ko & 0x3fffffff) + 4 * w * RX + row;
+      const int base = (ko & 0x3fffffff) + lane_off;
       if (!wlive) continue;  // every candidate row of this wavefront is past the grid
       // all 32 LDS words of the four row blocks first, then the MFMAs in an
       // order where consecutive ones use different accumulators
       uint32_t wv[4][8];
 #pragma unroll
-      for (int i = 0; i < 4; i++)
+      for (int i = 0; i < 4; i++) {
+        const int rb = base + i * istr;
 #pragma unroll
-        for (int e = 0; e < 8; e++) wv[i][e] = lw[base + i * RX + e * es];
+        for (int e = 0; e < 8; e++) wv[i][e] = lw[rb + e];
+      }
       iq_bf16x8 ah[4], al[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
@@ -668,6 +678,10 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
       const float gs2 = (float)((gerr + g64) * (1.0 + 0x1p-10)) * 2.0f * (1.0f + 0x1p-10f);
       const float u32 = 0x1p-24f;
       const float tiny2 = 2.0f * (float)(1e3 * V * 0x1p-125);
+      // e = gs2 S + 2 (8 u32 ((A + B) + 2 S)) + tiny2, regrouped as
+      // S (gs2 + 32 u32) + 16 u32 (A + B) + tiny2 and evaluated by two fmas:
+      // fewer roundings than the plain sum, each within the margins above
+      const float k1 = gs2 + 32.0f * u32, k2 = 16.0f * u32;
       const auto lo_out = iq_globw(D->lo32 + slot * g.P);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
@@ -685,10 +699,10 @@ k_scan_mfma(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
         for (int v = 0; v < 16; v++) {
           const int ox = 8 * (v >> 2) + (v & 3);
           const float a = A[v];
-          const float d = __builtin_fabsf((a - 2.0f * acc[i][v]) + B);
+          const float d = __builtin_fabsf(__builtin_fmaf(-2.0f, acc[i][v], a) + B);  // 2 acc is exact
           const float S = __builtin_amdgcn_sqrtf(fmaxf(a, 0x1p-100f)) * sB;
-          const float e = gs2 * S + 2.0f * (8.0f * u32 * ((a + B) + 2.0f * S)) + tiny2;
-          float lo = d - e > 0.0f ? (d - e) * (1.0f - 0x1p-20f) : 0.0f;
+          const float e = __builtin_fmaf(S, k1, __builtin_fmaf(a + B, k2, tiny2));
+          float lo = fmaxf(d - e, 0.0f) * (1.0f - 0x1p-20f);
           float hi = (d + e) * (1.0f + 0x1p-20f);
           if (a != a) lo = hi = __builtin_inff();
           if (ox < xl) {
