@@ -1071,7 +1071,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     HCL();
     hipLaunchKernelGGL(k_short_select, dim3(16, nq), dim3(256), 0, s, D, q);
     HCL();
-    hipLaunchKernelGGL(k_band_recheck, dim3(nq, 64), dim3(256), 0, s, D, q);
+    hipLaunchKernelGGL(k_band_recheck, dim3(nq, RC_GROUPS), dim3(256), 0, s, D, q);
     HCL();
     hipLaunchKernelGGL(k_band_finalize, dim3(nq), dim3(FB), 0, s, D, q, 1);
     HCL();
@@ -1124,7 +1124,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
       }
       hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256 * SEL_PT), nq), dim3(256), 0, s, D, q);
       HCL();
-      hipLaunchKernelGGL(k_band_recheck, dim3(nq, 64), dim3(256), 0, s, D, q);
+      hipLaunchKernelGGL(k_band_recheck, dim3(nq, RC_GROUPS), dim3(256), 0, s, D, q);
       HCL();
     } else {
       hipLaunchKernelGGL(k_band_exact_all, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q);

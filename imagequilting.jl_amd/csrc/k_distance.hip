@@ -788,6 +788,7 @@ k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 #define RC_CHUNK 1024
 #define RC_SMALL 512
 #define RC_WCH 512
+#define RC_GROUPS 128  // member groups per query: 512 members in one pass, more loop
 __global__ void __launch_bounds__(256)
 k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   __shared__ double kl[RC_CHUNK];
