@@ -1067,7 +1067,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
   }
   if (has_plain && sel == SEL_LEARNED) {
     // shortlist -> FP32 bounds of its members -> band superset -> exact recheck
-    hipLaunchKernelGGL(k_short_bound, dim3(64, nq), dim3(256), (size_t)pl->lrn_vmax * sizeof(float), s, D, q);
+    hipLaunchKernelGGL(k_short_bound, dim3(64, nq), dim3(256), (size_t)pl->lrn_vmax * 8, s, D, q);
     HCL();
     hipLaunchKernelGGL(k_short_select, dim3(16, nq), dim3(256), 0, s, D, q);
     HCL();
@@ -1392,10 +1392,10 @@ void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
     H.lrn.slist = pl->mem.alloc<int32_t>((size_t)pl->nslots * g.P);
     H.lrn.nslist = pl->mem.alloc<int32_t>(pl->nslots);
     for (int m = 0; m < nm; m++) pl->lrn_vmax = std::max(pl->lrn_vmax, pl->masks[m].V);
-    if ((size_t)pl->lrn_vmax * sizeof(float) > 64 * 1024) {
-      HC(hipFuncSetAttribute((const void*)k_short_bound, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)(pl->lrn_vmax * sizeof(float))));
-    }
+    const size_t bound_lds = (size_t)pl->lrn_vmax * 8;  // query values and window offsets
+    if (bound_lds > 160 * 1024) bad(IQ_ERR_UNSUPPORTED, "learned proposal: overlap masks of more than 20,480 voxels");
+    if (bound_lds > 64 * 1024)
+      HC(hipFuncSetAttribute((const void*)k_short_bound, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bound_lds));
     pl->lrn_wofs = wofs;
     pl->lrn_bofs = bofs;
   }

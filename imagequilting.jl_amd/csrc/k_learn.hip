@@ -368,7 +368,7 @@ k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 // Grid: (groups, query), 4 wavefronts per group.
 __global__ void __launch_bounds__(256)
 k_short_bound(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
-  extern __shared__ float qv[];  // the query values, V floats
+  extern __shared__ float qv[];  // the query values (V floats), then the window offsets (V ints)
   const IqQuery q = qs[blockIdx.y];
   const IqStep& st = D->steps[q.step];
   if (!learn_applies(D, q, st)) return;
@@ -378,8 +378,15 @@ k_short_bound(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const int V = m.V;
   const int32_t* toff = D->toff + m.off;
   const double* kv = D->kv + slot * D->kv_stride;
-  for (int v = threadIdx.x; v < V; v += blockDim.x) qv[v] = (float)kv[v];
+  int32_t* tl = (int32_t*)(qv + V);
+  for (int v = threadIdx.x; v < V; v += blockDim.x) {
+    qv[v] = (float)kv[v];
+    tl[v] = toff[v];
+  }
   __syncthreads();
+  const bool f32 = D->ti_f32;
+  const auto ti32 = iq_glob((const float*)D->ti);
+  const auto ti64 = iq_glob((const double*)D->ti);
   const double* a2map = D->a2maps[m.a2];
   const double b2 = D->b2[slot * (2 + D->naux)];
   const double u32 = 0x1p-24, u64 = 0x1p-53;
@@ -395,10 +402,28 @@ k_short_bound(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     const int cx = (int)(c % g.dist[0]), cy = (int)((c / g.dist[0]) % g.dist[1]);
     const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
     const int64_t base = iq_lin3(g.ti, cx, cy, cz);
+    // each lane sums its strided voxels in ascending order, eight window
+    // values in flight at a time
     float acc = 0.0f;
-    for (int v = lane; v < V; v += 64) {
-      const float t = D->ti_f32 ? ((const float*)D->ti)[base + toff[v]] : (float)((const double*)D->ti)[base + toff[v]];
-      acc = __builtin_fmaf(t, qv[v], acc);
+    int v = lane;
+    if (f32) {
+      for (; v + 64 * 7 < V; v += 64 * 8) {
+        float t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = ti32[base + tl[v + 64 * u]];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = __builtin_fmaf(t[u], qv[v + 64 * u], acc);
+      }
+      for (; v < V; v += 64) acc = __builtin_fmaf(ti32[base + tl[v]], qv[v], acc);
+    } else {
+      for (; v + 64 * 7 < V; v += 64 * 8) {
+        float t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = (float)ti64[base + tl[v + 64 * u]];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = __builtin_fmaf(t[u], qv[v + 64 * u], acc);
+      }
+      for (; v < V; v += 64) acc = __builtin_fmaf((float)ti64[base + tl[v]], qv[v], acc);
     }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     const double a2 = a2map[c];
