@@ -264,16 +264,35 @@ k_learn_mscan(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const
         if (ca < g.P) v = *(const uint4*)(bank + ca * IQ_EMB + 16 * ks + 8 * kb);
         a[ks] = __builtin_bit_cast(iq_bf16x8, v);
       }
-      const double* a2m = D->a2maps[D->masks[cur].a2];
+      // A2 of the lane's 16 candidate rows and whether each is live: all 32
+      // loads in flight at once (the FP32 map's NaN marks a disabled window)
+      const int ma = D->masks[cur].a2;
+      const auto a2m = iq_glob(D->a2maps[ma]);
+      const auto a2f = iq_glob(D->a2f[ma]);
+      float fl[16];
+      double ad[16];
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const int64_t c = c0 + 8 * (v >> 2) + 4 * kb + (v & 3);
+        const int64_t cc = c < g.P ? c : g.P - 1;
+        fl[v] = a2f[cc];
+        ad[v] = a2m[cc];
+      }
       offm = 0;
 #pragma unroll
       for (int v = 0; v < 16; v++) {
         const int64_t c = c0 + 8 * (v >> 2) + 4 * kb + (v & 3);
-        const bool ok = c < g.P && !D->disabled[c];
-        a2[v] = ok ? a2m[c] : 0.0;
+        const bool ok = c < g.P && fl[v] == fl[v];
+        a2[v] = ok ? ad[v] : 0.0;
         offm |= ok ? 0u : (1u << v);
       }
     }
+    // the lane's query (slot, B2, bracket) is fetched before the MFMAs run
+    const bool qlive = (lane & 31) < tl.z;
+    const int64_t slot = qs[tl.y + (qlive ? (lane & 31) : 0)].slot;
+    const double b2 = D->b2[slot * (2 + D->naux)];
+    const uint32_t* br = D->lrn.brk + slot * 4;
+    const uint32_t lo = br[0], hi = br[1], shift = br[2];
     iq_f32x16 acc;
 #pragma unroll
     for (int v = 0; v < 16; v++) acc[v] = 0.0f;
@@ -281,11 +300,7 @@ k_learn_mscan(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const
 #pragma unroll
     for (int ks = 0; ks < IQ_EMB / 16; ks++)
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ks], __builtin_bit_cast(iq_bf16x8, bq[ks * 64]), acc, 0, 0, 0);
-    if ((lane & 31) >= tl.z) continue;
-    const int64_t slot = qs[tl.y + (lane & 31)].slot;
-    const double b2 = D->b2[slot * (2 + D->naux)];
-    const uint32_t* br = D->lrn.brk + slot * 4;
-    const uint32_t lo = br[0], hi = br[1], shift = br[2];
+    if (!qlive) continue;
     uint32_t keys[16];
     uint32_t in = 0;
 #pragma unroll
