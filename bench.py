@@ -147,11 +147,14 @@ def main():
     t0 = time.perf_counter()
     dev_time = 0.0
     launches, scan_sec, scan_flop, mfma_flop = 0, 0.0, 0.0, 0.0
+    cut_launches, cut_sec = 0, 0.0
     for i in range(args.steps):
         plan.run(U[args.warmup + i])  # returns after the stream has drained
         dev_time += plan.last_time()
         n, s, f, mf = plan.scan_stats()
         launches, scan_sec, scan_flop, mfma_flop = launches + n, scan_sec + s, scan_flop + f, mfma_flop + mf
+        cn, cs = plan.cut_time()
+        cut_launches, cut_sec = cut_launches + cn, cut_sec + cs
     lib.iq_ctx_synchronize(ctx.handle)
     elapsed = time.perf_counter() - t0
     cut_rounds, cut_levels, ncuts = plan.cut_stats()
@@ -183,7 +186,11 @@ def main():
             "device_ms_per_step": dev_time / args.steps * 1e3,
             "learned_proposal": "off",
             "mincut": {"cuts": ncuts, "push_rounds_per_cut": cut_rounds / max(ncuts, 1),
-                       "relabel_levels_per_cut": cut_levels / max(ncuts, 1)},
+                       "relabel_levels_per_cut": cut_levels / max(ncuts, 1),
+                       "avg_launch_us": cut_sec / max(cut_launches, 1) * 1e6,
+                       "share_of_device_time": cut_sec / max(dev_time, 1e-12),
+                       "bound": "latency (dependent LDS accesses and barriers per push-relabel round)"},
+            "scan_share_of_device_time": scan_sec / max(dev_time, 1e-12),
             "selection": {"queries": nsel, "superset_per_query": nsuper / max(nsel, 1),
                           "band_per_query": nband / max(nsel, 1)},
             "scan_tuning": {"wavefronts_per_group": {"1_tile": tune_nw[0], "2+_tiles": tune_nw[1]},

@@ -354,6 +354,9 @@ struct iq_plan {
   bool prof = false;
   std::vector<hipEvent_t> prof_ev;
   int64_t prof_used = 0;
+  std::vector<hipEvent_t> cut_ev;  // min-cut launches (profiling): pairs of events
+  int64_t cut_used = 0, cut_launches = 0;
+  double cut_sec = 0.0;
   double prof_flop = 0.0, prof_sec = 0.0, prof_mflop = 0.0;
   int64_t prof_launches = 0;
   std::vector<double> batch_flop;
@@ -1167,6 +1170,16 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     hipLaunchKernelGGL(k_relax_finalize, dim3(nq), dim3(FB), 0, s, D, q);
     HCL();
   }
+  if (pl->prof) {
+    if (pl->cut_used + 2 > (int64_t)pl->cut_ev.size()) {
+      hipEvent_t a, b;
+      HC(hipEventCreate(&a));
+      HC(hipEventCreate(&b));
+      pl->cut_ev.push_back(a);
+      pl->cut_ev.push_back(b);
+    }
+    HC(hipEventRecord(pl->cut_ev[pl->cut_used], s));
+  }
   if (pl->cut_wave)
     hipLaunchKernelGGL((k_graphcut_wave<T>), dim3(IQ_MAX_SLAB, nq), dim3(64), pl->cut_lds_max, s, D, q);
   else if (pl->cut_big)
@@ -1174,6 +1187,10 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
   else
     hipLaunchKernelGGL((k_graphcut<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB), pl->cut_lds_max, s, D, q);
   HCL();
+  if (pl->prof) {
+    HC(hipEventRecord(pl->cut_ev[pl->cut_used + 1], s));
+    pl->cut_used += 2;
+  }
   hipLaunchKernelGGL((k_paste<T>), dim3(nblk(g.tilevox, 256), nq), dim3(256), 0, s, D, q);
   HCL();
 }
@@ -1188,6 +1205,9 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   pl->prof_mflop = 0.0;
   pl->prof_sec = 0.0;
   pl->prof_launches = 0;
+  pl->cut_used = 0;
+  pl->cut_sec = 0.0;
+  pl->cut_launches = 0;
   pl->batch_flop.clear();
   // query table of every batch
   std::vector<IqQuery> qs;
@@ -1294,6 +1314,12 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
     HC(hipEventElapsedTime(&ms, pl->prof_ev[k], pl->prof_ev[k + 1]));
     pl->prof_sec += ms * 1e-3;
     pl->prof_launches++;
+  }
+  for (int64_t k = 0; k + 1 < pl->cut_used; k += 2) {
+    float ms = 0.0f;
+    HC(hipEventElapsedTime(&ms, pl->cut_ev[k], pl->cut_ev[k + 1]));
+    pl->cut_sec += ms * 1e-3;
+    pl->cut_launches++;
   }
   int32_t code = 0;
   HC(hipMemcpy(&code, H.err, sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -1753,6 +1779,7 @@ int iq_plan_destroy(iq_plan* plan) {
   if (!plan) return IQ_OK;
   if (plan->ctx) (void)hipSetDevice(plan->ctx->device);
   for (hipEvent_t e : plan->prof_ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : plan->cut_ev) (void)hipEventDestroy(e);
   if (plan->e0) (void)hipEventDestroy(plan->e0);
   if (plan->e1) (void)hipEventDestroy(plan->e1);
   plan->mem.release();
@@ -1810,6 +1837,13 @@ int iq_plan_scan_tuning(iq_plan* plan, int nw[2], float us[4]) {
   us[1] = plan->tune_us[1][4];
   us[2] = plan->tune_us[2][2];
   us[3] = plan->tune_us[2][4];
+  return IQ_OK;
+}
+
+int iq_plan_cut_time(iq_plan* plan, int64_t* launches, double* seconds) {
+  if (!plan) return fail(nullptr, IQ_ERR_ARG, "null plan");
+  if (launches) *launches = plan->cut_launches;
+  if (seconds) *seconds = plan->cut_sec;
   return IQ_OK;
 }
 

@@ -31,7 +31,7 @@ EXPORTS = (
     "iq_device_count", "iq_ctx_create", "iq_ctx_destroy", "iq_last_error", "iq_ctx_synchronize",
     "iq_imfilter", "iq_fastdistance", "iq_band_pick", "iq_relaxation", "iq_taumodel", "iq_sample",
     "iq_graphcut", "iq_plan_create", "iq_plan_destroy", "iq_plan_run", "iq_plan_results",
-    "iq_plan_create_multi", "iq_plan_last_time", "iq_plan_profile", "iq_plan_scan_stats", "iq_plan_scan_tuning", "iq_plan_cut_stats",
+    "iq_plan_create_multi", "iq_plan_last_time", "iq_plan_profile", "iq_plan_scan_stats", "iq_plan_scan_tuning", "iq_plan_cut_time", "iq_plan_cut_stats",
     "iq_simulate", "iq_prepare", "iq_prepare_soft", "iq_finddisabled",
     "iq_plan_learn", "iq_plan_set_proposal", "iq_plan_proposal_stats", "iq_plan_encoder", "iq_plan_select_stats",
 )
@@ -102,6 +102,7 @@ def load(path: str = LIB_PATH):
             "iq_plan_profile": [vp, c_int],
             "iq_plan_scan_stats": [vp, vp, vp, vp, vp],
             "iq_plan_scan_tuning": [vp, vp, vp],
+            "iq_plan_cut_time": [vp, vp, vp],
             "iq_plan_cut_stats": [vp, vp, vp, vp],
             "iq_simulate": [vp, vp, vp, vp, vp, vp, vp],
             "iq_prepare": [vp, c_int, vp, vp, vp],

@@ -306,6 +306,14 @@ class Plan:
         meas = {(t, w): float(us[2 * (t - 1) + (0 if w == 2 else 1)]) for t in (1, 2) for w in (2, 4)}
         return (int(nw[0]), int(nw[1])), {k: v for k, v in meas.items() if v > 0}
 
+    def cut_time(self):
+        """(launches, seconds) of the min-cut kernel during the last run
+        (profiling enabled)."""
+        n = np.zeros(1, dtype=np.int64)
+        t = np.zeros(1, dtype=np.float64)
+        self.ctx.check(self.ctx._L.iq_plan_cut_time(self.handle, _ptr(n), _ptr(t)))
+        return int(n[0]), float(t[0])
+
     def select_stats(self):
         """(plain queries, summed FP64-rechecked superset sizes, summed band sizes) of the last run."""
         v = np.zeros(3, dtype=np.int64)

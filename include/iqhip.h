@@ -189,6 +189,11 @@ int iq_plan_last_time(iq_plan* plan, double* seconds);
 int iq_plan_profile(iq_plan* plan, int enable);
 int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double* flop, double* mfma_flop);
 
+/* Min-cut launches of the last run and their summed device time (HIP events
+ * around each launch, recorded while profiling is enabled; no counterpart in
+ * the reference).  Diagnostic only. */
+int iq_plan_cut_time(iq_plan* plan, int64_t* launches, double* seconds);
+
 /* Scan autotuning result of iq_plan_create (no counterpart in the reference):
  * wavefronts per MFMA-scan workgroup chosen for launches of one and of two or
  * more 32-query tiles (nw[0], nw[1]; 0 when the plan has no MFMA scan), and
