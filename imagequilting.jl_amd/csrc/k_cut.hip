@@ -499,12 +499,17 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
         double ex = e[k];
         const int hu = h[u];
         if (!(ex > 0.0 && hu < HMAX)) continue;
-        double rbm[3], rfp[3];
+        // both directions of every incident edge are read up front: an edge
+        // has one pusher per round (its higher end), so nothing read here is
+        // written by another work-item in this phase
+        double rbm[3], rfm[3], rfp[3], rbp[3];
         int hm[3], hp[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           rbm[d] = S.rb[3 * vm[k][d] + d];
+          rfm[d] = S.rf[3 * vm[k][d] + d];
           rfp[d] = S.rf[3 * u + d];
+          rbp[d] = S.rb[3 * u + d];
           hm[d] = h[vm[k][d]];
           hp[d] = h[vp[k][d]];
         }
@@ -523,7 +528,7 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
               double dl = ex < r ? ex : r;
               ex -= dl;
               S.rb[3 * v + d] = r - dl;
-              S.rf[3 * v + d] += dl;
+              S.rf[3 * v + d] = rfm[d] + dl;
               prc[3 * v + d] = -dl;
             }
           }
@@ -533,7 +538,7 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
               double dl = ex < r ? ex : r;
               ex -= dl;
               S.rf[3 * u + d] = r - dl;
-              S.rb[3 * u + d] += dl;
+              S.rb[3 * u + d] = rbp[d] + dl;
               prc[3 * u + d] = dl;
             }
           }
