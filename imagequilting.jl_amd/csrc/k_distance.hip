@@ -176,16 +176,25 @@ k_query_prep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    // B2 chain in ascending voxel order; the next eight values are read from
+    // LDS while the current eight run through the fmas
     double s = 0.0;
     int j = 0;
     const int vl = V < PREP_LDS ? V : PREP_LDS;
+    double a[8];
+    if (vl >= 8) {
+#pragma unroll
+      for (int i = 0; i < 8; i++) a[i] = kl[i];
+    }
     for (; j + 8 <= vl; j += 8) {
-      double a0 = kl[j], a1 = kl[j + 1], a2 = kl[j + 2], a3 = kl[j + 3];
-      double a4 = kl[j + 4], a5 = kl[j + 5], a6 = kl[j + 6], a7 = kl[j + 7];
-      s = __builtin_fma(a0, a0, s); s = __builtin_fma(a1, a1, s);
-      s = __builtin_fma(a2, a2, s); s = __builtin_fma(a3, a3, s);
-      s = __builtin_fma(a4, a4, s); s = __builtin_fma(a5, a5, s);
-      s = __builtin_fma(a6, a6, s); s = __builtin_fma(a7, a7, s);
+      const int jn = j + 16 <= vl ? j + 8 : j;
+      double an[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) an[i] = kl[jn + i];
+#pragma unroll
+      for (int i = 0; i < 8; i++) s = __builtin_fma(a[i], a[i], s);
+#pragma unroll
+      for (int i = 0; i < 8; i++) a[i] = an[i];
     }
     for (; j < vl; j++) s = __builtin_fma(kl[j], kl[j], s);
     for (; j < V; j++) s = __builtin_fma(kv[j], kv[j], s);  // beyond the LDS stage
