@@ -1,7 +1,7 @@
 // iq_api.hip -- host driver and C ABI (include/iqhip.h) of libiqhip.
 //
 // One translation unit holds the host code and the kernels (k_*.hip are
-// included below); the radix sort lives in iq_sort.hip.
+// included below).
 #include "../../include/iqhip.h"
 #include "iq_internal.h"
 
@@ -19,14 +19,6 @@
 #include "k_select.hip"
 #include "k_cut.hip"
 #include "k_learn.hip"
-
-size_t iq_sort_temp_bytes(int n);
-size_t iq_segsort_temp_bytes(int n, int nseg);
-hipError_t iq_segsort_pairs(void* tmp, size_t tmp_bytes, const double* keys_in, double* keys_out,
-                            const int32_t* vals_in, int32_t* vals_out, int n, int nseg, const int64_t* begin,
-                            const int64_t* end, hipStream_t s);
-hipError_t iq_sort_pairs(void* tmp, size_t tmp_bytes, const double* keys_in, double* keys_out,
-                         const int32_t* vals_in, int32_t* vals_out, int n, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // errors
@@ -361,19 +353,6 @@ struct iq_plan {
   int64_t prof_launches = 0;
   std::vector<double> batch_flop;
   int64_t cut_stats[6] = {0, 0, 0, 0, 0, 0};  // push rounds, relabel BFS levels, cuts; plain queries, superset and band sizes
-  // stable sorts of the multi-source maps
-  void* sort_tmp = nullptr;
-  size_t sort_tmp_bytes = 0;
-  double* sort_keys = nullptr;
-  int32_t* sort_iota = nullptr;
-  // one segmented sort per batch (all maps of all multi-source queries)
-  bool segsort = false;
-  void* seg_tmp = nullptr;
-  size_t seg_tmp_bytes = 0;
-  double* seg_keys = nullptr;
-  int32_t* seg_iota = nullptr;
-  int64_t* seg_begin = nullptr;
-  int64_t* seg_end = nullptr;
 };
 
 namespace {
@@ -902,7 +881,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   size_t per_slot = (size_t)nmap * g.tilevox * 8 + (size_t)kmax * 4 + 64 + (size_t)P * (4 + 8 + 4 + 4 + 16) + nw64 * 8 +
                     (size_t)IQ_SORT_MAX * 12 + (size_t)(nw64 + 1) * 8 + (size_t)IQ_MAX_SLAB * g.tilevox +
                     (size_t)IQ_MAX_SLAB * cutper;
-  if (pl->any_multi) per_slot += (size_t)nmap * P * (12 + 12) + (size_t)nmap * nw32 * 4 + (size_t)P * 12;  // + segmented-sort keys and values
+  if (pl->any_multi) per_slot += (size_t)nmap * P * 8 + (size_t)nmap * nw32 * 4 + (size_t)P * 12;  // maps, bitmaps, events, values
   int64_t widest = 1;
   for (size_t k = 0; k < pl->nwaves; k++) {
     int64_t width = 0;  // lockstep wave k over every grid and realization
@@ -970,28 +949,9 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
         HC(hipFuncSetAttribute((const void*)k_multi_maps_tiled, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
     }
     H.maps = pl->mem.alloc<double>((size_t)NS * nmap * P);
-    H.sorted = pl->mem.alloc<int32_t>((size_t)NS * nmap * P);
     H.bits = pl->mem.alloc<uint32_t>((size_t)NS * nmap * nw32);
     H.ev = pl->mem.alloc<int32_t>((size_t)NS * P);
     H.vals = pl->mem.alloc<double>((size_t)NS * P);
-    pl->sort_keys = pl->mem.alloc<double>(P);
-    pl->sort_iota = pl->mem.alloc<int32_t>(P);
-    hipLaunchKernelGGL(k_iota32, dim3(nblk(P, 256)), dim3(256), 0, s, pl->sort_iota, P);
-    HCL();
-    pl->sort_tmp_bytes = iq_sort_temp_bytes((int)P);
-    pl->sort_tmp = pl->mem.alloc<char>(pl->sort_tmp_bytes);
-    const int64_t nall = NS * nmap * P;
-    if (nall < (1ll << 31)) {
-      pl->segsort = true;
-      pl->seg_keys = pl->mem.alloc<double>((size_t)nall);
-      pl->seg_iota = pl->mem.alloc<int32_t>((size_t)nall);
-      hipLaunchKernelGGL(k_iota_mod, dim3(nblk(nall, 256)), dim3(256), 0, s, pl->seg_iota, nall, P);
-      HCL();
-      pl->seg_begin = pl->mem.alloc<int64_t>((size_t)NS * nmap);
-      pl->seg_end = pl->mem.alloc<int64_t>((size_t)NS * nmap);
-      pl->seg_tmp_bytes = iq_segsort_temp_bytes((int)nall, (int)(NS * nmap));
-      pl->seg_tmp = pl->mem.alloc<char>(pl->seg_tmp_bytes);
-    }
   }
   if (pl->mfma) {
     pl->mfma_ntile_max = (NS + 31) / 32;
@@ -1148,25 +1108,6 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
       hipLaunchKernelGGL(k_multi_maps, dim3(nblk(P, 256), nq, 2 + pl->naux), dim3(256), 0, s, D, q);
     }
     HCL();
-    const IqDev& H = pl->hd;
-    if (pl->segsort) {
-      // segments (query i, map k) = [(slot nmap + k) P, + P), empty when unused
-      hipLaunchKernelGGL(k_seg_offsets, dim3(nblk((int64_t)nq * H.nmap, 256)), dim3(256), 0, s, D, q, nq, pl->seg_begin,
-                         pl->seg_end);
-      HCL();
-      HC(iq_segsort_pairs(pl->seg_tmp, pl->seg_tmp_bytes, H.maps, pl->seg_keys, pl->seg_iota, H.sorted,
-                          (int)((int64_t)pl->nslots * H.nmap * P), nq * H.nmap, pl->seg_begin, pl->seg_end, s));
-    }
-    for (int i = 0; i < nq && !pl->segsort; i++) {
-      if (!hq[i].multi) continue;
-      const IqStep& st = pl->steps[hq[i].step];
-      for (int k = 0; k < H.nmap; k++) {
-        if (k == 1 && !st.hard) continue;
-        const int64_t m = (int64_t)hq[i].slot * H.nmap + k;
-        HC(iq_sort_pairs(pl->sort_tmp, pl->sort_tmp_bytes, H.maps + m * P, pl->sort_keys, pl->sort_iota,
-                         H.sorted + m * P, (int)P, s));
-      }
-    }
     hipLaunchKernelGGL(k_relax_finalize, dim3(nq), dim3(FB), 0, s, D, q);
     HCL();
   }
@@ -1613,30 +1554,18 @@ int iq_relaxation(iq_ctx* ctx, const double* D, const double* const* Ds, int nau
     DMem m;
     int nsrc = naux + 1;
     std::vector<const double*> maps;
-    std::vector<const int32_t*> sorted;
-    double* keys = m.alloc<double>(n);
-    int32_t* iota = m.alloc<int32_t>(n);
-    hipLaunchKernelGGL(k_iota32, dim3(nblk(n, 256)), dim3(256), 0, s, iota, n);
-    HCL();
-    size_t tb = iq_sort_temp_bytes((int)n);
-    void* tmp = m.alloc<char>(tb);
     for (int j = 0; j < nsrc; j++) {
       const double* src = j == 0 ? D : Ds[j - 1];
       if (!src) bad(IQ_ERR_ARG, "null auxiliary map");
-      double* dm = m.upload(src, n, s);
-      int32_t* so = m.alloc<int32_t>(n);
-      HC(iq_sort_pairs(tmp, tb, dm, keys, iota, so, (int)n, s));
-      maps.push_back(dm);
-      sorted.push_back(so);
+      maps.push_back(m.upload(src, n, s));
     }
     const double** dmaps = (const double**)m.upload(maps.data(), maps.size(), s);
-    const int32_t** dsorted = (const int32_t**)m.upload(sorted.data(), sorted.size(), s);
     uint32_t* bits = m.alloc<uint32_t>((size_t)((n + 31) / 32) * nsrc);
     int32_t* ev = m.alloc<int32_t>(n);
     int64_t* cnt = m.alloc<int64_t>(1);
     int32_t* err = m.alloc<int32_t>(1);
     HC(hipMemsetAsync(err, 0, sizeof(int32_t), s));
-    hipLaunchKernelGGL(k_relax_host, dim3(1), dim3(FB), 0, s, n, nsrc, dmaps, dsorted, tol, bits, ev, cnt, err);
+    hipLaunchKernelGGL(k_relax_host, dim3(1), dim3(FB), 0, s, n, nsrc, dmaps, tol, bits, ev, cnt, err);
     HCL();
     int32_t code = 0;
     int64_t c = 0;

@@ -180,7 +180,6 @@ struct IqDev {
   char* cutscratch;  int64_t cutscratch_per;
   // multi-source selection (relaxation.jl / taumodel.jl), per slot
   double* maps;      int32_t nmap;         // nmap maps of P: 0 overlap, 1 hard, 2+a soft
-  int32_t* sorted;                         // stable ascending order of each map
   uint32_t* bits;    int64_t nw32;         // nmap bitmaps of P bits
   int32_t* ev;                             // pattern database (P)
   double* vals;                            // P

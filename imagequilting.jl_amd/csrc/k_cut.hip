@@ -778,30 +778,3 @@ template __global__ void k_fill<double>(double*, int64_t, double);
 template __global__ void k_fill<uint8_t>(uint8_t*, int64_t, uint8_t);
 template __global__ void k_fill<int64_t>(int64_t*, int64_t, int64_t);
 template __global__ void k_fill<int32_t>(int32_t*, int64_t, int32_t);
-
-// 0, 1, 2, ... (values of the stable sorts)
-extern "C" __global__ void k_iota32(int32_t* __restrict__ v, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    v[i] = (int32_t)i;
-}
-
-// i mod P: candidate indices of every map segment of a batch
-extern "C" __global__ void k_iota_mod(int32_t* __restrict__ v, int64_t n, int64_t P) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    v[i] = (int32_t)(i % P);
-}
-
-// Segment bounds of the batch's segmented sort: segment (i, k) is map k of
-// query i, [(slot * nmap + k) * P, + P); unused maps (single-source queries,
-// the hard map of a tile without hard data) are empty.
-extern "C" __global__ void k_seg_offsets(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
-                                         int64_t* __restrict__ begin, int64_t* __restrict__ end) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nq * D->nmap) return;
-  const int i = (int)(t / D->nmap), k = (int)(t % D->nmap);
-  const IqQuery q = qs[i];
-  const int64_t b = ((int64_t)q.slot * D->nmap + k) * D->g.P;
-  const bool used = q.multi && (k != 1 || D->steps[q.step].hard);
-  begin[t] = b;
-  end[t] = used ? b + D->g.P : b;
-}

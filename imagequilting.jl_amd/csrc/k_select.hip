@@ -17,6 +17,7 @@ struct SelScratch {
   double f64[NW + 4];
   int64_t result;
   int64_t n;
+  int64_t aux;
   int32_t flag;
 };
 
@@ -572,37 +573,218 @@ k_band_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int
 }
 
 // ---------------------------------------------------------------------------
-// Relaxation (relaxation.jl:5-39): pattern database from stable orders.
-// maps[0] is the primary map D, maps[1..nsrc) the auxiliary maps Ds;
-// sorted[j] the stable ascending order of maps[j] (ties by index, Julia's Perm
-// ordering).  top-k sets are prefixes of those orders; the running
-// intersection is kept as bitmaps (bits: nsrc maps of nwd words).
-// Returns the size of the database written to ev (ascending), -1 if every
-// candidate is disabled.
+// Relaxation (relaxation.jl:5-39) by order statistics.
+// maps[0] is the primary map D, maps[1..nsrc) the auxiliary maps Ds.  Every
+// set the loop builds is a prefix of a map's stable ascending order (value,
+// then index: Julia's Perm ordering for partialsortperm, relaxation.jl:12,27),
+// and a prefix of length m is { i : (bits(v_i), i) <= the m-th smallest
+// pair }: non-negative doubles and +inf order like their IEEE bit patterns.
+// So each set needs one order statistic (radix selection, blk_kth_key) and
+// one pass that writes it as a bitmap (bits: nsrc maps of nwd words); the
+// running intersection is the AND of the bitmaps.
 // ---------------------------------------------------------------------------
-__device__ int64_t relax_db(int64_t P, int nsrc, const double* const* maps, const int32_t* const* sorted, double tol,
-                            uint32_t* bits, int32_t* ev, SelScratch& sh) {
-  __shared__ int64_t lenj[IQ_MAX_AUX + 2];
-  const int64_t nwd = (P + 31) / 32;
-  const double* Dp = maps[0];
-  // enabled patterns = finite entries = prefix of the ascending order (+inf sorts last)
-  if (threadIdx.x == 0) {
-    int64_t lo = 0, hi = P;
-    while (lo < hi) { int64_t mid = (lo + hi) >> 1; if (!__builtin_isinf(Dp[sorted[0][mid]])) lo = mid + 1; else hi = mid; }
-    sh.n = lo;
+#define KS_CAP 512          // bins this small are ranked pairwise in LDS
+#define KS_LDS_WORDS 1792   // 64-bit LDS words: histogram 2048 x u32, KS_CAP values, KS_CAP indices
+
+__device__ inline uint64_t f64_bits(double x) { return (uint64_t)__double_as_longlong(x); }
+
+// Bin of the running 1-based rank r over nb histogram counts; r becomes the
+// rank inside that bin and cnt its count.
+__device__ __forceinline__ int blk_find_bin(const uint32_t* hist, int nb, int64_t& r, int64_t& cnt, SelScratch& sh) {
+  const int per = (nb + FB - 1) / FB;
+  const int b0 = threadIdx.x * per;
+  int64_t c = 0;
+  for (int k = 0; k < per; k++)
+    if (b0 + k < nb) c += hist[b0 + k];
+  const int64_t ex = blk_excl_scan(c, nullptr, sh);
+  if (ex < r && r <= ex + c) {
+    int64_t acc = ex;
+    for (int k = 0; k < per; k++) {
+      const int64_t h = hist[b0 + k];
+      if (r <= acc + h) { sh.result = b0 + k; sh.n = r - acc; sh.aux = h; break; }
+      acc += h;
+    }
   }
   __syncthreads();
-  const int64_t nen = sh.n;
+  const int bin = (int)sh.result;
+  r = sh.n;
+  cnt = sh.aux;
   __syncthreads();
+  return bin;
+}
+
+// The m-th smallest (1 <= m <= P) pair (bits(map[i]), i) over i in [0, P).
+// Radix digits of 11 bits over the value bits, each a histogram pass over
+// the map restricted to the prefix fixed so far, until the bin holding rank
+// m has at most KS_CAP members; those are compacted into LDS and ranked
+// pairwise.  When the whole value is fixed with more than KS_CAP ties, the
+// m-th is found by index order directly (a counting pass per wavefront
+// segment, then a walk of the one segment that holds it).
+__device__ __forceinline__ void blk_kth_key(const double* __restrict__ map, int64_t P, int64_t m, uint64_t* lds, SelScratch& sh,
+                            uint64_t* tv, int64_t* ti) {
+  uint32_t* hist = (uint32_t*)lds;
+  uint64_t* cv = lds + 1024;
+  uint32_t* ci = (uint32_t*)(lds + 1024 + KS_CAP);
+  const auto* mg = iq_glob(map);
+  const int lane = threadIdx.x & 63;
+  uint64_t pv = 0, hm = 0;  // prefix value and mask of the fixed top bits
+  int known = 0;
+  int64_t r = m, cnt = P;
+  while (known < 64 && cnt > KS_CAP) {
+    const int w = 64 - known >= 11 ? 11 : 64 - known;
+    const int sft = 64 - known - w;
+    const uint32_t dm = (1u << w) - 1u;
+    for (int b = threadIdx.x; b < (1 << w); b += FB) hist[b] = 0u;
+    __syncthreads();
+    for (int64_t i0 = threadIdx.x; i0 < P; i0 += 8 * FB) {
+      uint64_t v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = i0 + k * FB < P ? f64_bits(mg[i0 + k * FB]) : 0ull;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const bool mt = i0 + k * FB < P && (v[k] & hm) == pv;
+        const uint32_t d = (uint32_t)(v[k] >> sft) & dm;
+        // lanes sharing the first matching lane's digit add once, through it
+        const uint64_t act = __ballot(mt);
+        if (act) {
+          const int lead = __ffsll((unsigned long long)act) - 1;
+          const uint32_t ld = __shfl(d, lead);
+          const uint64_t same = __ballot(mt && d == ld);
+          if (lane == lead) atomicAdd(&hist[ld], (uint32_t)__popcll(same));
+          if (mt && d != ld) atomicAdd(&hist[d], 1u);
+        }
+      }
+    }
+    __syncthreads();
+    const int bin = blk_find_bin(hist, 1 << w, r, cnt, sh);
+    pv |= (uint64_t)bin << sft;
+    hm |= (uint64_t)dm << sft;
+    known += w;
+  }
+  if (cnt > KS_CAP) {
+    // value fixed (pv), rank r among its ties in index order
+    const int wv = threadIdx.x >> 6;
+    const int64_t seg = (P + NW * 64 - 1) / (NW * 64) * 64;
+    const int64_t a = wv * seg, e = a + seg < P ? a + seg : P;
+    int64_t c = 0;
+    for (int64_t base = a; base < e; base += 64) {
+      const int64_t i = base + lane;
+      c += __popcll(__ballot(i < e && f64_bits(mg[i]) == pv));
+    }
+    const int64_t ex = blk_excl_scan(lane == 0 ? c : 0, nullptr, sh);
+    const int64_t pre = __shfl(ex, 0);
+    if (pre < r && r <= pre + c) {
+      int64_t need = r - pre;
+      for (int64_t base = a; base < e; base += 64) {
+        const int64_t i = base + lane;
+        uint64_t b = __ballot(i < e && f64_bits(mg[i]) == pv);
+        const int k = __popcll(b);
+        if (need <= k) {
+          for (int64_t t = 1; t < need; t++) b &= b - 1;
+          if (lane == 0) sh.result = base + __ffsll((unsigned long long)b) - 1;
+          break;
+        }
+        need -= k;
+      }
+    }
+    __syncthreads();
+    *tv = pv;
+    *ti = sh.result;
+    __syncthreads();
+    return;
+  }
+  // compact the bin's members into LDS (any order), then rank them pairwise
+  if (threadIdx.x == 0) sh.flag = 0;
+  __syncthreads();
+  for (int64_t i0 = threadIdx.x; i0 < P; i0 += 8 * FB) {
+    uint64_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = i0 + k * FB < P ? f64_bits(mg[i0 + k * FB]) : 0ull;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const bool mt = i0 + k * FB < P && (v[k] & hm) == pv;
+      const uint64_t b = __ballot(mt);
+      if (b) {
+        const int lead = __ffsll((unsigned long long)b) - 1;
+        int base = 0;
+        if (lane == lead) base = atomicAdd(&sh.flag, __popcll(b));
+        base = __shfl(base, lead);
+        if (mt) {
+          const int slot = base + __popcll(b & ((1ull << lane) - 1ull));
+          cv[slot] = v[k];
+          ci[slot] = (uint32_t)(i0 + k * FB);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int n = sh.flag;
+  if ((int)threadIdx.x < n) {
+    const uint64_t myv = cv[threadIdx.x];
+    const uint32_t myi = ci[threadIdx.x];
+    int rank = 0;
+    for (int f = 0; f < n; f++) {
+      const uint64_t fv = cv[f];
+      rank += (fv < myv || (fv == myv && ci[f] < myi)) ? 1 : 0;
+    }
+    if (rank == r - 1) { sh.result = myi; sh.aux = (int64_t)myv; }
+  }
+  __syncthreads();
+  *tv = (uint64_t)sh.aux;
+  *ti = sh.result;
+  __syncthreads();
+}
+
+// bits[w] for the prefix set {i : (bits(map[i]), i) <= (tv, ti)}: one
+// wavefront per 64 consecutive candidates, its ballot is two words.
+__device__ __forceinline__ void blk_prefix_bits(const double* __restrict__ map, int64_t P, uint64_t tv, int64_t ti,
+                                uint32_t* __restrict__ bits) {
+  const auto* mg = iq_glob(map);
+  const int lane = threadIdx.x & 63;
+  const int64_t nwd = (P + 31) / 32;
+  for (int64_t base = (int64_t)(threadIdx.x >> 6) * 64; base < P; base += FB) {
+    const int64_t i = base + lane;
+    bool in = false;
+    if (i < P) {
+      const uint64_t v = f64_bits(mg[i]);
+      in = v < tv || (v == tv && i <= ti);
+    }
+    const uint64_t b = __ballot(in);
+    if (lane == 0) {
+      const int64_t w = base >> 5;
+      bits[w] = (uint32_t)b;
+      if (w + 1 < nwd) bits[w + 1] = (uint32_t)(b >> 32);
+    }
+  }
+}
+
+// Returns the size of the database written to ev (ascending), -1 if every
+// candidate is disabled.  lds: KS_LDS_WORDS 64-bit words.
+__device__ __forceinline__ int64_t relax_db(int64_t P, int nsrc, const double* const* maps, double tol, uint32_t* bits, int32_t* ev,
+                            uint64_t* lds, SelScratch& sh) {
+  __shared__ int64_t lenj[IQ_MAX_AUX + 2];
+  const int64_t nwd = (P + 31) / 32;
+  // enabled patterns = finite entries of D; all zero when none is finite and nonzero (relaxation.jl:7-11)
+  const auto* dg = iq_glob(maps[0]);
+  int64_t cf = 0, cz = 0;
+  for (int64_t i = threadIdx.x; i < P; i += FB) {
+    const double x = dg[i];
+    const bool fin = !__builtin_isinf(x);
+    cf += fin;
+    cz += fin && x != 0.0;
+  }
+  const int64_t nen = blk_sum_i64(cf, sh);
+  const int64_t nnz = blk_sum_i64(cz, sh);
   if (nen == 0) return -1;  // relaxation.jl would never terminate
-  const bool allzero = Dp[sorted[0][nen - 1]] == 0.0;  // largest finite value is 0 (relaxation.jl:11)
+  const bool allzero = nnz == 0;
   const int64_t dbsize = allzero ? nen : (int64_t)ceil(tol * (double)nen);
-  for (int64_t i = threadIdx.x; i < nwd * nsrc; i += FB) bits[i] = 0u;
+  if (dbsize <= 0) return -1;  // tol outside (0, 1]: the loop would never end
   if (threadIdx.x < IQ_MAX_AUX + 2) lenj[threadIdx.x] = 0;
-  __syncthreads();
-  for (int64_t i = threadIdx.x; i < dbsize; i += FB) {
-    int32_t c = sorted[0][i];
-    atomicOr(&bits[c >> 5], 1u << (c & 31));
+  {
+    uint64_t tv;
+    int64_t ti;
+    blk_kth_key(maps[0], P, dbsize, lds, sh, &tv, &ti);
+    blk_prefix_bits(maps[0], P, tv, ti, bits);
   }
   __syncthreads();
   double frac = 0.1 * ((double)dbsize / (double)nen);
@@ -611,13 +793,15 @@ __device__ int64_t relax_db(int64_t P, int nsrc, const double* const* maps, cons
     const int64_t softdbsize = (int64_t)ceil(frac * (double)nen);
     cnt = dbsize;
     for (int j = 1; j < nsrc; j++) {
-      const int64_t l0 = lenj[j];
-      for (int64_t i = l0 + threadIdx.x; i < softdbsize; i += FB) {
-        int32_t c = sorted[j][i];
-        atomicOr(&bits[j * nwd + (c >> 5)], 1u << (c & 31));
+      // the prefix only grows (frac does); an unchanged length keeps its bitmap
+      if (softdbsize != lenj[j]) {
+        uint64_t tv = 0;
+        int64_t ti = -1;  // empty prefix
+        if (softdbsize > 0) blk_kth_key(maps[j], P, softdbsize < P ? softdbsize : P, lds, sh, &tv, &ti);
+        blk_prefix_bits(maps[j], P, tv, ti, bits + j * nwd);
+        __syncthreads();
+        if (threadIdx.x == 0) lenj[j] = softdbsize;
       }
-      __syncthreads();
-      if (threadIdx.x == 0 && softdbsize > l0) lenj[j] = softdbsize;
       int64_t lc = 0;
       for (int64_t w = threadIdx.x; w < nwd; w += FB) {
         uint32_t x = bits[w];
@@ -663,7 +847,6 @@ k_relax_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   __shared__ int32_t dp[IQ_SORT_MAX];
   __shared__ double s2[S2MAX];
   __shared__ const double* mp[IQ_MAX_AUX + 2];
-  __shared__ const int32_t* sp[IQ_MAX_AUX + 2];
   const IqQuery q = qs[blockIdx.x];
   if (!q.multi) return;
   const IqGeo& g = D->g;
@@ -674,12 +857,10 @@ k_relax_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   int nsrc = 0;
   if (threadIdx.x == 0) {
     int k = 0;
-    if (st.hard) { mp[k] = D->maps + (slot * D->nmap + 1) * g.P; sp[k] = D->sorted + (slot * D->nmap + 1) * g.P; k++; }
-    mp[k] = D->maps + (slot * D->nmap + 0) * g.P; sp[k] = D->sorted + (slot * D->nmap + 0) * g.P; k++;
+    if (st.hard) mp[k++] = D->maps + (slot * D->nmap + 1) * g.P;
+    mp[k++] = D->maps + (slot * D->nmap + 0) * g.P;
     for (int a = 0; a < D->naux; a++) {
-      mp[k] = D->maps + (slot * D->nmap + 2 + a) * g.P;
-      sp[k] = D->sorted + (slot * D->nmap + 2 + a) * g.P;
-      k++;
+      mp[k++] = D->maps + (slot * D->nmap + 2 + a) * g.P;
     }
     sh.flag = k;
   }
@@ -687,7 +868,7 @@ k_relax_finalize(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   nsrc = sh.flag;
   __syncthreads();
   int32_t* ev = D->ev + slot * g.P;
-  const int64_t nev = relax_db(g.P, nsrc, mp, sp, D->tol, D->bits + slot * D->nmap * D->nw32, ev, sh);
+  const int64_t nev = relax_db(g.P, nsrc, mp, D->tol, D->bits + slot * D->nmap * D->nw32, ev, (uint64_t*)sv, sh);
   if (nev < 0) {
     if (threadIdx.x == 0) { set_err(D->err, IQ_KERR_NOCAND); *o.pick = -2; }
     return;
@@ -731,11 +912,11 @@ k_band_pick_host(const double* __restrict__ Dm, int64_t n, double tol, int32_t* 
 }
 
 extern "C" __global__ void __launch_bounds__(FB)
-k_relax_host(int64_t P, int nsrc, const double* const* __restrict__ maps, const int32_t* const* __restrict__ sorted,
-             double tol, uint32_t* __restrict__ bits, int32_t* __restrict__ ev, int64_t* __restrict__ count,
-             int32_t* __restrict__ err) {
+k_relax_host(int64_t P, int nsrc, const double* const* __restrict__ maps, double tol, uint32_t* __restrict__ bits,
+             int32_t* __restrict__ ev, int64_t* __restrict__ count, int32_t* __restrict__ err) {
   __shared__ SelScratch sh;
-  const int64_t n = relax_db(P, nsrc, maps, sorted, tol, bits, ev, sh);
+  __shared__ uint64_t ks[KS_LDS_WORDS];
+  const int64_t n = relax_db(P, nsrc, maps, tol, bits, ev, ks, sh);
   if (threadIdx.x == 0) {
     if (n < 0) set_err(err, IQ_KERR_NOCAND);
     *count = n < 0 ? 0 : n;

@@ -85,6 +85,35 @@ def test_relaxation_matches_oracle(ctx):
         assert np.array_equal(got, ref), trial
 
 
+@pytest.mark.parametrize("kind", ["continuous", "ties", "allzero", "mixed_inf"])
+def test_relaxation_large_matches_oracle(ctx, kind):
+    """Pattern databases of 300k candidates: each prefix set comes from an
+    order statistic over the whole map (histogram passes, then a bin ranked
+    in LDS, or the tied value's members in index order)."""
+    rng = np.random.default_rng({"continuous": 1, "ties": 2, "allzero": 3, "mixed_inf": 4}[kind])
+    n = 300_001  # not a multiple of 64
+    if kind == "continuous":
+        D = rng.random(n) * 1e3
+        Ds = [rng.random(n) * 50.0, rng.gamma(2.0, 3.0, n)]
+    elif kind == "ties":
+        D = np.round(rng.random(n) * 2)
+        Ds = [np.round(rng.random(n) * 3), np.zeros(n)]
+    elif kind == "allzero":
+        D = np.zeros(n)
+        Ds = [rng.random(n)]
+    else:
+        D = rng.random(n) * 10
+        Ds = [rng.random(n) * 10]
+        dis = rng.random(n) < 0.3
+        D[dis] = np.inf
+        Ds[0][dis | (rng.random(n) < 0.05)] = np.inf
+        D[rng.random(n) < 0.2] = 0.0
+    for tol in (0.1, 0.01):
+        got = iqhip.relaxation(D, Ds, tol, ctx=ctx)
+        ref = iqref.relaxation(D, Ds, tol)
+        assert np.array_equal(got, ref), (kind, tol, len(got), len(ref))
+
+
 # ---------------------------------------------------------------- taumodel
 def test_taumodel_matches_oracle_bitwise(ctx):
     rng = np.random.default_rng(8)
