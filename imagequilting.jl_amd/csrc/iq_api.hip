@@ -398,7 +398,7 @@ std::vector<std::vector<int64_t>> build_waves(const HostGrid& G, const IqStep* s
 
 // Scan autotuner: the MFMA scan's group size (2 or 4 wavefronts) is measured
 // on the plan itself.  A batch of the plan's own masked steps (32 queries,
-// then 64 when the slots allow) is prepared once; each group size runs one
+// then up to 64 as the slots allow) is prepared once; each group size runs one
 // warm-up and three timed launches; the faster is kept per tile count.  The
 // choice changes speed only: both sizes compute the same bounds.
 template <typename T>
@@ -433,8 +433,8 @@ void tune_scan(iq_plan* pl) {
   HC(hipEventCreate(&a));
   HC(hipEventCreate(&b));
   for (int ntile = 1; ntile <= 2; ntile++) {
-    const int nq = 32 * ntile;
-    if ((int)hq.size() < nq) break;
+    const int nq = ntile == 1 ? 32 : (int)hq.size();  // two tiles: every prepared query (33 to 64)
+    if (ntile == 2 && nq <= 32) break;
     hipLaunchKernelGGL((k_query_prep<T>), dim3(nq, 1), dim3(256), 0, s, D, dq);
     HCL();
     hipLaunchKernelGGL(k_mfma_bprep, dim3(pl->hd.ktab.nstep, ntile), dim3(64), 0, s, D, dq, nq);
@@ -461,7 +461,7 @@ void tune_scan(iq_plan* pl) {
       if (us < best) { best = us; pl->nw_tiles[ntile] = nw; }
     }
   }
-  if ((int)hq.size() < 64) pl->nw_tiles[2] = pl->nw_tiles[1];
+  if ((int)hq.size() <= 32) pl->nw_tiles[2] = pl->nw_tiles[1];
   HC(hipEventDestroy(a));
   HC(hipEventDestroy(b));
 }
