@@ -186,7 +186,7 @@ def main():
             "device_ms_per_step": dev_time / args.steps * 1e3,
             "learned_proposal": "off",
             "mincut": {"cuts": ncuts, "push_rounds_per_cut": cut_rounds / max(ncuts, 1),
-                       "relabel_levels_per_cut": cut_levels / max(ncuts, 1),
+                       "relabel_passes_per_cut": cut_levels / max(ncuts, 1),
                        "avg_launch_us": cut_sec / max(cut_launches, 1) * 1e6,
                        "share_of_device_time": cut_sec / max(dev_time, 1e-12),
                        "bound": "latency (dependent LDS accesses and barriers per push-relabel round)"},

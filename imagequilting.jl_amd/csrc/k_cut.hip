@@ -408,21 +408,41 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
         canm[k][d] = (fl[k] & (16 << d)) && S.rb[3 * vm[k][d] + d] > 0.0;
         canp[k][d] = (fl[k] & (1 << d)) && S.rf[3 * uu[k] + d] > 0.0;
       }
-    for (int lev = 1;; lev++) {
+    // The labels are the residual hop distances to the sink (+1): the least
+    // fixed point of h(u) = min over residual edges u->v of h(v) + 1 from
+    // these starting values, which relaxing in any order reaches as well as
+    // a level-by-level BFS does (every label written is the length of a real
+    // path, and labels only decrease).
+    for (;;) {
+      // Each wavefront relaxes its own nodes until they hold still (no
+      // barrier; labels of other wavefronts' nodes read meanwhile are valid
+      // upper bounds), then one team OR asks whether any wavefront lowered a
+      // label; a pass in which none did ends the relabel.  Labels strictly
+      // decrease, so both loops end.
       int ch = 0;
+      for (;;) {
+        int lc = 0;
 #pragma unroll
-      for (int k = 0; k < NPT; k++) {
-        const int u = uu[k];
-        if (u >= n) continue;
-        const int hu = h[u];
-        int hm[3], hp[3];
+        for (int k = 0; k < NPT; k++) {
+          const int u = uu[k];
+          if (u >= n) continue;
+          const int hu = h[u];
+          int hm[3], hp[3];
 #pragma unroll
-        for (int d = 0; d < 3; d++) { hm[d] = h[vm[k][d]]; hp[d] = h[vp[k][d]]; }
-        if (hu != HMAX) continue;
-        bool hit = false;
+          for (int d = 0; d < 3; d++) { hm[d] = h[vm[k][d]]; hp[d] = h[vp[k][d]]; }
+          int best = hu;
 #pragma unroll
-        for (int d = 0; d < 3; d++) hit |= (canm[k][d] && hm[d] == lev) || (canp[k][d] && hp[d] == lev);
-        if (hit) { h[u] = lev + 1; ch = 1; }
+          for (int d = 0; d < 3; d++) {
+            if (canm[k][d] && hm[d] + 1 < best) best = hm[d] + 1;
+            if (canp[k][d] && hp[d] + 1 < best) best = hp[d] + 1;
+          }
+          if (best < hu) { h[u] = best; lc = 1; }
+        }
+        if (!__any(lc)) break;
+        ch = 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
       levels++;
       if (!team_or<TEAM>(ch, tflag, par)) break;

@@ -322,7 +322,7 @@ class Plan:
         return int(v[0]), int(v[1]), int(v[2])
 
     def cut_stats(self):
-        """(push rounds, relabel BFS levels, cuts) of the last run."""
+        """(push rounds, global-relabel passes, cuts) of the last run."""
         v = np.zeros(3, dtype=np.int64)
         self.ctx.check(self.ctx._L.iq_plan_cut_stats(self.handle, v[0:].ctypes.data, v[1:].ctypes.data,
                                                      v[2:].ctypes.data))

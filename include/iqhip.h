@@ -202,8 +202,9 @@ int iq_plan_cut_time(iq_plan* plan, int64_t* launches, double* seconds);
 int iq_plan_scan_tuning(iq_plan* plan, int nw[2], float us[4]);
 
 /* Min-cut work of the last iq_plan_run (graphcut.jl:73 replacement):
- * push-relabel rounds, global-relabel BFS levels and cuts (register-resident
- * kernel; slabs beyond its size limit are not counted).  Diagnostic only. */
+ * push-relabel rounds, global-relabel passes (team-wide relaxation passes
+ * of the distance labels) and cuts (register-resident kernel; slabs beyond
+ * its size limit are not counted).  Diagnostic only. */
 int iq_plan_cut_stats(iq_plan* plan, int64_t* rounds, int64_t* levels, int64_t* cuts);
 
 /* Selection work of the last iq_plan_run (plain queries with an overlap):
