@@ -330,6 +330,7 @@ struct iq_plan {
   int64_t dt_cap = 0;
   size_t cut_lds_max = 0;
   size_t multi_lds = 0;    // LDS of the tiled multi-source maps (0: gathering kernel)
+  int multi_bx = 16;       // candidate block width of the tiled maps (32 or 16; 16 rows)
   bool cut_big = false;    // largest slab needs the 512-lane register-resident cut
   bool cut_wave = false;   // every slab fits the one-wavefront cut
   std::vector<int32_t> enabled_host;
@@ -929,10 +930,14 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
     // tiled exact maps when the candidate block's training-image region, the
     // query values and their region offsets fit in LDS
     // (voxel lists as offsets into the staged region; read as scalar loads)
-    const int RX = MT_X + g.tile[0] - 1, RY = MT_Y + g.tile[1] - 1;
-    const size_t nreg = (size_t)RX * RY * g.tile[2];
-    const size_t need = nreg * 8;
-    if (need <= 150 * 1024) {
+    // (candidate blocks of 32 x 16 where the region fits, else 16 x 16;
+    // IQHIP_MAPS_BX=16 forces the narrow blocks)
+    const char* ebx = std::getenv("IQHIP_MAPS_BX");
+    for (int bx : {32, 16}) {
+      if (bx == 32 && ebx && std::atoi(ebx) == 16) continue;
+      const int RX = bx + g.tile[0] - 1, RY = 16 + g.tile[1] - 1;
+      const size_t need = (size_t)RX * RY * g.tile[2] * 8;
+      if (need > 150 * 1024) continue;
       auto region = [&](const std::vector<int32_t>& tv) {
         std::vector<int32_t> r(tv.size());
         for (size_t i = 0; i < tv.size(); i++) {
@@ -945,8 +950,10 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
       H.rtvox = pl->mem.upload(rt.data(), rt.size(), s);
       H.rhtvox = rh.empty() ? nullptr : pl->mem.upload(rh.data(), rh.size(), s);
       pl->multi_lds = need;
-      if (need > 64 * 1024)
-        HC(hipFuncSetAttribute((const void*)k_multi_maps_tiled, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+      pl->multi_bx = bx;
+      const void* fn = bx == 32 ? (const void*)k_multi_maps_tiled<32, 16> : (const void*)k_multi_maps_tiled<16, 16>;
+      if (need > 64 * 1024) HC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+      break;
     }
     H.maps = pl->mem.alloc<double>((size_t)NS * nmap * P);
     H.bits = pl->mem.alloc<uint32_t>((size_t)NS * nmap * nw32);
@@ -1102,8 +1109,12 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
   }
   if (has_multi) {
     if (pl->multi_lds > 0) {
-      const unsigned nb = nblk(g.dist[0], MT_X) * nblk(g.dist[1], MT_Y) * (unsigned)g.dist[2];
-      hipLaunchKernelGGL(k_multi_maps_tiled, dim3(nb, nq, 2 + pl->naux), dim3(256), pl->multi_lds, s, D, q);
+      const int bx = pl->multi_bx;
+      const unsigned nb = nblk(g.dist[0], bx) * nblk(g.dist[1], 16) * (unsigned)g.dist[2];
+      if (bx == 32)
+        hipLaunchKernelGGL((k_multi_maps_tiled<32, 16>), dim3(nb, nq, 2 + pl->naux), dim3(512), pl->multi_lds, s, D, q);
+      else
+        hipLaunchKernelGGL((k_multi_maps_tiled<16, 16>), dim3(nb, nq, 2 + pl->naux), dim3(256), pl->multi_lds, s, D, q);
     } else {
       hipLaunchKernelGGL(k_multi_maps, dim3(nblk(P, 256), nq, 2 + pl->naux), dim3(256), 0, s, D, q);
     }

@@ -1029,17 +1029,18 @@ k_multi_maps(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   out[c] = __builtin_fabs((a2 - 2.0 * ab) + b2);
 }
 
-// Tiled form of k_multi_maps: a workgroup owns MT_X x MT_Y candidates at one
-// z; the training-image (or auxiliary-TI) region their windows cover is
+// Tiled form of k_multi_maps: a workgroup of MX x MY work-items owns MX x MY
+// candidates at one z (32 x 16 where the region fits, else 16 x 16: rows of
+// 32 consecutive candidates keep a half-wavefront's 8-byte reads on distinct
+// LDS banks, and 512-wide groups hold more wavefronts per compute unit); the training-image (or auxiliary-TI) region their windows cover is
 // staged in LDS as FP64, so every work-item's chain (same ascending order,
 // same operations as k_multi_maps) reads LDS instead of gathering from HBM.
 // The query values and the voxels' region offsets (prepared per plan) are
 // the same for every work-item and come through scalar loads.  Grid: (x
 // blocks * y blocks * dist z, query, source); LDS = the region (the host
 // checks it fits).
-#define MT_X 16
-#define MT_Y 16
-__global__ void __launch_bounds__(256)
+template <int MT_X, int MT_Y>
+__global__ void __launch_bounds__(MT_X * MT_Y)
 k_multi_maps_tiled(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   extern __shared__ double mreg[];
   const IqQuery q = qs[blockIdx.y];

@@ -86,6 +86,22 @@ def test_sim_soft_and_hard():
     _compare(T, (10, 10), (50, 50), soft=[(aux, aux_ti)], hard=hard, tol=0.2)
 
 
+@pytest.mark.parametrize("bx", ["32", "16"])
+def test_sim_soft_and_hard_3d_map_blocks(monkeypatch, bx):
+    # exact multi-source maps with 32 x 16 and 16 x 16 candidate blocks
+    # (IQHIP_MAPS_BX=16 forces the narrow ones, which plans take when the
+    # wide block's region does not fit in LDS); candidate grid 27 x 23 x 5
+    monkeypatch.setenv("IQHIP_MAPS_BX", bx)
+    rng = np.random.default_rng(25)
+    T = _smooth(rng, (36, 30, 8), 4, np.float32)
+    aux_ti = _smooth(rng, (36, 30, 8), 3, np.float32)
+    aux = _smooth(rng, (30, 30, 10), 3, np.float32)
+    hard = {(int(i), int(j), int(k)): float(T[i % 36, j % 30, k % 8])
+            for i, j, k in zip(rng.integers(0, 30, 10), rng.integers(0, 30, 10), rng.integers(0, 10, 10))}
+    print("soft + hard 3-D, map blocks", bx, flush=True)
+    _compare(T, (10, 8, 4), (30, 30, 10), overlap=(0.3, 0.25, 0.25), soft=[(aux, aux_ti)], hard=hard, tol=0.2)
+
+
 def test_sim_masked_ti_and_grid():
     T = np.ones((20, 20, 20))
     T[:, 4, :] = np.nan
