@@ -742,18 +742,23 @@ __device__ __forceinline__ void blk_prefix_bits(const double* __restrict__ map, 
   const auto* mg = iq_glob(map);
   const int lane = threadIdx.x & 63;
   const int64_t nwd = (P + 31) / 32;
-  for (int64_t base = (int64_t)(threadIdx.x >> 6) * 64; base < P; base += FB) {
-    const int64_t i = base + lane;
-    bool in = false;
-    if (i < P) {
-      const uint64_t v = f64_bits(mg[i]);
-      in = v < tv || (v == tv && i <= ti);
+  // eight loads in flight per work-item
+  for (int64_t base0 = (int64_t)(threadIdx.x >> 6) * 64; base0 < P; base0 += 8 * FB) {
+    uint64_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int64_t i = base0 + k * FB + lane;
+      v[k] = i < P ? f64_bits(mg[i]) : 0ull;
     }
-    const uint64_t b = __ballot(in);
-    if (lane == 0) {
-      const int64_t w = base >> 5;
-      bits[w] = (uint32_t)b;
-      if (w + 1 < nwd) bits[w + 1] = (uint32_t)(b >> 32);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int64_t base = base0 + k * FB, i = base + lane;
+      const uint64_t b = __ballot(i < P && (v[k] < tv || (v[k] == tv && i <= ti)));
+      if (lane == 0 && base < P) {
+        const int64_t w = base >> 5;
+        bits[w] = (uint32_t)b;
+        if (w + 1 < nwd) bits[w + 1] = (uint32_t)(b >> 32);
+      }
     }
   }
 }
@@ -767,11 +772,16 @@ __device__ __forceinline__ int64_t relax_db(int64_t P, int nsrc, const double* c
   // enabled patterns = finite entries of D; all zero when none is finite and nonzero (relaxation.jl:7-11)
   const auto* dg = iq_glob(maps[0]);
   int64_t cf = 0, cz = 0;
-  for (int64_t i = threadIdx.x; i < P; i += FB) {
-    const double x = dg[i];
-    const bool fin = !__builtin_isinf(x);
-    cf += fin;
-    cz += fin && x != 0.0;
+  for (int64_t i0 = threadIdx.x; i0 < P; i0 += 8 * FB) {  // eight loads in flight
+    double x[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = i0 + k * FB < P ? dg[i0 + k * FB] : __builtin_inf();
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const bool fin = !__builtin_isinf(x[k]);
+      cf += fin;
+      cz += fin && x[k] != 0.0;
+    }
   }
   const int64_t nen = blk_sum_i64(cf, sh);
   const int64_t nnz = blk_sum_i64(cz, sh);
