@@ -1019,7 +1019,14 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
   const IqDev* D = pl->dd;
   const int64_t P = g.P;
   const int nsrc_prep = has_multi ? 2 + pl->naux : 1;
-  hipLaunchKernelGGL((k_query_prep<T>), dim3(nq, nsrc_prep), dim3(256), 0, s, D, q);
+  // plain batches of the MFMA scan: query preparation and B fragments in one launch
+  const bool fused_prep = !has_multi && has_plain && sel == SEL_SCAN && pl->fast && pl->mfma;
+  if (fused_prep) {
+    const int nitems = pl->hd.ktab.nstep * ((nq + 31) / 32);
+    hipLaunchKernelGGL((k_prep_bprep<T>), dim3(nq + (nitems + 3) / 4), dim3(256), 0, s, D, q, nq, nitems);
+  } else {
+    hipLaunchKernelGGL((k_query_prep<T>), dim3(nq, nsrc_prep), dim3(256), 0, s, D, q);
+  }
   HCL();
   if (has_plain && sel != SEL_SCAN && ntiles > 0) {
     // shortlist: query encodings -> bracket from a sample -> MFMA scan of the
@@ -1056,8 +1063,10 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
       }
       if (pl->mfma) {
         const int ntile = (nq + 31) / 32;
-        hipLaunchKernelGGL(k_mfma_bprep, dim3(pl->hd.ktab.nstep, ntile), dim3(64), 0, s, D, q, nq);
-        HCL();
+        if (!fused_prep) {
+          hipLaunchKernelGGL(k_mfma_bprep, dim3(pl->hd.ktab.nstep, ntile), dim3(64), 0, s, D, q, nq);
+          HCL();
+        }
         const int nw = pl->mfma_nw_forced ? pl->mfma_nw : pl->nw_tiles[ntile > 1 ? 2 : 1];
         const int YB = 4 * nw;
         dim3 grid(nblk(g.dist[0], 32), nblk(g.dist[1], YB), (unsigned)(g.dist[2] * ntile));

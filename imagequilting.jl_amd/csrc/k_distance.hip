@@ -92,11 +92,9 @@ k_gather_b2(const double* __restrict__ kern, const int32_t* __restrict__ tv, int
 #define PREP_LDS 8192
 
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_query_prep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
-  __shared__ double kl[PREP_LDS];
-  const IqQuery q = qs[blockIdx.x];
-  const int src = blockIdx.y;
+__device__ __forceinline__ void query_prep_body(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int qx,
+                                                int src, double* kl) {
+  const IqQuery q = qs[qx];
   const IqGeo& g = D->g;
   const IqStep& st = D->steps[q.step];
   const int64_t slot = q.slot;
@@ -200,6 +198,13 @@ k_query_prep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     for (; j < V; j++) s = __builtin_fma(kv[j], kv[j], s);  // beyond the LDS stage
     D->b2[slot * (2 + D->naux) + src] = s;
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_query_prep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  __shared__ double kl[PREP_LDS];
+  query_prep_body<T>(D, qs, blockIdx.x, blockIdx.y, kl);
 }
 
 template __global__ void k_query_prep<float>(const IqDev*, const IqQuery*);
@@ -477,9 +482,8 @@ __device__ inline uint32_t iq_split_word(float t) {  // hi | lo << 16
 
 // B fragments of a batch: lane l of (tile, step) holds the eight values of
 // chunk 2*step + l/32 for query tile*32 + l%32, as bf16 hi and lo.
-__global__ void __launch_bounds__(64)
-k_mfma_bprep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq) {
-  const int step = blockIdx.x, tile = blockIdx.y, lane = threadIdx.x;
+__device__ __forceinline__ void mfma_bprep_body(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq,
+                                                int step, int tile, int lane) {
   const int nstep = D->ktab.nstep;
   const int qi = tile * 32 + (lane & 31);
   uint32_t h[4] = {0u, 0u, 0u, 0u}, l[4] = {0u, 0u, 0u, 0u};
@@ -528,6 +532,31 @@ k_mfma_bprep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq
   D->bh[o] = make_uint4(h[0], h[1], h[2], h[3]);
   D->bl[o] = make_uint4(l[0], l[1], l[2], l[3]);
 }
+
+__global__ void __launch_bounds__(64)
+k_mfma_bprep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq) {
+  mfma_bprep_body(D, qs, nq, blockIdx.x, blockIdx.y, threadIdx.x);
+}
+
+// Query preparation and B fragments in one launch (both read the grid, neither
+// reads what the other writes): blocks [0, nq) prepare one query each (the
+// overlap source), the others hold four (k-step, query tile) items, one per
+// wavefront.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_prep_bprep(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, int nq, int nitems) {
+  __shared__ double kl[PREP_LDS];
+  if ((int)blockIdx.x < nq) {
+    query_prep_body<T>(D, qs, blockIdx.x, 0, kl);
+    return;
+  }
+  const int item = ((int)blockIdx.x - nq) * 4 + (int)(threadIdx.x >> 6);
+  if (item >= nitems) return;
+  const int nstep = D->ktab.nstep;
+  mfma_bprep_body(D, qs, nq, item % nstep, item / nstep, threadIdx.x & 63);
+}
+template __global__ void k_prep_bprep<float>(const IqDev*, const IqQuery*, int, int);
+template __global__ void k_prep_bprep<double>(const IqDev*, const IqQuery*, int, int);
 
 // FP32 copy of an A2 map for the scan's interval bound; NaN marks a disabled
 // window (an A2 that overflows FP32 becomes +inf and stays a candidate)
