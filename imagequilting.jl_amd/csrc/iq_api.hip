@@ -252,17 +252,41 @@ bool build_ktab(const std::vector<uint8_t>& ubits, const int t3[3], const std::v
   return true;
 }
 
+// Values the FP32 / split-bf16 scans take: FP32-exact, |x| <= 2^50, and zero
+// or |x| >= 2^-100 (so every split part and product stays a normal number and
+// the relative error bounds of scan_gerr hold).
 bool fits_f32(const double* v, int64_t n) {
   for (int64_t i = 0; i < n; i++) {
     double x = v[i];
-    if ((double)(float)x != x || std::fabs(x) > 0x1p50) return false;
+    if ((double)(float)x != x || std::fabs(x) > 0x1p50 || (x != 0.0 && std::fabs(x) < 0x1p-100)) return false;
   }
   return true;
 }
 bool fits_range(const float* v, int64_t n) {
   for (int64_t i = 0; i < n; i++)
-    if (!(std::fabs(v[i]) <= 0x1p50f)) return false;
+    if (!(std::fabs(v[i]) <= 0x1p50f) || (v[i] != 0.0f && std::fabs(v[i]) < 0x1p-100f)) return false;
   return true;
+}
+
+// Relative error bound of the MFMA scan's correlation: |AB32 - AB| <=
+// scan_gerr * sum |a||b| over a mask of nstep k-steps (16 voxels each).
+//  * Accumulation: every bf16 x bf16 product is exact in FP32; the FP32 sums
+//    of kterms = 16 nstep (3 or 1) terms, in any order, err by at most
+//    gamma(kterms - 1) times the sum of the terms' magnitudes, which is at
+//    most (1 + 2^-8)^2 (1 + 2^-7) sum |a||b| <= 1.02 sum |a||b|; gamma of
+//    twice the count is used.
+//  * Split (FP32 data): t = hi + lo + r with hi = bf16(t), lo = bf16(t - hi)
+//    (t - hi is exact in FP32).  For 2^e <= |t|: |t - hi| <= 2^(e-8), so
+//    |lo| <= 2^-8 |t| and |r| <= half a bf16 ulp of |t - hi| < 2^(e-8), i.e.
+//    |r| <= 2^(e-17) <= 2^-17 |t|; |hi + lo| <= (1 + 2^-17) |t|.  The three
+//    products hi hi, hi lo, lo hi drop ab - (...) = (ha + la) rb + la lb + ra b,
+//    at most (2^-16 + 2^-17 (1 + 2^-17) + 2^-17) |a||b|
+//    = 2^-15 (1 + 2^-19) |a||b|.  Values are zero or at least 2^-100 in
+//    magnitude (fits_f32 / fits_range), so no part is subnormal.
+double scan_gerr(int64_t nstep, bool split) {
+  const double kterms = 16.0 * (double)nstep * (split ? 3.0 : 1.0);
+  const double ga = 2.0 * kterms * 0x1p-24 / (1.0 - 2.0 * kterms * 0x1p-24);
+  return 1.02 * ga + (split ? 0x1p-15 * (1.0 + 0x1p-19) : 0.0);
 }
 
 void geo_init(IqGeo& g, const int64_t ti[3], const int64_t tile[3]) {
@@ -467,13 +491,56 @@ void tune_scan(iq_plan* pl) {
   HC(hipEventDestroy(b));
 }
 
-void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) {
+// Argument checks of a plan description (the @asserts of iqsim.jl:69-89 as
+// they reach the ABI, and the geometry identities of iqsim.jl:92-115); host
+// only, no device work.
+void validate_descs(const iq_plan_desc* descs, int ngrid) {
   if (!descs || ngrid <= 0) bad(IQ_ERR_ARG, "no simulation grid");
-  const iq_plan_desc* d = &descs[0];  // training image, tile geometry, tol and soft pairs of every grid
+  const iq_plan_desc* d = &descs[0];
   if (!d->ti || !d->disabled) bad(IQ_ERR_ARG, "null plan input");
   if (d->ti_dtype != IQ_F32 && d->ti_dtype != IQ_F64) bad(IQ_ERR_ARG, "ti_dtype must be IQ_F32 or IQ_F64");
   if (!(d->tol > 0.0 && d->tol <= 1.0)) bad(IQ_ERR_ARG, "tolerance must be in range (0,1]");
   if (d->naux < 0 || d->naux > IQ_MAX_AUX) bad(IQ_ERR_ARG, "naux must be in 0..8");
+  for (int k = 0; k < 3; k++) {
+    if (d->tilesize[k] <= 0 || d->tilesize[k] > d->tisize[k]) bad(IQ_ERR_ARG, "invalid tile size");
+    if (d->ovlsize[k] < 0 || d->ovlsize[k] > d->tilesize[k]) bad(IQ_ERR_ARG, "invalid overlap size");
+    if (d->spacing[k] != d->tilesize[k] - d->ovlsize[k]) bad(IQ_ERR_ARG, "spacing must be tilesize - ovlsize");
+  }
+  if (d->naux > 0) {
+    if (!d->auxti) bad(IQ_ERR_ARG, "null soft data");
+    for (int a = 0; a < d->naux; a++)
+      if (!d->auxti[a]) bad(IQ_ERR_ARG, "null soft data");
+  }
+  for (int gi = 0; gi < ngrid; gi++) {
+    const iq_plan_desc* e = &descs[gi];
+    if (!e->path || !e->skipped) bad(IQ_ERR_ARG, "null plan input");
+    if (e->nreal <= 0) bad(IQ_ERR_ARG, "invalid number of realizations");
+    if (e->ti_dtype != d->ti_dtype || e->naux != d->naux || e->tol != d->tol)
+      bad(IQ_ERR_ARG, "grids of one plan share the training image, tol and soft pairs");
+    for (int k = 0; k < 3; k++) {
+      if (e->tisize[k] != d->tisize[k] || e->tilesize[k] != d->tilesize[k] || e->ovlsize[k] != d->ovlsize[k] ||
+          e->spacing[k] != d->spacing[k])
+        bad(IQ_ERR_ARG, "grids of one plan share the training image and tile geometry");
+      if (e->ntiles[k] <= 0) bad(IQ_ERR_ARG, "invalid ntiles");
+      if (e->padsize[k] != e->ntiles[k] * e->spacing[k] + e->ovlsize[k]) bad(IQ_ERR_ARG, "padsize must be ntiles*(tilesize-ovlsize)+ovlsize");
+    }
+    const int64_t nt = e->ntiles[0] * e->ntiles[1] * e->ntiles[2];
+    std::vector<uint8_t> seen((size_t)nt, 0);
+    for (int64_t i = 0; i < e->npath; i++) {
+      if (e->path[i] < 0 || e->path[i] >= nt) bad(IQ_ERR_ARG, "path entry outside the tile grid");
+      if (seen[e->path[i]]++) bad(IQ_ERR_ARG, "path visits a tile twice");
+    }
+    if (e->naux > 0) {
+      if (!e->aux) bad(IQ_ERR_ARG, "null soft data");
+      for (int a = 0; a < e->naux; a++)
+        if (!e->aux[a]) bad(IQ_ERR_ARG, "null soft data");
+    }
+  }
+}
+
+void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) {
+  validate_descs(descs, ngrid);
+  const iq_plan_desc* d = &descs[0];  // training image, tile geometry, tol and soft pairs of every grid
   hipStream_t s = ctx->stream;
   pl->ctx = ctx;
   pl->dtype = d->ti_dtype;
@@ -674,6 +741,10 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   for (int64_t c = 0; c < g.P; c++)
     if (!d->disabled[c]) en.push_back((int32_t)c);
   pl->nen = (int64_t)en.size();
+  // every window holds an inactive voxel: the reference's band would take
+  // every (infinitely distant) pattern and its relaxation would not end
+  // (iqsim.jl:237, relaxation.jl:7-36); the plan is refused instead
+  if (pl->nen == 0) bad(IQ_ERR_NOCAND, "every candidate pattern is disabled (all training-image windows hold inactive voxels)");
   H.enabled = pl->mem.upload(en.data(), en.size(), s);
   pl->enabled_host = en;
   H.nen = pl->nen;
@@ -785,9 +856,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
         pl->mfma = true;
         pl->mfma_lds = lmax;
         pl->mfma_split = !bf16_exact(H.ti32, tivox);
-        const double kterms = 16.0 * H.ktab.nstep * (pl->mfma_split ? 3.0 : 1.0);
-        const double ga = 2.0 * kterms * 0x1p-24 / (1.0 - 2.0 * kterms * 0x1p-24);
-        pl->mfma_gerr = 1.02 * ga + (pl->mfma_split ? 4.0 * 0x1p-18 : 0.0);
+        pl->mfma_gerr = scan_gerr(H.ktab.nstep, pl->mfma_split);
         H.chunks = pl->mem.upload(chunks.data(), chunks.size(), s);
         for (int nw : {2, 4}) {
           if (!pl->lds_nw[nw]) continue;
@@ -1701,6 +1770,15 @@ int iq_graphcut(iq_ctx* ctx, const void* A, const void* B, int dtype, const int6
     check_kernel_error(code);
   });
 }
+
+int iq_scan_error_bound(int64_t nstep, int split, double* gerr) {
+  return guard(nullptr, [&] {
+    if (nstep <= 0 || !gerr) bad(IQ_ERR_ARG, "nstep must be positive");
+    *gerr = scan_gerr(nstep, split != 0);
+  });
+}
+
+int iq_plan_validate(const iq_plan_desc* descs, int ngrid) { return guard(nullptr, [&] { validate_descs(descs, ngrid); }); }
 
 int iq_plan_create(iq_ctx* ctx, const iq_plan_desc* desc, iq_plan** out) {
   if (!ctx) return fail(nullptr, IQ_ERR_ARG, "null context");

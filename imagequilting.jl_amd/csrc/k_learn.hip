@@ -191,6 +191,7 @@ __device__ inline float learn_score(const IqDev* D, int mask, int64_t c, const f
 __global__ void __launch_bounds__(1024)
 k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   __shared__ uint32_t smp[LKS];
+  __shared__ unsigned long long best;  // (key, sample) of the sample minimum
   const IqQuery q = qs[blockIdx.x];
   const IqStep& st = D->steps[q.step];
   if (!learn_applies(D, q, st)) return;
@@ -199,9 +200,12 @@ k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const float* eq = D->lrn.eq + slot * IQ_EMB;
   const double b2 = D->b2[slot * (2 + D->naux)];
   const int64_t K = D->lrn.K < D->nen ? D->lrn.K : D->nen;
+  if (threadIdx.x == 0) best = ~0ull;
+  __syncthreads();
   for (int i = threadIdx.x; i < LKS; i += blockDim.x) {
     const int64_t c = (int64_t)i * g.P / LKS;
     smp[i] = D->disabled[c] ? 0xffffffffu : iq_f32_key(learn_score(D, st.mask, c, eq, b2));
+    atomicMin(&best, ((unsigned long long)smp[i] << 32) | (unsigned)i);
   }
   int32_t* hist = D->lrn.hist + slot * 2048;
   for (int i = threadIdx.x; i < 2048; i += blockDim.x) hist[i] = 0;
@@ -229,6 +233,11 @@ k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     while (shift < 32 && (span >> shift) >= 2048u) shift++;
     uint32_t* b = D->lrn.brk + slot * 4;
     b[0] = lo; b[1] = hi; b[2] = shift;
+    // seed of the shortlist should no candidate fall inside the bracket: the
+    // sample minimum, or the first enabled candidate when every sampled
+    // window is disabled
+    b[3] = (uint32_t)((best >> 32) != 0xffffffffull ? (int64_t)(best & 0xffffffffull) * g.P / LKS
+                                                      : (int64_t)D->enabled[0]);
     D->lrn.lcnt[slot] = 0;
     D->lrn.nslist[slot] = 0;
   }
@@ -372,6 +381,16 @@ k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
         atomicOr(&D->lrn.sbits[slot * D->nw64 + (it.x >> 6)], 1ull << (it.x & 63));
       }
     }
+  }
+  // an empty shortlist (no approximate distance fell inside the bracket,
+  // e.g. ties resolved one ulp apart) takes the seed: the exact recheck and
+  // the pick then run on it instead of finding no candidate at all
+  __syncthreads();
+  if (threadIdx.x == 0 && D->lrn.nslist[slot] == 0) {
+    const int64_t c = D->lrn.brk[slot * 4 + 3];
+    D->lrn.slist[slot * g.P] = (int32_t)c;
+    D->lrn.nslist[slot] = 1;
+    atomicOr(&D->lrn.sbits[slot * D->nw64 + (c >> 6)], 1ull << (c & 63));
   }
 }
 

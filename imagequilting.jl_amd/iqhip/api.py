@@ -531,7 +531,7 @@ def finish(info, grids, cuts=None):
             voxs.append(float(cg.sum()) / geo["ovlvol"] if geo["ovlvol"] > 0 else float("nan"))  # iqsim.jl:288
         for k, v in hard.items():  # iqsim.jl:291-294
             g[tuple(k)] = v
-            if cg is not None and isinstance(v, float) and math.isnan(v):
+            if cg is not None and prep.isnan_value(v):
                 cg[tuple(k)] = v
         reals.append(prep.unprepare(T, g[crop]))
         if cg is not None:

@@ -29,6 +29,12 @@ def geometry(tisize: Sequence[int], tilesize: Sequence[int], simsize: Sequence[i
                 ovlvol=ovlvol)
 
 
+def isnan_value(v) -> bool:
+    """isnan of one hard-data or object-array value (utils.jl:137,150): true
+    for NaN of any floating type (Python float, numpy float16/32/64)."""
+    return isinstance(v, (float, np.floating)) and bool(np.isnan(v))
+
+
 def _missing_mask(a: np.ndarray) -> np.ndarray:
     if isinstance(a, np.ma.MaskedArray):
         m = np.ma.getmaskarray(a)
@@ -37,7 +43,7 @@ def _missing_mask(a: np.ndarray) -> np.ndarray:
             m = m | np.isnan(data)
         return m
     if a.dtype == object:
-        return np.vectorize(lambda v: v is None or (isinstance(v, float) and math.isnan(v)), otypes=[bool])(a)
+        return np.vectorize(lambda v: v is None or isnan_value(v), otypes=[bool])(a)
     if a.dtype.kind == "f":
         return np.isnan(a)
     return np.zeros(a.shape, dtype=bool)
@@ -49,7 +55,7 @@ def element_type(a) -> np.dtype:
         return a.data.dtype
     a = np.asarray(a)
     if a.dtype == object:
-        vals = [v for v in a.ravel() if v is not None and not (isinstance(v, float) and math.isnan(v))]
+        vals = [v for v in a.ravel() if v is not None and not isnan_value(v)]
         return np.asarray(vals).dtype if vals else np.dtype(np.float64)
     return a.dtype
 
@@ -134,7 +140,7 @@ def findskipped(hard: Dict, geo) -> Tuple[set, List[int]]:
     nan_g = np.zeros(pad, dtype=bool)
     dat_g = np.zeros(pad, dtype=bool)
     for k, v in hard.items():
-        if isinstance(v, float) and math.isnan(v):
+        if isnan_value(v):
             nan_g[tuple(k)] = True
         else:
             dat_g[tuple(k)] = True
@@ -154,14 +160,19 @@ def findskipped(hard: Dict, geo) -> Tuple[set, List[int]]:
 
 
 def _dilate(grid: np.ndarray) -> np.ndarray:
-    """Max filter over the 3^N box neighbourhood (ImageMorphology.dilate)."""
+    """ImageMorphology 0.4 ``dilate(grid)``: max filter over its default
+    structuring element, the diamond of radius 1 (each voxel and its 2N face
+    neighbours, 1-connectivity)."""
     out = grid.copy()
-    N = grid.ndim
-    for off in np.ndindex(*([3] * N)):
-        d = [o - 1 for o in off]
-        src = tuple(slice(max(-x, 0), grid.shape[i] - max(x, 0)) for i, x in enumerate(d))
-        dst = tuple(slice(max(x, 0), grid.shape[i] - max(-x, 0)) for i, x in enumerate(d))
-        out[dst] |= grid[src]
+    for ax in range(grid.ndim):
+        n = grid.shape[ax]
+        if n < 2:
+            continue
+        lo = [slice(None)] * grid.ndim
+        hi = [slice(None)] * grid.ndim
+        lo[ax], hi[ax] = slice(0, n - 1), slice(1, n)
+        out[tuple(lo)] |= grid[tuple(hi)]
+        out[tuple(hi)] |= grid[tuple(lo)]
     return out
 
 

@@ -40,7 +40,11 @@ struct PlanDesc
   nreal::Int64
 end
 
+# extents padded to three axes: a missing trailing axis is one voxel thick
 pad3(t::NTuple{N,Int}) where {N} = ntuple(i -> i <= N ? Int64(t[i]) : Int64(1), 3)
+# overlaps padded to three axes: a missing trailing axis has no overlap
+# (spacing = tilesize - ovlsize = 1 and padsize = ntiles * spacing + ovlsize = 1 there)
+pad3ovl(t::NTuple{N,Int}) where {N} = ntuple(i -> i <= N ? Int64(t[i]) : Int64(0), 3)
 
 function check(rc, ctx=C_NULL)
   rc == 0 && return nothing
@@ -200,7 +204,7 @@ function iqsim_hip(trainimg::AbstractArray{T,N}, tilesize::Dims{N}, simsize::Dim
   chosen = Array{Int64}(undef, length(simpath), nreal)
   cuts = debug ? Array{Float64}(undef, padsize..., nreal) : Float64[]
   GC.@preserve tiarr SOFT disabled simpath skp hardgrid auxp auxtip uniforms out chosen cuts begin
-    desc = PlanDesc(pointer(tiarr), tidt, pad3(TIsize), pad3(tilesize), pad3(Tuple(ovlsize)), pad3(Tuple(spacing)),
+    desc = PlanDesc(pointer(tiarr), tidt, pad3(TIsize), pad3(tilesize), pad3ovl(Tuple(ovlsize)), pad3(Tuple(spacing)),
                     pad3(Tuple(ntiles)), pad3(Tuple(padsize)), pointer(disabled), pointer(simpath), length(simpath),
                     pointer(skp), isempty(hard) ? Ptr{Float64}(C_NULL) : pointer(hardgrid), Cint(length(SOFT)),
                     isempty(SOFT) ? Ptr{Ptr{Float64}}(C_NULL) : pointer(auxp),

@@ -148,6 +148,12 @@ typedef struct iq_plan_desc {
   int64_t nreal;
 } iq_plan_desc;
 
+/* Argument checks of iq_plan_create_multi without any device work (the
+ * @asserts of iqsim.jl:69-89 as they reach the ABI and the geometry
+ * identities of iqsim.jl:92-115): IQ_OK, or IQ_ERR_ARG with the reason in
+ * iq_last_error(NULL).  Lets a binding check the descriptors it builds. */
+int iq_plan_validate(const iq_plan_desc* descs, int ngrid);
+
 /* Upload everything once (the TI stays resident in HBM). */
 int iq_plan_create(iq_ctx* ctx, const iq_plan_desc* desc, iq_plan** out);
 
@@ -188,6 +194,13 @@ int iq_plan_last_time(iq_plan* plan, double* seconds);
  * MFMA scan issues (0 for the VALU scan).  Diagnostic only; off by default. */
 int iq_plan_profile(iq_plan* plan, int enable);
 int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double* flop, double* mfma_flop);
+
+/* Relative error bound of the MFMA distance scan's correlation term (no
+ * counterpart in the reference; host only): |AB32 - AB| <= *gerr * sum|a||b|
+ * for a mask of nstep k-steps of 16 voxels, split != 0 for FP32 data (three
+ * bf16 products per voxel pair), 0 for bf16-exact data (one product).  The
+ * scan's interval around each exact FP64 distance is built from it. */
+int iq_scan_error_bound(int64_t nstep, int split, double* gerr);
 
 /* Min-cut launches of the last run and their summed device time (HIP events
  * around each launch, recorded while profiling is enabled; no counterpart in

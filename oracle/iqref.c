@@ -654,6 +654,10 @@ typedef struct {
   double* cutgrid;           /* nreal x padsize or NULL (debug) */
   int64_t* cut_iters;        /* total push-relabel rounds, or NULL */
   int64_t* nband;            /* nreal x npath pattern-database sizes (0 for skipped), or NULL */
+  uint8_t* step_cut;         /* nreal x npath x tilevox cut masks per step (1 = keep the grid), or NULL */
+  double* step_before;       /* nreal x npath x tilevox grid values of the tile before the paste, or NULL */
+  int mutate;                /* mistakes planted on purpose by tests of the checks themselves:
+                                0 none, 1 next slabs not negated, 2 prev slabs one voxel further in */
 } ora_sim_args;
 
 static void tile_offsets(const uint8_t* m, const int64_t ts[3], int64_t* jx, int64_t* jy, int64_t* jz, int64_t* V) {
@@ -799,6 +803,7 @@ int ora_simulate(const ora_sim_args* a) {
           int64_t lo[3] = {0, 0, 0}, sl[3] = {ts[0], ts[1], ts[2]};
           if (side == 0) sl[d] = a->ovlsize[d];
           else { lo[d] = a->spacing[d]; sl[d] = ts[d] - a->spacing[d]; }
+          if (a->mutate == 2 && side == 0 && sl[d] < ts[d]) lo[d] = 1;
           for (int64_t z = 0; z < sl[2]; z++)
             for (int64_t y = 0; y < sl[1]; y++)
               for (int64_t x = 0; x < sl[0]; x++) {
@@ -814,7 +819,7 @@ int ora_simulate(const ora_sim_args* a) {
             for (int64_t y = 0; y < sl[1]; y++)
               for (int64_t x = 0; x < sl[0]; x++) {
                 uint8_t mv = smask[lin3(sl, x, y, z)];
-                if (side == 1) mv = !mv;
+                if (side == 1 && a->mutate != 1) mv = !mv;
                 cutmask[lin3(ts, lo[0] + x, lo[1] + y, lo[2] + z)] |= mv;
               }
         }
@@ -827,6 +832,9 @@ int ora_simulate(const ora_sim_args* a) {
           for (int64_t x = 0; x < ts[0]; x++) {
             int64_t g = lin3(a->padsize, st[0] + x, st[1] + y, st[2] + z);
             uint8_t cm = cutmask[lin3(ts, x, y, z)];
+            const int64_t lg = (r * a->npath + sidx) * tvox + lin3(ts, x, y, z);
+            if (a->step_cut) a->step_cut[lg] = cm;
+            if (a->step_before) a->step_before[lg] = sim[g];
             if (!cm) sim[g] = a->ti[lin3(a->tisize, cs[0] + x, cs[1] + y, cs[2] + z)];
             if (cg) cg[g] = (double)cm;
           }

@@ -71,6 +71,11 @@ def num_threads() -> int:
 # ---------------------------------------------------------------------------
 # ctypes helpers
 # ---------------------------------------------------------------------------
+def _isnan_value(v) -> bool:
+    """isnan of one hard-data value (utils.jl:137,150) of any floating type."""
+    return isinstance(v, (float, np.floating)) and bool(np.isnan(v))
+
+
 def _d3(shape: Sequence[int]) -> np.ndarray:
     s = list(shape) + [1] * (3 - len(shape))
     return np.asarray(s, dtype=np.int64)
@@ -217,7 +222,7 @@ def prepare(img):
     a = np.asarray(_unmask(img))
     if a.dtype == object:
         vals = [np.nan if v is None else v for v in a.ravel(order="F")]
-        base = np.asarray([v for v in vals if not (isinstance(v, float) and math.isnan(v))])
+        base = np.asarray([v for v in vals if not _isnan_value(v)])
         ftype = base.dtype if base.dtype.kind == "f" else np.float64
         f = np.asarray(vals, dtype=np.float64).astype(ftype).reshape(a.shape, order="F")
     elif a.dtype.kind == "f":
@@ -232,7 +237,7 @@ def finddisabled(trainimg, geo) -> np.ndarray:
     """utils.jl:115-129: candidate windows holding a NaN/missing voxel."""
     a = np.asarray(_unmask(trainimg))
     if a.dtype == object:
-        nanmask = np.vectorize(lambda v: v is None or (isinstance(v, float) and math.isnan(v)))(a)
+        nanmask = np.vectorize(lambda v: v is None or _isnan_value(v))(a)
     elif a.dtype.kind == "f":
         nanmask = np.isnan(a)
     else:
@@ -253,8 +258,8 @@ def _tile_coords(tileind, geo):
 def findskipped(hard: Dict, geo):
     """utils.jl:131-156 (with activation!/indicator!, utils.jl:31-55)."""
     N = len(geo["ntiles"])
-    nan_keys = {k for k, v in hard.items() if isinstance(v, float) and math.isnan(v)}
-    data_keys = {k for k, v in hard.items() if not (isinstance(v, float) and math.isnan(v))}
+    nan_keys = {k for k, v in hard.items() if _isnan_value(v)}
+    data_keys = {k for k, v in hard.items() if not _isnan_value(v)}
     skipped, datainds = set(), []
     for lin in range(int(np.prod(geo["ntiles"]))):
         tileind = np.unravel_index(lin, geo["ntiles"], order="F")
@@ -272,15 +277,20 @@ def findskipped(hard: Dict, geo):
     return skipped, datainds
 
 
-def _dilate_box(grid: np.ndarray) -> np.ndarray:
-    """Max filter over the 3^N box neighbourhood."""
+def _dilate_diamond(grid: np.ndarray) -> np.ndarray:
+    """ImageMorphology 0.4 ``dilate`` with its default structuring element,
+    the radius-1 diamond: a voxel is set when it or one of its face
+    neighbours is (written as a loop over voxels, independent of the
+    product's shifted-slice form)."""
     out = grid.copy()
-    N = grid.ndim
-    for off in np.ndindex(*([3] * N)):
-        d = [o - 1 for o in off]
-        src = tuple(slice(max(-x, 0), grid.shape[i] - max(x, 0)) for i, x in enumerate(d))
-        dst = tuple(slice(max(x, 0), grid.shape[i] - max(-x, 0)) for i, x in enumerate(d))
-        out[dst] |= grid[src]
+    shape = grid.shape
+    for idx in zip(*np.nonzero(grid)):
+        for ax in range(grid.ndim):
+            for step in (-1, 1):
+                j = list(idx)
+                j[ax] += step
+                if 0 <= j[ax] < shape[ax]:
+                    out[tuple(j)] = True
     return out
 
 
@@ -299,7 +309,7 @@ def genpath(rng: np.random.Generator, extent, kind: str, datainds: List[int]) ->
             path = [pivot]
             grid = flat.reshape(extent, order="F")
             while not grid.all():
-                dil = _dilate_box(grid)
+                dil = _dilate_diamond(grid)
                 new = np.flatnonzero((dil & ~grid).ravel(order="F"))
                 path.extend(int(i) for i in new)
                 grid = dil
@@ -313,7 +323,7 @@ def genpath(rng: np.random.Generator, extent, kind: str, datainds: List[int]) ->
         path.append(int(p))
     grid = flat.reshape(extent, order="F")
     while not grid.all():
-        dil = _dilate_box(grid)
+        dil = _dilate_diamond(grid)
         new = np.flatnonzero((dil & ~grid).ravel(order="F"))
         path.extend(int(i) for i in new)
         grid = dil
@@ -332,7 +342,13 @@ def pad_symmetric(aux, padsize):
 # ---------------------------------------------------------------------------
 def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None, tol=0.1,
           path="raster", nreal=1, debug=False, uniforms=None, forced=None, seed=0,
-          gr_freq=GR_FREQ, return_trace=False, max_steps=None, simpath=None):
+          gr_freq=GR_FREQ, return_trace=False, max_steps=None, simpath=None, step_log=False, mutate=0):
+    """The tile loop of iqsim.jl:163-300 in FP64 C.  ``step_log``: the trace
+    also holds, per realization and path step, the tile's grid values before
+    the paste and its cut mask (``before``, ``cutmask``: nreal x npath x
+    tilesize).  ``mutate`` plants a mistake in the loop (1: next slabs not
+    negated, 2: prev slabs one voxel further in); only tests of the checks
+    themselves use it."""
     T = np.asarray(trainimg)
     N = T.ndim
     if simsize is None:
@@ -400,6 +416,7 @@ def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None,
             ("uniforms", ctypes.c_void_p), ("forced", ctypes.c_void_p), ("gr_freq", ctypes.c_int),
             ("out", ctypes.c_void_p), ("chosen", ctypes.c_void_p), ("cutgrid", ctypes.c_void_p),
             ("cut_iters", ctypes.c_void_p), ("nband", ctypes.c_void_p),
+            ("step_cut", ctypes.c_void_p), ("step_before", ctypes.c_void_p), ("mutate", ctypes.c_int),
         ]
 
     a = Args()
@@ -427,6 +444,12 @@ def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None,
     a.cut_iters = cut_iters.ctypes.data
     nband = np.zeros(nreal * npath, dtype=np.int64)
     a.nband = nband.ctypes.data
+    tvox = int(np.prod(geo["tilesize"]))
+    step_cut = np.zeros(nreal * npath * tvox, dtype=np.uint8) if step_log else None
+    step_before = np.zeros(nreal * npath * tvox, dtype=np.float64) if step_log else None
+    a.step_cut = None if step_cut is None else step_cut.ctypes.data
+    a.step_before = None if step_before is None else step_before.ctypes.data
+    a.mutate = int(mutate)
     rc = lib().ora_simulate(ctypes.byref(a))
     if rc != ORA_OK:
         raise RuntimeError(f"oracle simulate failed: {rc}")
@@ -439,7 +462,7 @@ def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None,
             voxs.append(float(cg.sum()) / geo["ovlvol"])                  # iqsim.jl:288
         for k, v in hard.items():                                          # iqsim.jl:291-294
             sim[tuple(k)] = v
-            if debug and isinstance(v, float) and math.isnan(v):
+            if debug and _isnan_value(v):
                 cg[tuple(k)] = v
         crop = tuple(slice(0, s) for s in geo["simsize"])                 # iqsim.jl:303
         sim = sim[crop]
@@ -448,6 +471,13 @@ def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None,
             cuts.append(cg[crop])
     trace = dict(chosen=chosen.reshape(nreal, npath), path=simpath, geo=geo, uniforms=uniforms,
                  cut_iters=int(cut_iters[0]), nband=nband.reshape(nreal, npath))
+    if step_log:
+        tshape = (nreal, npath) + tuple(geo["tilesize"])
+        # per step a column-major tile: reversed axes in C order, then transposed
+        cshape = tshape[:2] + tuple(geo["tilesize"])[::-1]
+        axes = (0, 1) + tuple(range(len(tshape) - 1, 1, -1))
+        trace["cutmask"] = step_cut.reshape(cshape).transpose(axes).astype(bool)
+        trace["before"] = step_before.reshape(cshape).transpose(axes)
     res = (reals, cuts, voxs) if debug else reals
     return (res, trace) if return_trace else res
 

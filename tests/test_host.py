@@ -92,14 +92,57 @@ def test_findskipped_matches_oracle():
     assert 0 in a[0]
 
 
-def test_genpath_matches_oracle():
-    for kind in ("raster", "random", "dilation"):
-        a = prep.genpath(np.random.default_rng(5), (6, 5, 4), kind, [])
-        b = iqref.genpath(np.random.default_rng(5), (6, 5, 4), kind, [])
-        assert a == b and sorted(a) == list(range(120))
-    a = prep.genpath(np.random.default_rng(5), (6, 5), "raster", [3, 17, 29])
-    b = iqref.genpath(np.random.default_rng(5), (6, 5), "raster", [3, 17, 29])
-    assert a == b and set(a[:3]) == {3, 17, 29}
+def _l1_ring_order(extent, seeds):
+    """Tiles after `seeds` in the order repeated dilations with ImageMorphology
+    0.4's default diamond structuring element add them (utils.jl:176-200):
+    by Manhattan distance to the nearest seed, each ring in ascending
+    column-major index order (findall over the grid, utils.jl:182)."""
+    n = int(np.prod(extent))
+    coords = np.stack(np.unravel_index(np.arange(n), extent, order="F"), axis=1)
+    S = coords[list(seeds)]
+    dist = np.abs(coords[:, None, :] - S[None, :, :]).sum(-1).min(1)
+    seen = set(seeds)
+    return [t for t in sorted(range(n), key=lambda t: (dist[t], t)) if t not in seen]
+
+
+@pytest.mark.parametrize("impl", ["product", "oracle"])
+def test_genpath_closed_forms(impl):
+    gen = prep.genpath if impl == "product" else iqref.genpath
+    for extent in ((6, 5, 4), (7, 3), (1, 9), (4, 4, 1)):
+        n = int(np.prod(extent))
+        assert gen(np.random.default_rng(5), extent, "raster", []) == list(range(n))  # utils.jl:162-166
+        assert gen(np.random.default_rng(5), extent, "random", []) == \
+            [int(i) for i in np.random.default_rng(5).permutation(n)]
+        p = gen(np.random.default_rng(5), extent, "dilation", [])
+        assert p[0] == int(np.random.default_rng(5).integers(n))
+        assert p[1:] == _l1_ring_order(extent, [p[0]])
+        data = [1, n - 1, n // 2]
+        p = gen(np.random.default_rng(8), extent, "raster", list(data))  # data-first, utils.jl:186-201
+        order = [data[i] for i in np.random.default_rng(8).permutation(len(data))]
+        assert p[:3] == order and p[3:] == _l1_ring_order(extent, order)
+    # runtests.jl:131-133: the data tiles come first
+    p = gen(np.random.default_rng(123), (10, 10, 10), "raster", [0, 999])
+    assert p[:2] in ([0, 999], [999, 0]) and len(p) == 1000
+
+
+def test_dilation_rings_are_diamond_shells():
+    # one dilation of a single voxel adds exactly its 2N face neighbours
+    g = np.zeros((5, 5, 5), dtype=bool)
+    g[2, 2, 2] = True
+    d = prep._dilate(g)
+    assert d.sum() == 7 and d[1, 2, 2] and d[2, 3, 2] and not d[1, 1, 2]
+
+
+def test_findskipped_float32_nan_is_inactive():
+    # hard values of any floating type: NaN marks an inactive voxel
+    # (utils.jl:131-156 tests isnan); np.float32 NaN behaves as a float NaN
+    g = prep.geometry((30, 30), (4, 4), (16, 16), (0.25, 0.25))
+    for impl in (prep.findskipped, iqref.findskipped):
+        want = impl({(i, j): float("nan") for i in range(4) for j in range(4)}, g)
+        got = impl({(i, j): np.float32("nan") for i in range(4) for j in range(4)}, g)
+        assert got == want and 0 in got[0] and got[1] == []
+    assert prep.isnan_value(np.float32("nan")) and prep.isnan_value(float("nan"))
+    assert not prep.isnan_value(np.int64(3)) and not prep.isnan_value(None)
 
 
 def test_pad_symmetric_edge_repeats():
@@ -174,3 +217,130 @@ def test_native_finddisabled_matches_oracle():
     obj = np.array([[1, None, 3, 4], [5, 6, 7, 8], [1, 2, 3, 4]], dtype=object)
     geo = iqref.geometry(obj.shape, (2, 2), obj.shape, (0.5, 0.5))
     assert np.array_equal(iqhip.api.finddisabled_native(obj, (2, 2)).astype(bool), iqref.finddisabled(obj, geo))
+
+
+def _binding_pad_functions():
+    """The padding function the Julia binding applies to each extent of the
+    plan descriptor (read from the ccall site of IQHip.jl's iqsim_hip)."""
+    src = open(os.path.join(ROOT, "imagequilting.jl_amd", "julia", "IQHip.jl")).read()
+    call = src[src.index("desc = PlanDesc("):]
+    call = call[:call.index("pointer(disabled)")]
+    fns = re.findall(r"(pad3\w*)\(", call)
+    defs = dict(re.findall(r"^(pad3\w*)\(t::NTuple\{N,Int\}\) where \{N\} = ntuple\(i -> i <= N \? Int64\(t\[i\]\) : "
+                           r"Int64\((\d)\), 3\)", src, re.M))
+    return [int(defs[f]) for f in fns]  # fill value of each padded field, in PlanDesc order
+
+
+def _desc_like_binding(fills, T, tile, sim, ov):
+    import ctypes
+    geo = prep.geometry(T.shape, tile, sim, ov)
+    fields = [T.shape, tile, geo["ovlsize"], geo["spacing"], geo["ntiles"], geo["padsize"]]
+    pad = lambda t, f: list(t) + [f] * (3 - len(t))
+    d = iqhip._lib.PlanDesc()
+    d.ti = T.ctypes.data
+    d.ti_dtype = iqhip._lib.IQ_F64
+    for name, vals, f in zip(("tisize", "tilesize", "ovlsize", "spacing", "ntiles", "padsize"), fields, fills):
+        getattr(d, name)[:] = pad(vals, f)
+    nt = int(np.prod(geo["ntiles"]))
+    keep = [np.zeros(int(np.prod(geo["distsize"])), np.uint8), np.arange(nt, dtype=np.int64), np.zeros(nt, np.uint8)]
+    d.disabled, d.path, d.skipped = (a.ctypes.data for a in keep)
+    d.npath, d.naux, d.tol, d.nreal = nt, 0, 0.1, 1
+    return d, keep
+
+
+def test_julia_binding_descriptor_passes_plan_checks():
+    # a 2-D call through IQHip.jl: trailing axis one voxel thick, no overlap
+    import ctypes
+    L = iqhip._lib.load()
+    fills = _binding_pad_functions()
+    assert fills == [1, 1, 0, 1, 1, 1]  # tisize, tilesize, ovlsize, spacing, ntiles, padsize
+    T = np.zeros((40, 30))
+    for tile, sim, ov in (((10, 10), (50, 45), (1 / 6, 1 / 6)), ((30, 30), (40, 30), (0.3, 0.5))):
+        d, keep = _desc_like_binding(fills, T, tile, sim, ov)
+        assert L.iq_plan_validate(ctypes.byref(d), 1) == 0, L.iq_last_error(None)
+        # padding the overlap with 1 (one voxel of overlap on a one-voxel
+        # axis) breaks spacing = tilesize - ovlsize there
+        bad, keep2 = _desc_like_binding([1] * 6, T, tile, sim, ov)
+        assert L.iq_plan_validate(ctypes.byref(bad), 1) == -1
+        assert b"spacing" in L.iq_last_error(None)
+
+
+# --------------------------------------- the MFMA scan's interval bound
+def _bf16_rne(x):
+    """Round FP32 values to bfloat16 (round to nearest even), as FP32."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return r.astype(np.uint32).view(np.float32)
+
+
+def _scan_ab32(a, b, split, order):
+    """The scan's FP32 correlation, emulated: split parts (hi, lo) of each
+    FP32 value, products hi*hi + hi*lo + lo*hi (exact in FP32), summed in
+    FP32 one term at a time in the given order."""
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    ah, bh = _bf16_rne(a), _bf16_rne(b)
+    if split:
+        al, bl = _bf16_rne(a - ah), _bf16_rne(b - bh)
+        terms = np.concatenate([ah.astype(np.float64) * bh, ah.astype(np.float64) * bl, al.astype(np.float64) * bh])
+    else:
+        terms = ah.astype(np.float64) * bh
+    assert np.array_equal(terms.astype(np.float32).astype(np.float64), terms)  # products exact in FP32
+    terms = terms.astype(np.float32)
+    if order == "ascending":
+        terms = terms[np.argsort(np.abs(terms))]
+    elif order == "descending":
+        terms = terms[np.argsort(-np.abs(terms))]
+    acc = np.float32(0.0)
+    for t in terms:
+        acc = np.float32(acc + t)
+    return float(acc)
+
+
+def _adversarial_f32(n, rng, sign=1.0):
+    """FP32 values whose bf16 split drops the most: t - hi just below half a
+    bf16 ulp (hi rounds down) and t - hi - lo just below half an ulp of lo
+    (lo rounds down), so every dropped part has the sign of t."""
+    mant = (rng.integers(0, 2, n).astype(np.uint32) << 16) | np.uint32(0x7FBF)
+    expo = rng.integers(120, 134, n).astype(np.uint32) << 23
+    return (sign * (expo | mant).view(np.float32)).astype(np.float32)
+
+
+def test_scan_bound_covers_the_bf16_split():
+    from fractions import Fraction
+    import ctypes
+    L = iqhip._lib.load()
+    rng = np.random.default_rng(60)
+
+    def bound(nstep, split):
+        g = ctypes.c_double(0.0)
+        assert L.iq_scan_error_bound(nstep, int(split), ctypes.byref(g)) == 0
+        return g.value
+
+    worst = {}
+    for nstep in (1, 2, 3, 4):
+        V = 16 * nstep
+        for kind in ("adversarial", "random"):
+            for trial in range(6):
+                if kind == "adversarial":
+                    a = _adversarial_f32(V, rng)
+                    b = a.copy() if trial % 2 == 0 else _adversarial_f32(V, rng)
+                else:
+                    a = np.abs(rng.standard_normal(V)).astype(np.float32)
+                    b = np.abs(rng.standard_normal(V)).astype(np.float32)
+                exact = sum(Fraction(float(x)) * Fraction(float(y)) for x, y in zip(a, b))
+                mag = float(sum(abs(Fraction(float(x)) * Fraction(float(y))) for x, y in zip(a, b)))
+                for split in (True, False):
+                    if not split:
+                        a, b = _bf16_rne(a), _bf16_rne(b)  # bf16-exact data take one product
+                        exact = sum(Fraction(float(x)) * Fraction(float(y)) for x, y in zip(a, b))
+                        mag = float(abs(exact))
+                    for order in ("sequential", "ascending", "descending"):
+                        err = abs(float(Fraction(_scan_ab32(a, b, split, order)) - exact)) / mag
+                        assert err <= bound(nstep, split), (nstep, kind, split, order, err)
+                        worst[(nstep, split)] = max(worst.get((nstep, split), 0.0), err)
+    # the split's dropped terms reach about 2 * 2^-16 of sum|a||b|: a budget
+    # of 4 * 2^-18 for them would not cover a one-k-step mask
+    old_budget = bound(1, True) - 2.0 ** -15 * (1 + 2.0 ** -19) + 4 * 2.0 ** -18
+    assert worst[(1, True)] > old_budget
+    assert worst[(1, True)] > 1.9 * 2.0 ** -16

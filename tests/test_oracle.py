@@ -349,3 +349,193 @@ def test_genpath_lengths_and_data_first():
         assert len(iqref.genpath(np.random.default_rng(123), (10, 10, 10), kind, [])) == 1000  # runtests.jl:124-128
     p = iqref.genpath(np.random.default_rng(123), (10, 10, 10), "raster", [0, 999])  # runtests.jl:131-133
     assert p[:2] in ([0, 999], [999, 0]) and sorted(p) == list(range(1000))
+
+
+# ------------------------ composition of the tile loop (iqsim.jl:172-285)
+# Per-step checks of a logged oracle simulation, derived from the reference's
+# rules and not from the oracle's code: the overlap mask and the cut slabs
+# follow from which neighbours were pasted before the step (iqsim.jl:188-205,
+# 252-275); a cut keeps its first slice along the cut axis and hands its last
+# slice to the training image (source/sink terminals, graphcut.jl:57-81), and
+# the next-side slabs are negated (iqsim.jl:273); outside every slab the
+# chosen window is pasted (iqsim.jl:278); the pick lies in the tolerance band
+# of distances recomputed here (iqsim.jl:237).
+def _sliding_sq_dist(TI, K, M, disabled):
+    """D[c] = sum over mask voxels of (TI[c + v] - K[v])^2, FP64 numpy."""
+    ts = K.shape
+    ds = tuple(a - b + 1 for a, b in zip(TI.shape, ts))
+    D = np.zeros(ds)
+    for v in zip(*np.nonzero(M)):
+        win = TI[tuple(slice(o, o + n) for o, n in zip(v, ds))]
+        D += (win - K[v]) ** 2
+    D[disabled] = np.inf
+    return D
+
+
+def _composition_violations(T, tr, tol, hardtiles=()):
+    geo = tr["geo"]
+    TI = iqref.prepare(T).astype(np.float64)
+    N = TI.ndim
+    ts, ov, sp, nt, pad = geo["tilesize"], geo["ovlsize"], geo["spacing"], geo["ntiles"], geo["padsize"]
+    ds = geo["distsize"]
+    disabled = iqref.finddisabled(T, geo)
+    errs = []
+    grid = np.zeros(pad)
+    pasted = set()
+    for s, ind in enumerate(tr["path"]):
+        c = int(tr["chosen"][0, s])
+        if c < 0:
+            continue
+        t3 = np.unravel_index(ind, nt, order="F")
+        org = [t3[d] * sp[d] for d in range(N)]
+        region = tuple(slice(o, o + n) for o, n in zip(org, ts))
+        before, cut = tr["before"][0, s], tr["cutmask"][0, s]
+        if not np.array_equal(before, grid[region]):
+            errs.append((s, "grid before the step"))
+        slabs = []
+        for d in range(N):
+            if ov[d] <= 1:
+                continue
+            for side, step in ((0, -1), (1, 1)):
+                nb = list(t3)
+                nb[d] += step
+                if 0 <= nb[d] < nt[d] and np.ravel_multi_index(nb, nt, order="F") in pasted:
+                    lo, hi = (0, ov[d]) if side == 0 else (sp[d], ts[d])
+                    sl = np.zeros(ts, dtype=bool)
+                    sl[(slice(None),) * d + (slice(lo, hi),)] = True
+                    slabs.append((d, side, lo, hi, sl))
+        union = np.zeros(ts, dtype=bool)
+        for *_, sl in slabs:
+            union |= sl
+        if cut[~union].any():
+            errs.append((s, "cut outside the slabs"))
+        for k, (d, side, lo, hi, sl) in enumerate(slabs):
+            others = np.zeros(ts, dtype=bool)
+            for j, (*_, o) in enumerate(slabs):
+                if j != k:
+                    others |= o
+            first = (slice(None),) * d + (lo,)
+            last = (slice(None),) * d + (hi - 1,)
+            keep_at, take_at = (first, last) if side == 0 else (last, first)
+            if not cut[keep_at].all():
+                errs.append((s, f"slab axis {d} side {side}: grid not kept at its grid-side face"))
+            if cut[take_at][~others[take_at]].any():
+                errs.append((s, f"slab axis {d} side {side}: training image not taken at its far face"))
+        cs = np.unravel_index(c, ds, order="F")
+        W = TI[tuple(slice(o, o + n) for o, n in zip(cs, ts))]
+        if ind not in hardtiles:
+            D = _sliding_sq_dist(TI, before, union, disabled)
+            dmin = D.min()
+            if not D.ravel(order="F")[c] <= (1 + tol) * dmin * (1 + 1e-9) + 1e-12:
+                errs.append((s, "pick outside the tolerance band"))
+        grid[region] = np.where(cut, before, W)
+        pasted.add(ind)
+    return errs, grid
+
+
+def _logged(T, tile, sim, seed=0, **kw):
+    return iqref.iqsim(T, tile, sim, seed=seed, return_trace=True, step_log=True, **kw)
+
+
+_COMPOSITION_CASES = [
+    ("random 3-D", dict(tile=(8, 8, 4), sim=(24, 24, 10), shape=(26, 24, 10), path="random", ov=(0.25, 0.25, 0.5))),
+    ("dilation 2-D", dict(tile=(10, 10), sim=(50, 45), shape=(40, 40), path="dilation", ov=(0.3, 0.3))),
+    ("data-first 2-D", dict(tile=(10, 10), sim=(50, 50), shape=(40, 40), path="raster", ov=(0.3, 0.3), hard=True)),
+]
+
+
+def _composition_case(name, mutate=0):
+    cfg = dict(_COMPOSITION_CASES)[name]
+    rng = np.random.default_rng(50)
+    x = rng.standard_normal(cfg["shape"])
+    for ax in range(x.ndim):
+        x = np.apply_along_axis(lambda v: np.convolve(v, np.ones(3) / 3, mode="same"), ax, x)
+    hard, hardtiles = {}, set()
+    if cfg.get("hard"):
+        hard = {(int(i), int(j)): float(x[i % cfg["shape"][0], j % cfg["shape"][1]])
+                for i, j in zip(rng.integers(0, cfg["sim"][0], 4), rng.integers(0, cfg["sim"][1], 4))}
+        geo = iqref.geometry(x.shape, cfg["tile"], cfg["sim"], cfg["ov"])
+        hardtiles = set(iqref.findskipped(hard, geo)[1])
+    res, tr = _logged(x, cfg["tile"], cfg["sim"], overlap=cfg["ov"], path=cfg["path"], tol=0.1, hard=hard,
+                      seed=3, debug=True, mutate=mutate)
+    return x, res, tr, hardtiles, hard
+
+
+@pytest.mark.parametrize("name", [n for n, _ in _COMPOSITION_CASES])
+def test_tile_loop_composition(name):
+    x, (reals, cuts, _), tr, hardtiles, hard = _composition_case(name)
+    errs, grid = _composition_violations(x, tr, 0.1, hardtiles)
+    assert not errs, errs[:5]
+    # the replayed pastes give the oracle's output (hard data overwrites
+    # after the loop, iqsim.jl:291-294)
+    geo = tr["geo"]
+    crop = tuple(slice(0, n) for n in geo["simsize"])
+    out = np.asarray(reals[0], dtype=np.float64)
+    keep = np.ones(out.shape, dtype=bool)
+    for k in hard:
+        keep[k] = False
+    assert np.array_equal(grid[crop][keep], out[keep])
+    # next-side slabs occur, so the negation is exercised
+    nexts = 0
+    nt, pasted = geo["ntiles"], set()
+    for s_, ind in enumerate(tr["path"]):
+        t3 = np.unravel_index(ind, nt, order="F")
+        for d in range(len(nt)):
+            nb = list(t3)
+            nb[d] += 1
+            if geo["ovlsize"][d] > 1 and nb[d] < nt[d] and np.ravel_multi_index(nb, nt, order="F") in pasted:
+                nexts += 1
+        pasted.add(ind)
+    assert nexts > 0
+
+
+@pytest.mark.parametrize("mutate", [1, 2])
+@pytest.mark.parametrize("name", ["random 3-D", "dilation 2-D"])
+def test_composition_checks_catch_planted_mistakes(name, mutate):
+    # a next slab left un-negated, or a prev slab one voxel off, must be caught
+    x, _, tr, hardtiles, _ = _composition_case(name, mutate=mutate)
+    errs, _ = _composition_violations(x, tr, 0.1, hardtiles)
+    assert errs
+
+
+def test_two_tile_golden_case():
+    # Derived by hand.  Training image: one row [0 1 2 3 10 11 12 13] (8 x 1);
+    # tile 4 x 1, overlap 1/2: two voxels along x, none along y; grid 4 x 1
+    # -> two tiles, at x = 0..3 (tile 0) and x = 2..5 (tile 1); padsize 6.
+    # Path: tile 1 first, then tile 0.
+    T = np.array([[0.0], [1.0], [2.0], [3.0], [10.0], [11.0], [12.0], [13.0]])
+    kw = dict(overlap=(0.5, 0.5), tol=0.1, simpath=[1, 0])
+    # Step 1 (tile 1, nothing pasted): empty mask, D == 0, every one of the
+    # five candidates is in the band; the forced pick 0 pastes [0 1 2 3] at
+    # x = 2..5.
+    # Step 2 (tile 0, its next neighbour pasted): the overlap is the next slab
+    # x = 2..3 holding [0 1]; D[c] = T[c+2]^2 + (T[c+3] - 1)^2.
+    D = [T[c + 2, 0] ** 2 + (T[c + 3, 0] - 1.0) ** 2 for c in range(5)]
+    assert D == [8.0, 90.0, 200.0, 242.0, 288.0]
+    # min 8, threshold 1.1 * 8: the band is {0}; the window is [0 1 2 3].
+    # The slab's cut (two slices, no interior node): its first slice x = 2
+    # is the source side, kept by the cut and negated for a next slab
+    # (iqsim.jl:273), so it takes the window's 2; its last slice x = 3 keeps
+    # the grid's 1.  Outside the slab the window is pasted: x = 0, 1 -> 0, 1.
+    (reals, cuts, _), tr = iqref.iqsim(T, (4, 1), (4, 1), forced=[[0, 0]], debug=True, return_trace=True,
+                                       step_log=True, **kw)
+    assert list(tr["chosen"][0]) == [0, 0]
+    assert tr["before"][0, 1][:, 0].tolist() == [0.0, 0.0, 0.0, 1.0]
+    assert tr["cutmask"][0, 1][:, 0].tolist() == [False, False, False, True]
+    assert reals[0][:, 0].tolist() == [0.0, 1.0, 2.0, 1.0]
+    assert cuts[0][:, 0].tolist() == [0.0, 0.0, 0.0, 1.0]
+    # with the next slab left un-negated the faces swap: [0 1 0 3]
+    bad = iqref.iqsim(T, (4, 1), (4, 1), forced=[[0, 0]], mutate=1, **kw)
+    assert bad[0][:, 0].tolist() == [0.0, 1.0, 0.0, 3.0]
+
+
+# ------------------------------------------ every candidate disabled
+def test_band_of_all_infinite_distances_is_everything():
+    # iqsim.jl:237 with minimum(D) = Inf: D .<= Inf holds for every entry
+    assert list(iqref.band(np.full(7, np.inf), 0.1)) == list(range(7))
+
+
+def test_relaxation_of_all_infinite_distances_is_refused():
+    # relaxation.jl:7-36 never terminates when no distance is finite
+    with pytest.raises(RuntimeError, match="-3"):
+        iqref.relaxation(np.full(9, np.inf), [np.full(9, np.inf)], 0.1)
