@@ -1,24 +1,39 @@
-"""Benchmark: simulated voxels/s of the plain GPU full scan on one MI355X.
+"""Benchmark: simulated voxels/s on one MI355X, learned + batched against the
+plain GPU full scan (the headline metric), or the plain full scan alone.
 
-Workload (the plain-GPU configuration of the metric): a seeded Float32
-Gaussian random field 100x100x50 (exponential covariance, ranges 20,20,5),
-tile 20x20x5, overlap 1/5 per axis, output 200x200x50, tol 0.1, nreal 1,
-raster path, no soft or hard data.  One step = one complete simulation (all
-2,197 tile steps: distance scan, band pick, tau weights, draw, min cuts,
-paste) with the training image already resident in HBM.
+Default workload (the learned-proposal configuration): a seeded Float32
+Gaussian random field 256x256x64 (exponential covariance, ranges 40,40,8),
+tile 32x32x8, overlap 1/4, output 384x384x64, tol 0.1, nreal 16, raster path,
+2,885,625 candidate patterns.  One plan holds the training image resident in
+HBM; all 16 realizations advance in lockstep (batched).  Two legs run on the
+same plan with the same uniforms:
+  full scan   every tile step scans all candidates exactly (MFMA scan with an
+              interval bound, FP64 recheck of the band superset);
+  learned     the shortlist of the K best approximate distances (bank scan of
+              64-d bf16 encodings) replaces the full scan, reranked in FP64;
+              every check_every-th batch also runs the full scan, which decides
+              that batch and measures recall on its exact band.
+One step = one complete simulation of all realizations (every tile step:
+distances, band, weights, draw, min cuts, paste) plus the download of the
+output grids to the host.  value = learned voxels/s; the full scan's voxels/s,
+the ratio and the recall go in `config.learned_proposal`.  Both legs are timed
+the same way (no instrumentation); one further run of each, with HIP events
+around the kernels, gives the roofline and kernel shares.
 
-Also reported:
-  roofline      the MFMA distance scan k_scan_mfma (dominant kernel), timed
-                live with HIP events on the library's stream around every scan
-                launch during the timed steps; algorithmic FLOPs = 2 *
-                candidates * masked voxels per query, summed over the queries
-                of a launch.  `traffic` (HBM bytes per launch) comes from the
-                rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/, scaled
-                by the same counters on a copy of known size (--copy-mb).
-  cpu_baseline  the FP64 CPU restatement (oracle/, OpenMP) on a bounded sample
-                of the same workload: the first tile steps of one realization.
+  roofline            k_learn_mscan (the learned leg's bank scan, HBM-bound):
+                      algorithmic bytes (bank rows + A2 maps of each mask shape
+                      a launch scans) / its HIP-event time; `traffic` from the
+                      rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/.
+  roofline_full_scan  k_scan_mfma (the full scan): algorithmic FLOPs
+                      (2 * candidates * masked voxels per query) / its time,
+                      against the FP32 and the bf16 matrix peaks.
+  cpu_baseline        the FP64 CPU restatement (oracle/, OpenMP) on the first
+                      tile steps of one realization of the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-sample N]
+--workload 2 gives the plain line (3-D Float32 GRF 100x100x50 -> 200x200x50,
+tile 20x20x5, full scan only); 1, 3, 4 time the other workloads the same way.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload N]
        python bench.py --copy-mb M    (device copy of M MiB only: counter calibration)
 """
 import argparse
@@ -33,8 +48,11 @@ sys.path.insert(0, os.path.join(ROOT, "imagequilting.jl_amd"))
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix peak (published)
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense bf16 MFMA peak (published)
 HBM_GBS = 8000.0
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "scan_mfma_grf3d_100x100x50_traffic.md")
+TRAFFIC = {2: os.path.join(ROOT, "profiles", "scan_mfma_grf3d_100x100x50_traffic.md"),
+           5: os.path.join(ROOT, "profiles", "bank_scan_grf3d_256x256x64_traffic.md")}
 
+METRIC_LEARNED = "simulated voxels/s on 1 MI355X: learned+batched vs full scan at ≥99% recall"
+METRIC_PLAIN = "simulated voxels/s on 1 MI355X: plain GPU full scan"
 
 WORKLOADS = {
     1: "2-D Float64 binary GRF 200x200 (range 20, 30% ones) -> 400x400, tile 30x30, overlap 1/6, tol 0.1, nreal 1",
@@ -44,20 +62,20 @@ WORKLOADS = {
        "overlap 1/4, tol 0.1, one plan, lockstep waves",
     4: "soft + hard: 2-D Float32 GRF 512x512 (range 32), box-blurred soft pair, 500 hard points, -> 1024x1024, "
        "tile 48x48, overlap 1/6, tol 0.05, nreal 32",
-    5: "3-D Float32 GRF 256x256x64 (ranges 40,40,8) -> 384x384x64, tile 32x32x8, overlap 1/4, tol 0.1, nreal 16, "
-       "full scan (no learned proposal)",
+    5: "learned + batched vs full scan: 3-D Float32 GRF 256x256x64 (ranges 40,40,8) -> 384x384x64, tile 32x32x8, "
+       "overlap 1/4, tol 0.1, nreal 16 in lockstep, 2,885,625 candidates",
 }
 
 
-def measured_traffic():
-    """HBM bytes per k_scan_mfma launch from the committed counter summary
-    (line 'traffic_bytes_per_launch: <bytes>'), or None."""
+def measured_traffic(workload):
+    """HBM bytes per launch of the roofline kernel from its committed counter
+    summary (line 'traffic_bytes_per_launch: <bytes>'), or None."""
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        with open(TRAFFIC[workload]) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
-    except OSError:
+    except (OSError, KeyError):
         pass
     return None
 
@@ -77,25 +95,90 @@ def copy_calibration(mib: int):
 
 
 def cpu_baseline(w, sample_steps: int):
-    """Time the oracle on the first `sample_steps` tile steps (one realization)."""
+    """Time the oracle on the first `sample_steps` raster tile steps of one realization."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import iqref
 
     iqref.build()
     T = w["trainimg"]
-    geo = iqref.geometry(T.shape, w["tilesize"], w["simsize"], w["overlap"])
+    sim = w["simsize"] if "simsize" in w else w["simsizes"][0]
+    geo = iqref.geometry(T.shape, w["tilesize"], sim, w["overlap"])
     nsteps = int(np.prod(geo["ntiles"]))
-    # a sub-grid covering the first `sample_steps` raster tiles along x/y rows
+    n = min(sample_steps, nsteps)
     t0 = time.perf_counter()
-    iqref.iqsim(T, w["tilesize"], w["simsize"], overlap=w["overlap"], tol=w["tol"], nreal=1, seed=1,
-                max_steps=sample_steps)
+    iqref.iqsim(T, w["tilesize"], sim, overlap=w["overlap"], tol=w["tol"], nreal=1, seed=1, max_steps=n,
+                soft=w.get("soft", ()), hard=w.get("hard"))
     dt = time.perf_counter() - t0
-    vox_per_step = float(np.prod(w["simsize"])) / nsteps
-    return dict(value=vox_per_step * min(sample_steps, nsteps) / dt, unit="voxels/s", cores=iqref.num_threads(),
-                kind="port",
-                sample=f"first {min(sample_steps, nsteps)} of {nsteps} raster tile steps of one realization "
-                       f"({dt:.1f} s, FP64 direct sums, OpenMP over candidates)")
+    vox_per_step = float(np.prod(sim)) / nsteps
+    return dict(value=vox_per_step * n / dt, unit="voxels/s", cores=iqref.num_threads(), kind="port",
+                sample=f"first {n} of {nsteps} tile steps of one realization ({dt:.1f} s; FP64 direct sums, "
+                       f"OpenMP over candidates; voxels = output voxels per tile step x steps)")
+
+
+def timed_leg(plan, U, warmup, steps, buf):
+    """Wall time of `steps` complete simulations, each with the download of
+    the output grids, after `warmup` untimed ones; also the device time."""
+    lib = plan.ctx._L
+    for i in range(warmup):
+        plan.run(U[i])
+        plan.results(grids=True, chosen=False, out=buf)
+    lib.iq_ctx_synchronize(plan.ctx.handle)
+    dev = 0.0
+    t0 = time.perf_counter()
+    for i in range(steps):
+        plan.run(U[warmup + i])  # returns once the stream has drained
+        dev += plan.last_time()
+        plan.results(grids=True, chosen=False, out=buf)
+    lib.iq_ctx_synchronize(plan.ctx.handle)
+    return time.perf_counter() - t0, dev
+
+
+def profiled_run(plan, u):
+    """One more run with HIP events around the scan, bank-scan and cut launches."""
+    plan.profile(True)
+    plan.run(u)
+    dev = plan.last_time()
+    scan = plan.scan_stats()
+    cut = plan.cut_time()
+    bank = plan.bank_stats()
+    rounds, levels, ncuts = plan.cut_stats()
+    nsel, nsuper, nband = plan.select_stats()
+    plan.profile(False)
+    return dict(dev=dev, scan=scan, cut=cut, bank=bank, cutwork=(rounds, levels, ncuts), sel=(nsel, nsuper, nband))
+
+
+def scan_roofline(p, bf16_exact, workload):
+    launches, sec, flop, mflop = p["scan"]
+    achieved = flop / sec / 1e12 if sec > 0 else 0.0
+    peak = PEAK_BF16_TFLOPS if bf16_exact else PEAK_FP32_TFLOPS
+    return {
+        "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+        "traffic": measured_traffic(workload) if workload == 2 else None,
+        "kernel": "k_scan_mfma", "launches": launches, "avg_launch_us": sec / max(launches, 1) * 1e6,
+        "flop_per_launch": flop / max(launches, 1),
+        "frac_of_fp32_peak": achieved / PEAK_FP32_TFLOPS,
+        "mfma_bf16_issued_tflops": mflop / sec / 1e12 if sec > 0 else 0.0,
+        "mfma_bf16_util": (mflop / sec / 1e12 / PEAK_BF16_TFLOPS) if sec > 0 else 0.0,
+    }
+
+
+def kernel_shares(p):
+    cut_n, cut_s = p["cut"]
+    rounds, levels, ncuts = p["cutwork"]
+    nsel, nsuper, nband = p["sel"]
+    dev = max(p["dev"], 1e-12)
+    return {
+        "device_ms": p["dev"] * 1e3,
+        "mincut": {"cuts": ncuts, "push_rounds_per_cut": rounds / max(ncuts, 1),
+                   "relabel_passes_per_cut": levels / max(ncuts, 1), "avg_launch_us": cut_s / max(cut_n, 1) * 1e6,
+                   "share_of_device_time": cut_s / dev,
+                   "bound": "latency (dependent LDS accesses and barriers per push-relabel round)"},
+        "scan_share_of_device_time": p["scan"][1] / dev,
+        "bank_scan_share_of_device_time": p["bank"][1] / dev,
+        "selection": {"queries": nsel, "superset_per_query": nsuper / max(nsel, 1),
+                      "band_per_query": nband / max(nsel, 1)},
+    }
 
 
 def main():
@@ -103,16 +186,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=2197,
-                    help="tile steps timed on the CPU (0 = skip); 2197 = the whole simulation")
+    ap.add_argument("--cpu-sample", type=int, default=-1,
+                    help="tile steps timed on the CPU oracle (0 = skip; default: 16 at workload 5, the whole "
+                         "simulation at workload 2, else skip)")
     ap.add_argument("--copy-mb", type=int, default=0, help="only run the copy used to calibrate HBM counters")
-    ap.add_argument("--workload", type=int, default=2, choices=sorted(WORKLOADS),
-                    help="2 (default): the plain full-scan workload; 1, 3, 4, 5: the other workloads")
-    ap.add_argument("--learned", action="store_true",
-                    help="also time the learned proposal (shortlist + exact rerank) on the same plan and report "
-                         "it as the value, with the full scan beside it")
-    ap.add_argument("--K", type=int, default=4096, help="shortlist size of the learned proposal")
+    ap.add_argument("--workload", type=int, default=5, choices=sorted(WORKLOADS),
+                    help="5 (default): learned + batched vs the full scan; 1-4: the full scan alone")
+    ap.add_argument("--K", type=int, default=2048, help="shortlist size of the learned proposal")
     ap.add_argument("--check-every", type=int, default=16, help="full-scan recall check every N-th batch")
+    ap.add_argument("--full-steps", type=int, default=-1, help="timed steps of the full-scan leg (default: --steps)")
     args = ap.parse_args()
     if args.copy_mb > 0:
         copy_calibration(args.copy_mb)
@@ -126,9 +208,11 @@ def main():
     import iqhip
     from iqhip import synth
 
+    learned = args.workload == 5
     w = synth.workload(args.workload)
     ctx = iqhip.default_context()
     rng = np.random.default_rng(1)
+    t0 = time.perf_counter()
     if "simsizes" in w:  # batched realizations, one grid each
         simsizes = w["simsizes"]
         plan, infos = iqhip.make_plan_multi(w["trainimg"], w["tilesize"], simsizes, overlap=w["overlap"],
@@ -137,116 +221,85 @@ def main():
         simsizes = [w["simsize"]] * w["nreal"]
         plan, info = iqhip.make_plan(w["trainimg"], w["tilesize"], w["simsize"], overlap=w["overlap"], tol=w["tol"],
                                      nreal=w["nreal"], soft=w.get("soft", ()), hard=w.get("hard"), rng=rng, ctx=ctx)
-        infos = [info]
-    U = [rng.random(sum(plan.real_npath)) for _ in range(args.warmup + args.steps)]
-    for i in range(args.warmup):
-        plan.run(U[i])
-    plan.profile(True)
-    lib = ctx._L
-    lib.iq_ctx_synchronize(ctx.handle)
-    t0 = time.perf_counter()
-    dev_time = 0.0
-    launches, scan_sec, scan_flop, mfma_flop = 0, 0.0, 0.0, 0.0
-    cut_launches, cut_sec = 0, 0.0
-    for i in range(args.steps):
-        plan.run(U[args.warmup + i])  # returns after the stream has drained
-        dev_time += plan.last_time()
-        n, s, f, mf = plan.scan_stats()
-        launches, scan_sec, scan_flop, mfma_flop = launches + n, scan_sec + s, scan_flop + f, mfma_flop + mf
-        cn, cs = plan.cut_time()
-        cut_launches, cut_sec = cut_launches + cn, cut_sec + cs
-    lib.iq_ctx_synchronize(ctx.handle)
-    elapsed = time.perf_counter() - t0
-    cut_rounds, cut_levels, ncuts = plan.cut_stats()
-    nsel, nsuper, nband = plan.select_stats()
-    tune_nw, tune_us = plan.scan_tuning()
-    voxels = float(sum(np.prod(z) for z in simsizes)) * args.steps
-    achieved = scan_flop / scan_sec / 1e12 if scan_sec > 0 else 0.0
-    # training images whose values are exact in bf16 (facies codes, binary)
-    # are scanned with one bf16 product: price them against the bf16 peak
+    plan_s = time.perf_counter() - t0
+    full_steps = args.steps if args.full_steps < 0 else args.full_steps
+    nU = args.warmup + max(args.steps, full_steps)
+    U = [rng.random(sum(plan.real_npath)) for _ in range(nU)]
+    buf = np.zeros(sum(int(np.prod(p)) for p in plan.real_padsize), dtype=plan.dtype)
+    voxels = float(sum(np.prod(z) for z in simsizes))
     t32 = np.asarray(w["trainimg"], dtype=np.float32)
     bf16_exact = bool(((np.ascontiguousarray(t32).view(np.uint32) & 0xFFFF) == 0).all())
-    peak = PEAK_BF16_TFLOPS if bf16_exact else PEAK_FP32_TFLOPS
+    dtype = "f32" if plan.dtype == np.float32 else "f64"
+
+    # full scan
+    f_el, f_dev = timed_leg(plan, U, args.warmup, full_steps, buf)
+    full_value = voxels * full_steps / f_el
+    f_prof = profiled_run(plan, U[0])
+    tune_nw, tune_us = plan.scan_tuning()
     res = {
-        "metric": "simulated voxels/s on 1 MI355X: learned+batched vs full scan at ≥99% recall",
-        "value": voxels / elapsed,
+        "metric": METRIC_PLAIN,
+        "value": full_value,
         "unit": "voxels/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": full_steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": f_el / full_steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if plan.dtype == np.float32 else "f64",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {
             "workload": WORKLOADS[args.workload],
             "tile_steps_per_step": int(sum(plan.real_npath)),
-            "device_ms_per_step": dev_time / args.steps * 1e3,
+            "step": "one complete simulation of every realization + download of the output grids",
+            "plan_create_seconds": plan_s,
+            "device_ms_per_step": f_dev / full_steps * 1e3,
             "learned_proposal": "off",
-            "mincut": {"cuts": ncuts, "push_rounds_per_cut": cut_rounds / max(ncuts, 1),
-                       "relabel_passes_per_cut": cut_levels / max(ncuts, 1),
-                       "avg_launch_us": cut_sec / max(cut_launches, 1) * 1e6,
-                       "share_of_device_time": cut_sec / max(dev_time, 1e-12),
-                       "bound": "latency (dependent LDS accesses and barriers per push-relabel round)"},
-            "scan_share_of_device_time": scan_sec / max(dev_time, 1e-12),
-            "selection": {"queries": nsel, "superset_per_query": nsuper / max(nsel, 1),
-                          "band_per_query": nband / max(nsel, 1)},
+            "full_scan_kernels": kernel_shares(f_prof),
             "scan_tuning": {"wavefronts_per_group": {"1_tile": tune_nw[0], "2+_tiles": tune_nw[1]},
                             "us_per_launch": {f"{t}_tile_{wv}_wavefronts": round(v, 1)
                                               for (t, wv), v in sorted(tune_us.items())}},
         },
-        "roofline": {
-            "bound": "mfma",
-            "achieved": achieved,
-            "peak": peak,
-            "unit": "TFLOP/s",
-            "frac": achieved / peak,
-            "traffic": measured_traffic() if args.workload == 2 else None,
-            "kernel": "k_scan_mfma",
-            "launches": launches,
-            "avg_launch_us": scan_sec / max(launches, 1) * 1e6,
-            "flop_per_launch": scan_flop / max(launches, 1),
-            "mfma_bf16_issued_tflops": mfma_flop / scan_sec / 1e12 if scan_sec > 0 else 0.0,
-            "mfma_bf16_util": (mfma_flop / scan_sec / 1e12 / PEAK_BF16_TFLOPS) if scan_sec > 0 else 0.0,
-        },
+        "roofline": scan_roofline(f_prof, bf16_exact, args.workload),
     }
-    if args.learned:
-        # learned + batched: same plan, same uniforms; the full scan above is the baseline
+    if learned:
         t0 = time.perf_counter()
         plan.learn(nsamples=8192, iters=16, seed=1)
-        lib.iq_ctx_synchronize(ctx.handle)
+        ctx._L.iq_ctx_synchronize(ctx.handle)
         learn_s = time.perf_counter() - t0
         plan.set_proposal(True, K=args.K, check_every=args.check_every, min_recall=0.99)
-        plan.profile(False)
-        for i in range(args.warmup):
-            plan.run(U[i])
-        t0 = time.perf_counter()
-        members = hits = queries = full = 0
-        fell = False
-        ldev = 0.0
-        for i in range(args.steps):
-            plan.run(U[args.warmup + i])
-            ldev += plan.last_time()
-            st = plan.proposal_stats()
-            members += st["band_members"]; hits += st["in_shortlist"]
-            queries += st["queries"]; full += st["fully_recalled"]; fell |= st["fell_back"]
-        lib.iq_ctx_synchronize(ctx.handle)
-        lel = time.perf_counter() - t0
-        full_scan_value = res["value"]
-        res["value"] = voxels / lel
-        res["ms_per_step"] = lel / args.steps * 1e3
+        l_el, l_dev = timed_leg(plan, U, args.warmup, args.steps, buf)
+        st = plan.proposal_stats()  # of the last timed run (every run draws the same way)
+        l_prof = profiled_run(plan, U[0])
+        bank_n, bank_s, bank_b = l_prof["bank"]
+        achieved = bank_b / bank_s / 1e9 if bank_s > 0 else 0.0
+        value = voxels * args.steps / l_el
+        res["metric"] = METRIC_LEARNED
+        res["value"] = value
+        res["steps"] = args.steps
+        res["ms_per_step"] = l_el / args.steps * 1e3
+        res["roofline_full_scan"] = res.pop("roofline")
+        res["roofline"] = {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_GBS, "unit": "GB/s", "frac": achieved / HBM_GBS,
+            "traffic": measured_traffic(5), "kernel": "k_learn_mscan", "launches": bank_n,
+            "avg_launch_us": bank_s / max(bank_n, 1) * 1e6, "bytes_per_launch": bank_b / max(bank_n, 1),
+        }
         res["config"]["learned_proposal"] = {
             "K": args.K, "check_every": args.check_every, "min_recall": 0.99,
-            "recall": hits / members if members else None, "checked_queries": queries,
-            "fully_recalled_queries": full, "fell_back": fell, "learn_seconds": learn_s,
-            "device_ms_per_step": ldev / args.steps * 1e3,
-            "full_scan_value": full_scan_value, "speedup_vs_full_scan": res["value"] / full_scan_value,
+            "recall": st["recall"], "checked_queries": st["queries"], "fully_recalled_queries": st["fully_recalled"],
+            "fell_back": st["fell_back"], "learn_seconds": learn_s,
+            "device_ms_per_step": l_dev / args.steps * 1e3,
+            "full_scan_value": full_value, "full_scan_ms_per_step": f_el / full_steps * 1e3,
+            "speedup_vs_full_scan": value / full_value,
+            "kernels": kernel_shares(l_prof),
         }
     plan.close()
-    if args.cpu_sample > 0 and args.workload == 2:
-        res["cpu_baseline"] = cpu_baseline(w, args.cpu_sample)
+    cs = args.cpu_sample
+    if cs < 0:
+        cs = {5: 16, 2: 2197}.get(args.workload, 0)
+    if cs > 0:
+        res["cpu_baseline"] = cpu_baseline(w, cs)
     print(json.dumps(res), flush=True)
 
 

@@ -372,6 +372,9 @@ struct iq_plan {
   std::vector<hipEvent_t> prof_ev;
   int64_t prof_used = 0;
   std::vector<hipEvent_t> cut_ev;  // min-cut launches (profiling): pairs of events
+  std::vector<hipEvent_t> bank_ev;  // shortlist bank-scan launches (profiling): pairs of events
+  int64_t bank_used = 0, bank_launches = 0;
+  double bank_sec = 0.0, bank_bytes = 0.0, bank_bytes_run = 0.0;
   int64_t cut_used = 0, cut_launches = 0;
   double cut_sec = 0.0;
   double prof_flop = 0.0, prof_sec = 0.0, prof_mflop = 0.0;
@@ -1082,7 +1085,7 @@ enum { SEL_SCAN = 0, SEL_LEARNED = 1, SEL_CHECK = 2 };
 
 template <typename T>
 void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool has_plain, bool has_multi, int sel,
-               const int4* tiles, int ntiles) {
+               const int4* tiles, const int4* htiles, int ntiles) {
   const IqGeo& g = pl->geo;
   hipStream_t s = pl->ctx->stream;
   const IqDev* D = pl->dd;
@@ -1106,8 +1109,27 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     HCL();
     hipLaunchKernelGGL(k_learn_bracket, dim3(nq), dim3(1024), 0, s, D, q);
     HCL();
+    if (pl->prof) {
+      if (pl->bank_used + 2 > (int64_t)pl->bank_ev.size()) {
+        hipEvent_t a, b;
+        HC(hipEventCreate(&a));
+        HC(hipEventCreate(&b));
+        pl->bank_ev.push_back(a);
+        pl->bank_ev.push_back(b);
+      }
+      HC(hipEventRecord(pl->bank_ev[pl->bank_used], s));
+      // algorithmic bytes: each mask shape's bank rows (64 bf16) and its A2
+      // maps (FP64 + FP32) once per launch (tiles of one mask are adjacent)
+      int nmk = 0;
+      for (int t = 0; t < ntiles; t++) nmk += (t == 0 || htiles[t].x != htiles[t - 1].x) ? 1 : 0;
+      pl->bank_bytes_run += (double)nmk * (double)P * (IQ_EMB * 2 + 8 + 4);
+    }
     hipLaunchKernelGGL(k_learn_mscan, dim3(nblk(P, 128)), dim3(256), 0, s, D, q, tiles, ntiles);
     HCL();
+    if (pl->prof) {
+      HC(hipEventRecord(pl->bank_ev[pl->bank_used + 1], s));
+      pl->bank_used += 2;
+    }
     hipLaunchKernelGGL(k_learn_pick, dim3(nq), dim3(1024), 0, s, D, q);
     HCL();
   }
@@ -1238,6 +1260,10 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   pl->cut_used = 0;
   pl->cut_sec = 0.0;
   pl->cut_launches = 0;
+  pl->bank_used = 0;
+  pl->bank_sec = 0.0;
+  pl->bank_launches = 0;
+  pl->bank_bytes_run = 0.0;
   pl->batch_flop.clear();
   // query table of every batch
   std::vector<IqQuery> qs;
@@ -1322,7 +1348,8 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   for (const Batch& b : batches) {
     int sel = SEL_SCAN;
     if (learned && b.plain) sel = (pl->lrn_check > 0 && nplain++ % pl->lrn_check == 0) ? SEL_CHECK : SEL_LEARNED;
-    run_batch<T>(pl, qs.data() + b.off, pl->dq + b.off, b.n, b.plain, b.multi, sel, pl->dt + b.toff, b.nt);
+    run_batch<T>(pl, qs.data() + b.off, pl->dq + b.off, b.n, b.plain, b.multi, sel, pl->dt + b.toff,
+                 tiles.data() + b.toff, b.nt);
     if (sel == SEL_CHECK) {
       // the full scan decided this batch: stop using the shortlist once its
       // cumulative recall on the exact bands falls below the threshold
@@ -1351,6 +1378,13 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
     pl->cut_sec += ms * 1e-3;
     pl->cut_launches++;
   }
+  for (int64_t k = 0; k + 1 < pl->bank_used; k += 2) {
+    float ms = 0.0f;
+    HC(hipEventElapsedTime(&ms, pl->bank_ev[k], pl->bank_ev[k + 1]));
+    pl->bank_sec += ms * 1e-3;
+    pl->bank_launches++;
+  }
+  pl->bank_bytes = pl->bank_bytes_run;
   int32_t code = 0;
   HC(hipMemcpy(&code, H.err, sizeof(int32_t), hipMemcpyDeviceToHost));
   HC(hipMemcpy(pl->cut_stats, H.rounds, 6 * sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -1807,6 +1841,7 @@ int iq_plan_destroy(iq_plan* plan) {
   if (plan->ctx) (void)hipSetDevice(plan->ctx->device);
   for (hipEvent_t e : plan->prof_ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : plan->cut_ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : plan->bank_ev) (void)hipEventDestroy(e);
   if (plan->e0) (void)hipEventDestroy(plan->e0);
   if (plan->e1) (void)hipEventDestroy(plan->e1);
   plan->mem.release();
@@ -1880,6 +1915,14 @@ int iq_plan_scan_stats(iq_plan* plan, int64_t* launches, double* seconds, double
   if (seconds) *seconds = plan->prof_sec;
   if (flop) *flop = plan->prof_flop;
   if (mfma_flop) *mfma_flop = plan->prof_mflop;
+  return IQ_OK;
+}
+
+int iq_plan_bank_stats(iq_plan* plan, int64_t* launches, double* seconds, double* bytes) {
+  if (!plan) return fail(nullptr, IQ_ERR_ARG, "null plan");
+  if (launches) *launches = plan->bank_launches;
+  if (seconds) *seconds = plan->bank_sec;
+  if (bytes) *bytes = plan->bank_bytes;
   return IQ_OK;
 }
 

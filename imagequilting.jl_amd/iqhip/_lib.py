@@ -34,7 +34,7 @@ EXPORTS = (
     "iq_plan_create_multi", "iq_plan_last_time", "iq_plan_profile", "iq_plan_scan_stats", "iq_plan_scan_tuning", "iq_plan_cut_time", "iq_plan_cut_stats",
     "iq_simulate", "iq_prepare", "iq_prepare_soft", "iq_finddisabled",
     "iq_plan_learn", "iq_plan_set_proposal", "iq_plan_proposal_stats", "iq_plan_encoder", "iq_plan_select_stats",
-    "iq_plan_validate", "iq_scan_error_bound",
+    "iq_plan_validate", "iq_scan_error_bound", "iq_plan_bank_stats",
 )
 
 
@@ -97,6 +97,7 @@ def load(path: str = LIB_PATH):
             "iq_plan_create": [vp, vp, P(vp)],
             "iq_plan_create_multi": [vp, vp, c_int, P(vp)],
             "iq_plan_validate": [vp, c_int],
+            "iq_plan_bank_stats": [vp, vp, vp, vp],
             "iq_scan_error_bound": [i64, c_int, P(f64)],
             "iq_plan_destroy": [vp],
             "iq_plan_run": [vp, vp, vp, vp],

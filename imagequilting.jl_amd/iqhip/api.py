@@ -314,6 +314,14 @@ class Plan:
         self.ctx.check(self.ctx._L.iq_plan_cut_time(self.handle, _ptr(n), _ptr(t)))
         return int(n[0]), float(t[0])
 
+    def bank_stats(self):
+        """(launches, seconds, algorithmic bytes) of the learned shortlist's
+        bank scan during the last run (profiling enabled)."""
+        n = np.zeros(1, dtype=np.int64)
+        t = np.zeros(2, dtype=np.float64)
+        self.ctx.check(self.ctx._L.iq_plan_bank_stats(self.handle, _ptr(n), t[0:].ctypes.data, t[1:].ctypes.data))
+        return int(n[0]), float(t[0]), float(t[1])
+
     def select_stats(self):
         """(plain queries, summed FP64-rechecked superset sizes, summed band sizes) of the last run."""
         v = np.zeros(3, dtype=np.int64)
@@ -361,11 +369,17 @@ class Plan:
         self.ctx._L.iq_plan_encoder(self.handle, 0, None, None, None, ctypes.byref(nm))
         return nm.value
 
-    def results(self, grids=True, chosen=True, cuts=False):
-        """Per realization: padded grids, chosen picks (npath), cut grids."""
+    def results(self, grids=True, chosen=True, cuts=False, out=None):
+        """Per realization: padded grids, chosen picks (npath), cut grids.
+        ``out``: a flat array of the plan's element type holding every
+        realization's padded grid, reused instead of a new one."""
         vox = [int(np.prod(p)) for p in self.real_padsize]
         nvox, nlog = sum(vox), sum(self.real_npath)
-        out = np.zeros(max(nvox, 1), dtype=self.dtype) if grids else None
+        if grids and out is not None:
+            if out.dtype != self.dtype or out.size < max(nvox, 1) or not out.flags.c_contiguous:
+                raise ValueError("out must be a contiguous array of the plan's element type and grid size")
+        else:
+            out = np.zeros(max(nvox, 1), dtype=self.dtype) if grids else None
         ch = np.zeros(max(nlog, 1), dtype=np.int64) if chosen else None
         cg = np.zeros(max(nvox, 1), dtype=np.float64) if cuts else None
         self.ctx.check(self.ctx._L.iq_plan_results(self.ctx.handle, self.handle, _ptr(out), _ptr(ch), _ptr(cg)))
@@ -425,23 +439,26 @@ def _validate(trainimg, tilesize, simsize, overlap, soft, hard, tol, path, nreal
 
 
 def make_plan(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard=None, tol=0.1, path="raster",
-              nreal=1, rng=None, ctx: Optional[Context] = None):
+              nreal=1, rng=None, ctx: Optional[Context] = None, simpath=None):
     """Host preparation of iqsim.jl:65-139 and the upload of everything the
-    tile loop reads.  Returns (plan, info)."""
+    tile loop reads.  ``simpath``: tile order to use instead of genpath's
+    (0-based column-major tile indices; may visit a subset of the tiles).
+    Returns (plan, info)."""
     plan, infos = make_plan_multi(trainimg, tilesize, [simsize], overlap=overlap, soft=soft,
                                   hard=None if hard is None else [hard], tol=tol, path=path, nreal=nreal, rng=rng,
-                                  ctx=ctx)
+                                  ctx=ctx, simpaths=None if simpath is None else [simpath])
     return plan, infos[0]
 
 
 def make_plan_multi(trainimg, tilesize, simsizes, *, overlap=None, soft=(), hard=None, tol=0.1, path="raster",
-                    nreal=1, rng=None, ctx: Optional[Context] = None):
+                    nreal=1, rng=None, ctx: Optional[Context] = None, simpaths=None):
     """One plan over several simulation grids (realizations of different
     output sizes advancing in lockstep).  ``simsizes``: one size per grid
     (None = the TI size); ``hard``: None or one dict per grid; ``soft``:
     (data, dataTI) pairs shared by all grids (each padded to its grid);
     ``nreal``: realizations per grid (int or one per grid).  Paths are drawn
-    from ``rng`` grid by grid.  Returns (plan, [info per grid])."""
+    from ``rng`` grid by grid unless ``simpaths`` gives one per grid (None
+    entries are drawn).  Returns (plan, [info per grid])."""
     ctx = ctx or default_context()
     T = trainimg
     tis = np.shape(T)
@@ -480,7 +497,10 @@ def make_plan_multi(trainimg, tilesize, simsizes, *, overlap=None, soft=(), hard
             keep.append(disabled)
         aux = [prepare_soft_native(a, geo["padsize"]) for a, _ in soft]
         skipped, datainds = prep.findskipped(hd, geo)
-        simpath = prep.genpath(rng, geo["ntiles"], path, list(datainds))
+        if simpaths is not None and simpaths[gi] is not None:
+            simpath = [int(t) for t in simpaths[gi]]
+        else:
+            simpath = prep.genpath(rng, geo["ntiles"], path, list(datainds))
         nt = int(np.prod(geo["ntiles"]))
         skp = np.zeros(nt, dtype=np.uint8)
         for s_ in skipped:
@@ -540,11 +560,12 @@ def finish(info, grids, cuts=None):
 
 
 def iqsim_batch(trainimg, tilesize, simsizes, *, overlap=None, soft=(), hard=None, tol=0.1, path="raster", nreal=1,
-                debug=False, rng=None, uniforms=None, ctx: Optional[Context] = None, return_trace=False):
+                debug=False, rng=None, uniforms=None, ctx: Optional[Context] = None, return_trace=False,
+                simpaths=None):
     """iqsim over several output grids at once (one plan, lockstep waves).
     Returns one result per grid, each what ``iqsim`` returns for it."""
     plan, infos = make_plan_multi(trainimg, tilesize, simsizes, overlap=overlap, soft=soft, hard=hard, tol=tol,
-                                  path=path, nreal=nreal, rng=rng, ctx=ctx)
+                                  path=path, nreal=nreal, rng=rng, ctx=ctx, simpaths=simpaths)
     try:
         if uniforms is None:
             uniforms = [infos[0]["rng"].random(n) for n in plan.real_npath]
@@ -570,16 +591,18 @@ def iqsim_batch(trainimg, tilesize, simsizes, *, overlap=None, soft=(), hard=Non
 
 def iqsim(trainimg, tilesize, simsize=None, *, overlap=None, soft=(), hard: Optional[Dict] = None, tol=0.1,
           path="raster", nreal=1, debug=False, showprogress=False, rng=None, uniforms=None,
-          ctx: Optional[Context] = None, return_trace=False):
+          ctx: Optional[Context] = None, return_trace=False, simpath=None):
     """Image quilting simulation (iqsim.jl:50-315).
 
     ``hard`` maps 0-based coordinate tuples to values (NaN marks inactive
     grid voxels); ``soft`` is a list of (data, dataTI) pairs.  Draws come from
     ``rng`` (a numpy Generator) unless ``uniforms`` (nreal x len(path)) is
-    given.  Returns ``reals`` or, with ``debug``, ``(reals, cuts, voxs)``.
+    given.  ``simpath`` replaces genpath's tile order (it may visit a subset
+    of the tiles: tiles it leaves out are not simulated).  Returns ``reals``
+    or, with ``debug``, ``(reals, cuts, voxs)``.
     """
     plan, info = make_plan(trainimg, tilesize, simsize, overlap=overlap, soft=soft, hard=hard, tol=tol, path=path,
-                           nreal=nreal, rng=rng, ctx=ctx)
+                           nreal=nreal, rng=rng, ctx=ctx, simpath=simpath)
     try:
         if uniforms is None:
             uniforms = info["rng"].random((int(nreal), plan.npath))

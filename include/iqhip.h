@@ -214,6 +214,12 @@ int iq_plan_cut_time(iq_plan* plan, int64_t* launches, double* seconds);
  * (0 where not measured).  Diagnostic only. */
 int iq_plan_scan_tuning(iq_plan* plan, int nw[2], float us[4]);
 
+/* Shortlist bank-scan launches of the last run (learned proposal; HIP
+ * events around each launch while profiling is enabled), their summed device
+ * time and their algorithmic HBM bytes (per launch: the bank rows and A2 maps
+ * of each mask shape it scans, P x (128 + 12) bytes).  Diagnostic only. */
+int iq_plan_bank_stats(iq_plan* plan, int64_t* launches, double* seconds, double* bytes);
+
 /* Min-cut work of the last iq_plan_run (graphcut.jl:73 replacement):
  * push-relabel rounds, global-relabel passes (team-wide relaxation passes
  * of the distance labels) and cuts (register-resident kernel; slabs beyond
