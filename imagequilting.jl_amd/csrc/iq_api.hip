@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <random>
 #include <string>
@@ -1528,6 +1529,83 @@ void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// allocation-free entry point: its scratch is this static per-device array
+// (status word, then min-cut node state), held until the process ends.  The
+// array's name is not a device address in host code: its address on each
+// device is looked up once (iq_ctx_create, that device current) and only
+// that address is used.  Every call on one device shares the array.
+// ---------------------------------------------------------------------------
+#define IQ_NOALLOC_HEAD 256
+static __device__ __align__(256) char scratch[IQ_NOALLOC_HEAD + (size_t)IQ_NOALLOC_MAX_NODES * CUT_NODE_BYTES];
+
+namespace {
+constexpr int IQ_MAX_DEVICES = 64;
+char* g_scratch[IQ_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// with `device` current
+void scratch_setup(int device) {
+  if (device >= IQ_MAX_DEVICES) return;  // iq_graphcut_noalloc then reports the device as unsupported
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return;
+  void* p = nullptr;
+  HC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch[device] = (char*)p;
+}
+
+char* scratch_ptr(int device) {
+  if (device >= IQ_MAX_DEVICES) bad(IQ_ERR_UNSUPPORTED, "allocation-free entry points support devices 0-63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device]) bad(IQ_ERR_HIP, "device scratch of the allocation-free entry points not set up");
+  return g_scratch[device];
+}
+
+// ---------------------------------------------------------------------------
+// standalone min cut: argument checks and the launch shared by iq_graphcut
+// and iq_graphcut_noalloc
+// ---------------------------------------------------------------------------
+// interior nodes of the slab (the first and last slices along `dim` are the
+// terminals, graphcut.jl:57-70)
+int64_t cut_check(int dtype, const int64_t dims[3], int dim) {
+  if (dtype != IQ_F32 && dtype != IQ_F64) bad(IQ_ERR_ARG, "dtype must be IQ_F32 or IQ_F64");
+  if (dim < 0 || dim > 2) bad(IQ_ERR_ARG, "cut axis must be 0, 1 or 2");
+  for (int d = 0; d < 3; d++)
+    if (dims[d] <= 0 || dims[d] > (1 << 20)) bad(IQ_ERR_ARG, "invalid slab extents");
+  if (dims[dim] < 2) bad(IQ_ERR_ARG, "cut axis needs at least two slices");
+  const int64_t nvox = dims[0] * dims[1] * dims[2];
+  if (nvox >= (1ll << 31)) bad(IQ_ERR_UNSUPPORTED, "slab of 2^31 or more voxels");
+  const int L = (int)dims[dim];
+  return nvox / L * std::max(0, L - 2);
+}
+
+// A, B, keep: device slabs of extents dims; gscr: node state for slabs of
+// more than IQ_PR_NLDS interior nodes (n * CUT_NODE_BYTES), else unused;
+// err: device status word, zero before the launch.
+void cut_launch(hipStream_t s, const void* dA, const void* dB, int dtype, const int64_t dims[3], int dim, int64_t n,
+                uint8_t* keep, char* gscr, int32_t* err) {
+  const size_t lds = n > IQ_PR_NLDS ? 0 : (size_t)n * CUT_FAST_NODE_BYTES;
+  if (lds > 64 * 1024) {
+    HC(hipFuncSetAttribute((const void*)k_graphcut_host<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HC(hipFuncSetAttribute((const void*)k_graphcut_host<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  const int s0 = (int)dims[0], s1 = (int)dims[1], s2 = (int)dims[2];
+  if (n <= CUT_WAVE_MAX && dtype == IQ_F32)
+    hipLaunchKernelGGL((k_graphcut_host_wave<float>), dim3(1), dim3(64), lds, s, (const float*)dA, (const float*)dB,
+                       s0, s1, s2, dim, keep, err);
+  else if (n <= CUT_WAVE_MAX)
+    hipLaunchKernelGGL((k_graphcut_host_wave<double>), dim3(1), dim3(64), lds, s, (const double*)dA,
+                       (const double*)dB, s0, s1, s2, dim, keep, err);
+  else if (dtype == IQ_F32)
+    hipLaunchKernelGGL((k_graphcut_host<float>), dim3(1), dim3(CB), lds, s, (const float*)dA, (const float*)dB,
+                       s0, s1, s2, dim, keep, gscr, err);
+  else
+    hipLaunchKernelGGL((k_graphcut_host<double>), dim3(1), dim3(CB), lds, s, (const double*)dA, (const double*)dB,
+                       s0, s1, s2, dim, keep, gscr, err);
+  HCL();
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1552,6 +1630,7 @@ int iq_ctx_create(int device, iq_ctx** out) {
     c->device = device;
     HC(hipSetDevice(device));
     HC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    scratch_setup(device);
   });
   if (rc != IQ_OK) { delete c; return rc; }
   *out = c;
@@ -1759,13 +1838,8 @@ int iq_graphcut(iq_ctx* ctx, const void* A, const void* B, int dtype, const int6
   if (!ctx) return fail(nullptr, IQ_ERR_ARG, "null context");
   return guard(ctx, [&] {
     if (!A || !B || !dims || !out_mask) bad(IQ_ERR_ARG, "null argument");
-    if (dtype != IQ_F32 && dtype != IQ_F64) bad(IQ_ERR_ARG, "dtype must be IQ_F32 or IQ_F64");
-    if (dim < 0 || dim > 2 || dims[dim] < 2) bad(IQ_ERR_ARG, "cut axis needs at least two slices");
-    for (int d = 0; d < 3; d++)
-      if (dims[d] <= 0) bad(IQ_ERR_ARG, "invalid slab extents");
+    const int64_t n = cut_check(dtype, dims, dim);
     const int64_t nvox = dims[0] * dims[1] * dims[2];
-    int L = (int)dims[dim];
-    int64_t n = nvox / L * std::max(0, L - 2);
     hipStream_t s = ctx->stream;
     DMem m;
     const size_t esz = dtype == IQ_F32 ? 4 : 8;
@@ -1776,30 +1850,35 @@ int iq_graphcut(iq_ctx* ctx, const void* A, const void* B, int dtype, const int6
     uint8_t* keep = m.alloc<uint8_t>(nvox);
     int32_t* err = m.alloc<int32_t>(1);
     HC(hipMemsetAsync(err, 0, sizeof(int32_t), s));
-    char* scratch = nullptr;
-    size_t lds = 0;
-    if (n > IQ_PR_NLDS) scratch = m.alloc<char>((size_t)n * CUT_NODE_BYTES);
-    else lds = (size_t)n * CUT_FAST_NODE_BYTES;
-    if (lds > 64 * 1024) {
-      HC(hipFuncSetAttribute((const void*)k_graphcut_host<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      HC(hipFuncSetAttribute((const void*)k_graphcut_host<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    }
-    if (n <= CUT_WAVE_MAX && dtype == IQ_F32)
-      hipLaunchKernelGGL((k_graphcut_host_wave<float>), dim3(1), dim3(64), lds, s, (const float*)dA, (const float*)dB,
-                         (int)dims[0], (int)dims[1], (int)dims[2], dim, keep, err);
-    else if (n <= CUT_WAVE_MAX)
-      hipLaunchKernelGGL((k_graphcut_host_wave<double>), dim3(1), dim3(64), lds, s, (const double*)dA,
-                         (const double*)dB, (int)dims[0], (int)dims[1], (int)dims[2], dim, keep, err);
-    else if (dtype == IQ_F32)
-      hipLaunchKernelGGL((k_graphcut_host<float>), dim3(1), dim3(CB), lds, s, (const float*)dA, (const float*)dB,
-                         (int)dims[0], (int)dims[1], (int)dims[2], dim, keep, scratch, err);
-    else
-      hipLaunchKernelGGL((k_graphcut_host<double>), dim3(1), dim3(CB), lds, s, (const double*)dA, (const double*)dB,
-                         (int)dims[0], (int)dims[1], (int)dims[2], dim, keep, scratch, err);
-    HCL();
+    char* gscr = n > IQ_PR_NLDS ? m.alloc<char>((size_t)n * CUT_NODE_BYTES) : nullptr;
+    cut_launch(s, dA, dB, dtype, dims, dim, n, keep, gscr, err);
     int32_t code = 0;
     HC(hipMemcpyAsync(&code, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HC(hipMemcpyAsync(out_mask, keep, nvox, hipMemcpyDeviceToHost, s));
+    HC(hipStreamSynchronize(s));
+    check_kernel_error(code);
+  });
+}
+
+int iq_graphcut_noalloc(iq_ctx* ctx, const void* A, const void* B, int dtype, const int64_t dims[3], int dim,
+                        uint8_t* out_mask) {
+  if (!ctx) return fail(nullptr, IQ_ERR_ARG, "null context");
+  return guard(ctx, [&] {
+    if (!A || !B || !dims || !out_mask) bad(IQ_ERR_ARG, "null argument");
+    const int64_t n = cut_check(dtype, dims, dim);
+    if (n > IQ_NOALLOC_MAX_NODES)
+      bad(IQ_ERR_UNSUPPORTED, "slab with " + std::to_string(n) + " interior nodes: the allocation-free cut holds at most " +
+                                  std::to_string(IQ_NOALLOC_MAX_NODES));
+    char* base = scratch_ptr(ctx->device);
+    int32_t* err = (int32_t*)base;
+    char* gscr = n > IQ_PR_NLDS ? base + IQ_NOALLOC_HEAD : nullptr;
+    hipStream_t s = ctx->stream;
+    // the status word keeps the previous call's value: cleared by a kernel
+    hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(64), 0, s, err, (int64_t)1, (int32_t)0);
+    HCL();
+    cut_launch(s, A, B, dtype, dims, dim, n, out_mask, gscr, err);
+    int32_t code = 0;
+    HC(hipMemcpyAsync(&code, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HC(hipStreamSynchronize(s));
     check_kernel_error(code);
   });

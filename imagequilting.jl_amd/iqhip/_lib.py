@@ -34,8 +34,10 @@ EXPORTS = (
     "iq_plan_create_multi", "iq_plan_last_time", "iq_plan_profile", "iq_plan_scan_stats", "iq_plan_scan_tuning", "iq_plan_cut_time", "iq_plan_cut_stats",
     "iq_simulate", "iq_prepare", "iq_prepare_soft", "iq_finddisabled",
     "iq_plan_learn", "iq_plan_set_proposal", "iq_plan_proposal_stats", "iq_plan_encoder", "iq_plan_select_stats",
-    "iq_plan_validate", "iq_scan_error_bound", "iq_plan_bank_stats",
+    "iq_plan_validate", "iq_scan_error_bound", "iq_plan_bank_stats", "iq_graphcut_noalloc",
 )
+
+IQ_NOALLOC_MAX_NODES = 16384  # include/iqhip.h
 
 
 class IQError(RuntimeError):
@@ -94,6 +96,7 @@ def load(path: str = LIB_PATH):
             "iq_taumodel": [vp, vp, i64, vp, vp, c_int, i64, vp],
             "iq_sample": [vp, vp, i64, f64, vp],
             "iq_graphcut": [vp, vp, vp, c_int, vp, c_int, vp],
+            "iq_graphcut_noalloc": [vp, vp, vp, c_int, vp, c_int, vp],
             "iq_plan_create": [vp, vp, P(vp)],
             "iq_plan_create_multi": [vp, vp, c_int, P(vp)],
             "iq_plan_validate": [vp, c_int],

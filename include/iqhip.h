@@ -101,6 +101,24 @@ int iq_sample(iq_ctx* ctx, const double* p, int64_t n, double u, int64_t* out_in
 int iq_graphcut(iq_ctx* ctx, const void* A, const void* B, int dtype, const int64_t dims[3], int dim,
                 uint8_t* out_mask);
 
+/* Allocation-free variant of iq_graphcut.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  A, B and
+ * out_mask are DEVICE pointers owned by the caller (A and B: slabs of extents
+ * dims and element type dtype, column-major; out_mask: prod(dims) bytes),
+ * their contents ready before the call (the work runs on the context's
+ * stream; the call waits for it and returns the cut's status).  The library
+ * allocates nothing: the status word and the node state of slabs beyond 1,536
+ * interior nodes live in one static array per device of
+ * 256 + 100 * IQ_NOALLOC_MAX_NODES bytes (about 1.6 MB), held until the
+ * process ends.  Slabs with more than IQ_NOALLOC_MAX_NODES interior nodes
+ * (voxels outside the first and last slices along dim) give
+ * IQ_ERR_UNSUPPORTED.  Every call on one device shares that array: calls that
+ * overlap on one device (two contexts, or two host threads) race. */
+#define IQ_NOALLOC_MAX_NODES 16384
+int iq_graphcut_noalloc(iq_ctx* ctx, const void* A, const void* B, int dtype, const int64_t dims[3], int dim,
+                        uint8_t* out_mask);
+
 /* ---- host preparation of raw arrays (no device work, no context) ------ */
 
 /* prepare (utils.jl:96-101): raw array (extents dims, element type
