@@ -195,6 +195,9 @@ def main():
     ap.add_argument("--K", type=int, default=2048, help="shortlist size of the learned proposal")
     ap.add_argument("--check-every", type=int, default=16, help="full-scan recall check every N-th batch")
     ap.add_argument("--full-steps", type=int, default=-1, help="timed steps of the full-scan leg (default: --steps)")
+    ap.add_argument("--encoder", choices=("conv", "linear"), default="conv",
+                    help="learned leg's encoder: the trained convolutional network (default) or the linear one alone")
+    ap.add_argument("--conv-steps", type=int, default=200, help="Adam steps of the convolutional encoder")
     args = ap.parse_args()
     if args.copy_mb > 0:
         copy_calibration(args.copy_mb)
@@ -268,6 +271,26 @@ def main():
         plan.learn(nsamples=8192, iters=16, seed=1)
         ctx._L.iq_ctx_synchronize(ctx.handle)
         learn_s = time.perf_counter() - t0
+        ablation = None
+        conv = None
+        if args.encoder == "conv":
+            # ablation: the linear encoders alone, one checked run (recall at the same K)
+            plan.set_proposal(True, K=args.K, check_every=1, min_recall=0.0)
+            plan.run(U[0])
+            st_lin = plan.proposal_stats()
+            plan.set_proposal(False)
+            # every batch was decided by the full scan: the grids are the plain
+            # method's, and the training queries come from them
+            t0 = time.perf_counter()
+            conv = plan.learn_conv(nqueries=640, topn=256, nrandom=64, steps=args.conv_steps, batch=16, lr=1e-3,
+                                   lr_head=0.0, seed=1)
+            conv["seconds"] = time.perf_counter() - t0
+            plan.set_proposal(True, K=args.K, check_every=1, min_recall=0.0)
+            plan.run(U[0])
+            st_conv = plan.proposal_stats()
+            ablation = {"K": args.K, "checked_queries": st_conv["queries"],
+                        "recall_linear_encoder": st_lin["recall"], "recall_conv_encoder": st_conv["recall"],
+                        "fully_recalled_linear": st_lin["fully_recalled"], "fully_recalled_conv": st_conv["fully_recalled"]}
         plan.set_proposal(True, K=args.K, check_every=args.check_every, min_recall=0.99)
         l_el, l_dev = timed_leg(plan, U, args.warmup, args.steps, buf)
         st = plan.proposal_stats()  # of the last timed run (every run draws the same way)
@@ -289,6 +312,9 @@ def main():
             "K": args.K, "check_every": args.check_every, "min_recall": 0.99,
             "recall": st["recall"], "checked_queries": st["queries"], "fully_recalled_queries": st["fully_recalled"],
             "fell_back": st["fell_back"], "learn_seconds": learn_s,
+            "encoder": ("3-layer 3x3x3 conv (1-32-32-1, residual) + 64-d linear heads, trained on the GPU"
+                        if args.encoder == "conv" else "64-d linear heads (subspace iteration)"),
+            "conv_training": conv, "encoder_ablation_every_batch_checked": ablation,
             "device_ms_per_step": l_dev / args.steps * 1e3,
             "full_scan_value": full_value, "full_scan_ms_per_step": f_el / full_steps * 1e3,
             "speedup_vs_full_scan": value / full_value,

@@ -6,6 +6,7 @@
 #include "iq_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
@@ -20,6 +21,7 @@
 #include "k_select.hip"
 #include "k_cut.hip"
 #include "k_learn.hip"
+#include "k_conv.hip"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -356,7 +358,9 @@ struct iq_plan {
   size_t cut_lds_max = 0;
   size_t multi_lds = 0;    // LDS of the tiled multi-source maps (0: gathering kernel)
   int multi_bx = 16;       // candidate block width of the tiled maps (32 or 16; 16 rows)
-  bool cut_big = false;    // largest slab needs the 512-lane register-resident cut
+  bool cut_big = false;    // largest slab needs the 512/1024-lane register-resident cut
+  int cut_team = 1024;     // its work-items per slab (IQHIP_CUT_TEAM=512: 512)
+  bool recheck_lanes = true;  // exact recheck with a lane per member (IQHIP_RECHECK=groups: older kernel)
   bool cut_wave = false;   // every slab fits the one-wavefront cut
   std::vector<int32_t> enabled_host;
   // learned proposal
@@ -366,8 +370,23 @@ struct iq_plan {
   int64_t lrn_stats[4] = {0, 0, 0, 0};
   std::vector<int64_t> lrn_wofs, lrn_bofs;
   int lrn_vmax = 0;
+  // convolutional encoder of the learned proposal
+  bool has_run = false;         // grids hold a finished run (training queries come from them)
+  bool cv_ready = false;        // banks built from the trained network's output map
+  float* cv_theta = nullptr;    // CV_NW convolution parameters
+  double cv_stats[8] = {};      // see iq_plan_conv_stats
+  double cv_val[2] = {};        // held-out loss at the start and of the kept parameters
   float last_ms = 0.0f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // a run's launch sequence captured once and replayed (runs that take no
+  // host decision between batches and record no profiling events); [0] full
+  // scan, [1] learned shortlist without checks
+  bool graphs = false;  // IQHIP_GRAPH=1: capture and replay (measured no faster: the runs are device-bound)
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  const IqQuery* gdq[2] = {nullptr, nullptr};
+  const int4* gdt[2] = {nullptr, nullptr};
+  int64_t gnq[2] = {-1, -1};
+  int64_t graph_launches = 0;
   // scan profiling
   bool prof = false;
   std::vector<hipEvent_t> prof_ev;
@@ -1040,6 +1059,13 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
     H.bl = pl->mem.alloc<uint4>(nb);
   }
   H.err = pl->mem.alloc<int32_t>(1);
+  // superset size from which the exact recheck gives each member a work-item
+  // instead of a wavefront (IQHIP_RC_SMALL; speed only, same values)
+  H.rc_small = RC_SMALL;
+  if (const char* erc = std::getenv("IQHIP_RC_SMALL")) H.rc_small = std::max(0, std::atoi(erc));
+  if (const char* eg = std::getenv("IQHIP_GRAPH")) pl->graphs = std::atoi(eg) != 0;
+  if (const char* et = std::getenv("IQHIP_CUT_TEAM")) pl->cut_team = std::atoi(et) == 512 ? 512 : 1024;
+  if (const char* er = std::getenv("IQHIP_RECHECK")) pl->recheck_lanes = std::string(er) != "groups";
   H.rounds = pl->mem.alloc<int64_t>(6);  // push rounds, BFS levels, cuts; plain queries, superset, band members
 
   // ---- results
@@ -1053,12 +1079,24 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   if (pl->cut_lds_max > 64 * 1024) {
     HC(hipFuncSetAttribute((const void*)k_graphcut<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
     HC(hipFuncSetAttribute((const void*)k_graphcut<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
-    HC(hipFuncSetAttribute((const void*)k_graphcut_big<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
-    HC(hipFuncSetAttribute((const void*)k_graphcut_big<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
+    HC(hipFuncSetAttribute((const void*)k_graphcut_big<float, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
+    HC(hipFuncSetAttribute((const void*)k_graphcut_big<double, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
+    HC(hipFuncSetAttribute((const void*)k_graphcut_big<float, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
+    HC(hipFuncSetAttribute((const void*)k_graphcut_big<double, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->cut_lds_max));
   }
   if (pl->dtype == IQ_F32) tune_scan<float>(pl);
   else tune_scan<double>(pl);
   HC(hipStreamSynchronize(s));
+}
+
+// exact FP64 recheck of the band supersets of a batch: a lane per member
+// (default), or the wavefront / work-item kernel (IQHIP_RECHECK=groups)
+void launch_recheck(iq_plan* pl, const IqQuery* q, int nq, hipStream_t s) {
+  if (pl->recheck_lanes)
+    hipLaunchKernelGGL(k_band_recheck_lanes, dim3(nq, RCL_GROUPS), dim3(64), 0, s, (const IqDev*)pl->dd, q);
+  else
+    hipLaunchKernelGGL(k_band_recheck, dim3(nq, RC_GROUPS), dim3(256), 0, s, (const IqDev*)pl->dd, q);
+  HCL();
 }
 
 void launch_scan_batch(iq_plan* pl, const IqQuery* q, int nq, hipStream_t s) {
@@ -1119,11 +1157,11 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
         pl->bank_ev.push_back(b);
       }
       HC(hipEventRecord(pl->bank_ev[pl->bank_used], s));
-      // algorithmic bytes: each mask shape's bank rows (64 bf16) and its A2
-      // maps (FP64 + FP32) once per launch (tiles of one mask are adjacent)
+      // algorithmic bytes: each mask shape's bank rows (64 bf16) and its FP32
+      // A2 map once per launch (tiles of one mask are adjacent)
       int nmk = 0;
       for (int t = 0; t < ntiles; t++) nmk += (t == 0 || htiles[t].x != htiles[t - 1].x) ? 1 : 0;
-      pl->bank_bytes_run += (double)nmk * (double)P * (IQ_EMB * 2 + 8 + 4);
+      pl->bank_bytes_run += (double)nmk * (double)P * (IQ_EMB * 2 + 4);
     }
     hipLaunchKernelGGL(k_learn_mscan, dim3(nblk(P, 128)), dim3(256), 0, s, D, q, tiles, ntiles);
     HCL();
@@ -1140,7 +1178,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     HCL();
     hipLaunchKernelGGL(k_short_select, dim3(16, nq), dim3(256), 0, s, D, q);
     HCL();
-    hipLaunchKernelGGL(k_band_recheck, dim3(nq, RC_GROUPS), dim3(256), 0, s, D, q);
+    launch_recheck(pl, q, nq, s);
     HCL();
     hipLaunchKernelGGL(k_band_finalize, dim3(nq), dim3(FB), 0, s, D, q, 1);
     HCL();
@@ -1195,7 +1233,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
       }
       hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256 * SEL_PT), nq), dim3(256), 0, s, D, q);
       HCL();
-      hipLaunchKernelGGL(k_band_recheck, dim3(nq, RC_GROUPS), dim3(256), 0, s, D, q);
+      launch_recheck(pl, q, nq, s);
       HCL();
     } else {
       hipLaunchKernelGGL(k_band_exact_all, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q);
@@ -1235,8 +1273,10 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
   }
   if (pl->cut_wave)
     hipLaunchKernelGGL((k_graphcut_wave<T>), dim3(IQ_MAX_SLAB, nq), dim3(64), pl->cut_lds_max, s, D, q);
+  else if (pl->cut_big && pl->cut_team == 1024)
+    hipLaunchKernelGGL((k_graphcut_big<T, 1024>), dim3(IQ_MAX_SLAB, nq), dim3(1024), pl->cut_lds_max, s, D, q);
   else if (pl->cut_big)
-    hipLaunchKernelGGL((k_graphcut_big<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB_BIG), pl->cut_lds_max, s, D, q);
+    hipLaunchKernelGGL((k_graphcut_big<T, CB_BIG>), dim3(IQ_MAX_SLAB, nq), dim3(CB_BIG), pl->cut_lds_max, s, D, q);
   else
     hipLaunchKernelGGL((k_graphcut<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB), pl->cut_lds_max, s, D, q);
   HCL();
@@ -1247,6 +1287,20 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
   hipLaunchKernelGGL((k_paste<T>), dim3(nblk(g.tilevox, 256), nq), dim3(256), 0, s, D, q);
   HCL();
 }
+
+// One batch of a run: queries [off, off + n) of the run's query table and
+// shortlist tiles [toff, toff + nt).
+struct RunBatch {
+  int64_t off;
+  int n;
+  bool plain, multi;
+  int64_t toff;
+  int nt;
+};
+template <typename T>
+void plan_run_launches(iq_plan* pl, const std::vector<IqQuery>& qs, const std::vector<RunBatch>& batches,
+                       const std::vector<int4>& tiles, bool learned);
+void plan_run_finish(iq_plan* pl);
 
 template <typename T>
 void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
@@ -1269,8 +1323,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   // query table of every batch
   std::vector<IqQuery> qs;
   std::vector<int4> tiles;
-  struct Batch { int64_t off; int n; bool plain, multi; int64_t toff; int nt; };
-  std::vector<Batch> batches;
+  std::vector<RunBatch> batches;
   for (size_t k = 0; k < pl->nwaves; k++) {
     // lockstep: wave k of every grid, all realizations
     std::vector<IqQuery> wq;
@@ -1293,7 +1346,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
         return pl->steps[a.step].mask < pl->steps[b.step].mask;
       });
     for (size_t b0 = 0; b0 < wq.size(); b0 += (size_t)pl->nslots) {
-      Batch bt{(int64_t)qs.size(), 0, false, false, 0, 0};
+      RunBatch bt{(int64_t)qs.size(), 0, false, false, 0, 0};
       for (size_t k = b0; k < wq.size() && k < b0 + (size_t)pl->nslots; k++) {
         IqQuery q = wq[k];
         q.slot = (int32_t)(k - b0);
@@ -1330,7 +1383,61 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
     pl->dt_cap = (int64_t)tiles.size();
   }
   if (!tiles.empty()) HC(hipMemcpyAsync(pl->dt, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+  bool learned = pl->lrn_ready && pl->lrn_on;
+  // replay: the launch sequence depends on the plan, the mode and the query
+  // table's place only (uniforms and forced picks are read from the table)
+  const int gk = learned ? 1 : 0;
+  const bool graphable = pl->graphs && !pl->prof && !(learned && pl->lrn_check > 0);
   HC(hipEventRecord(pl->e0, s));
+  if (graphable && pl->gexec[gk] &&
+      (pl->gdq[gk] != pl->dq || pl->gdt[gk] != pl->dt || pl->gnq[gk] != (int64_t)qs.size())) {
+    HC(hipGraphExecDestroy(pl->gexec[gk]));
+    pl->gexec[gk] = nullptr;
+  }
+  if (graphable && pl->gexec[gk]) {
+    HC(hipGraphLaunch(pl->gexec[gk], s));
+    pl->graph_launches++;
+    for (int k = 0; k < 4; k++) pl->lrn_stats[k] = 0;
+    pl->lrn_fell_back = false;
+  } else {
+    if (graphable) HC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      plan_run_launches<T>(pl, qs, batches, tiles, learned);
+    } catch (...) {
+      if (graphable) {
+        hipGraph_t gr = nullptr;
+        (void)hipStreamEndCapture(s, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+      }
+      throw;
+    }
+    if (graphable) {
+      hipGraph_t gr = nullptr;
+      HC(hipStreamEndCapture(s, &gr));
+      hipGraphExec_t ex = nullptr;
+      const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      HC(e);
+      pl->gexec[gk] = ex;
+      pl->gdq[gk] = pl->dq;
+      pl->gdt[gk] = pl->dt;
+      pl->gnq[gk] = (int64_t)qs.size();
+      HC(hipGraphLaunch(ex, s));
+      pl->graph_launches++;
+    }
+  }
+  HC(hipEventRecord(pl->e1, s));
+  HC(hipStreamSynchronize(s));
+  plan_run_finish(pl);
+}
+
+// Every launch of one run, in order: resets, then the batches.  Host decisions
+// between batches (the recall checks' fallback) read back device counters.
+template <typename T>
+void plan_run_launches(iq_plan* pl, const std::vector<IqQuery>& qs, const std::vector<RunBatch>& batches,
+                       const std::vector<int4>& tiles, bool learned) {
+  hipStream_t s = pl->ctx->stream;
+  IqDev& H = pl->hd;
   hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(64), 0, s, H.err, (int64_t)1, 0);
   hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, H.rounds, (int64_t)6, (int64_t)0);
   hipLaunchKernelGGL((k_fill<int64_t>), dim3(256), dim3(256), 0, s, H.log, pl->nlog, (int64_t)-1);
@@ -1338,7 +1445,6 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s, H.cuts, pl->nvox, (uint8_t)0);
   hipLaunchKernelGGL((k_fill<int64_t>), dim3(16), dim3(256), 0, s, H.pick, pl->nslots, (int64_t)-1);
   HCL();
-  bool learned = pl->lrn_ready && pl->lrn_on;
   pl->lrn_fell_back = false;
   for (int k = 0; k < 4; k++) pl->lrn_stats[k] = 0;
   if (learned) {
@@ -1346,7 +1452,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
     HCL();
   }
   int64_t nplain = 0;
-  for (const Batch& b : batches) {
+  for (const RunBatch& b : batches) {
     int sel = SEL_SCAN;
     if (learned && b.plain) sel = (pl->lrn_check > 0 && nplain++ % pl->lrn_check == 0) ? SEL_CHECK : SEL_LEARNED;
     run_batch<T>(pl, qs.data() + b.off, pl->dq + b.off, b.n, b.plain, b.multi, sel, pl->dt + b.toff,
@@ -1364,8 +1470,11 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
       }
     }
   }
-  HC(hipEventRecord(pl->e1, s));
-  HC(hipStreamSynchronize(s));
+}
+
+// After a run: device time, profiling events, counters, error word.
+void plan_run_finish(iq_plan* pl) {
+  IqDev& H = pl->hd;
   HC(hipEventElapsedTime(&pl->last_ms, pl->e0, pl->e1));
   for (int64_t k = 0; k + 1 < pl->prof_used; k += 2) {
     float ms = 0.0f;
@@ -1389,6 +1498,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   int32_t code = 0;
   HC(hipMemcpy(&code, H.err, sizeof(int32_t), hipMemcpyDeviceToHost));
   HC(hipMemcpy(pl->cut_stats, H.rounds, 6 * sizeof(int64_t), hipMemcpyDeviceToHost));
+  pl->has_run = true;
   if (pl->lrn_ready && pl->lrn_on) {
     unsigned long long st[4];
     HC(hipMemcpy(st, H.lrn.stats, sizeof(st), hipMemcpyDeviceToHost));
@@ -1476,7 +1586,7 @@ void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
     H.lrn.eq = pl->mem.alloc<float>((size_t)pl->nslots * IQ_EMB);
     H.lrn.bq = pl->mem.alloc<uint4>((size_t)pl->nslots * (IQ_EMB / 16) * 64);
     H.lrn.brk = pl->mem.alloc<uint32_t>((size_t)pl->nslots * 4);
-    H.lrn.hist = pl->mem.alloc<int32_t>((size_t)pl->nslots * 2048);
+    H.lrn.binfo = pl->mem.alloc<uint4>((size_t)pl->nslots);
     H.lrn.lcnt = pl->mem.alloc<int32_t>(pl->nslots);
     H.lrn.sbits = pl->mem.alloc<unsigned long long>((size_t)pl->nslots * H.nw64);
     H.lrn.stats = pl->mem.alloc<unsigned long long>(4);
@@ -1517,15 +1627,454 @@ void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
       orthonormalize(s, it, Y, Wm, V);
     }
     hipLaunchKernelGGL(k_learn_bank, dim3(nblk(g.P, 256)), dim3(256), 0, s, (const IqDev*)pl->dd, m, (const float*)Wm,
-                       (uint16_t*)H.lrn.bank + pl->lrn_bofs[m]);
+                       (const float*)nullptr, (uint16_t*)H.lrn.bank + pl->lrn_bofs[m]);
     HCL();
     HC(hipStreamSynchronize(s));
   }
   pl->lrn_ready = true;
+  pl->cv_ready = false;  // banks now hold the linear encodings of the training image
   HC(hipMemcpyAsync(pl->dd, &pl->hd, sizeof(IqDev), hipMemcpyHostToDevice, s));
   HC(hipStreamSynchronize(s));
 }
 
+
+// ---------------------------------------------------------------------------
+// convolutional encoder: training on (query, exact distance) pairs of the full
+// scan, banks from the trained network's output map
+// ---------------------------------------------------------------------------
+struct ConvBuf {
+  DMem mem;
+  CvDims d{};
+  const float* x = nullptr;  // training image (FP32)
+  float *h1 = nullptr, *h2 = nullptr, *g1 = nullptr, *g2 = nullptr, *f = nullptr, *gf = nullptr;
+  float *wr = nullptr, *part = nullptr, *grad = nullptr;
+  unsigned long long* gfix = nullptr;
+  unsigned* amax = nullptr;
+  int nparts = 0;
+  void init(iq_plan* pl) {
+    const IqGeo& g = pl->geo;
+    d.X = g.ti[0];
+    d.Y = g.ti[1];
+    d.Z = g.ti[2];
+    d.N = (int64_t)d.X * d.Y * d.Z;
+    x = pl->hd.ti32;
+    if (!x) bad(IQ_ERR_UNSUPPORTED, "convolutional encoder: training image values must be exact in FP32");
+    h1 = mem.alloc<float>((size_t)CV_C * d.N);
+    h2 = mem.alloc<float>((size_t)CV_C * d.N);
+    g1 = mem.alloc<float>((size_t)CV_C * d.N);
+    g2 = mem.alloc<float>((size_t)CV_C * d.N);
+    f = mem.alloc<float>(d.N);
+    gf = mem.alloc<float>(d.N);
+    gfix = mem.alloc<unsigned long long>(d.N);
+    amax = mem.alloc<unsigned>(1);
+    wr = mem.alloc<float>(CV_C * CV_C * 27);
+    const int64_t ntiles = (int64_t)((d.X + CVW_T - 1) / CVW_T) * ((d.Y + CVW_T - 1) / CVW_T) * ((d.Z + 1) / 2);
+    nparts = (int)std::min<int64_t>(1024, ntiles);
+    part = mem.alloc<float>((size_t)nparts * (CV_C * CV_C * 27 + CV_C));
+    grad = mem.alloc<float>(CV_NW);
+  }
+};
+
+unsigned cv_mid_blocks(const CvDims& d) {
+  return (unsigned)(((d.X + CVM_TX - 1) / CVM_TX) * ((d.Y + CVM_TY - 1) / CVM_TY) * ((d.Z + 1) / 2));
+}
+
+// f = x + conv3(relu(conv2(relu(conv1(x))))) for parameters th (device)
+void conv_forward(hipStream_t s, ConvBuf& B, const float* th) {
+  const CvDims& d = B.d;
+  hipLaunchKernelGGL((k_cv_in<false>), dim3(nblk(d.N, 256)), dim3(256), 0, s, d, B.x, th + CV_W1, th + CV_B1,
+                     (const float*)nullptr, B.h1);
+  hipLaunchKernelGGL((k_cv_wprep<false>), dim3(nblk(CV_C * CV_C * 27, 256)), dim3(256), 0, s, th + CV_W2, B.wr);
+  hipLaunchKernelGGL((k_cv_mid<false>), dim3(cv_mid_blocks(d)), dim3(256), 0, s, d, (const float*)B.h1,
+                     (const float*)B.wr, th + CV_B2, (const float*)nullptr, B.h2);
+  hipLaunchKernelGGL(k_cv_out, dim3(nblk(d.N, 256)), dim3(256), 0, s, d, (const float*)B.h2, th + CV_W3, th + CV_B3,
+                     B.x, B.f);
+  HCL();
+}
+
+template <int NGA, int NGB, int AG, int VG>
+void conv_wgrad(hipStream_t s, ConvBuf& B, const float* G, const float* A, float* out) {
+  hipLaunchKernelGGL((k_cv_wgrad<NGA, NGB, AG, VG>), dim3(B.nparts), dim3(256), 0, s, B.d, G, A, B.part);
+  const int n = NGA * NGB * 27 + NGA;
+  hipLaunchKernelGGL(k_cv_wreduce, dim3(nblk(n, 256)), dim3(256), 0, s, (const float*)B.part, B.nparts, n, out);
+  HCL();
+}
+
+// gradient of the convolution parameters from dL/df (B.gf) into B.grad;
+// each layer's weights and biases are adjacent in the parameter vector, as
+// in the partial rows of k_cv_wgrad
+void conv_backward(hipStream_t s, ConvBuf& B, const float* th) {
+  const CvDims& d = B.d;
+  conv_wgrad<1, CV_C, 1, 8>(s, B, B.gf, B.h2, B.grad + CV_W3);
+  hipLaunchKernelGGL((k_cv_in<true>), dim3(nblk(d.N, 256)), dim3(256), 0, s, d, (const float*)B.gf, th + CV_W3,
+                     (const float*)nullptr, (const float*)B.h2, B.g2);
+  conv_wgrad<CV_C, CV_C, 4, 1>(s, B, B.g2, B.h1, B.grad + CV_W2);
+  hipLaunchKernelGGL((k_cv_wprep<true>), dim3(nblk(CV_C * CV_C * 27, 256)), dim3(256), 0, s, th + CV_W2, B.wr);
+  hipLaunchKernelGGL((k_cv_mid<true>), dim3(cv_mid_blocks(d)), dim3(256), 0, s, d, (const float*)B.g2,
+                     (const float*)B.wr, (const float*)nullptr, (const float*)B.h1, B.g1);
+  conv_wgrad<CV_C, 1, 1, 8>(s, B, B.g1, B.x, B.grad + CV_W1);
+  HCL();
+}
+
+// Loss of nq queries of mask m over ns candidates each (host arrays), with
+// the network th and head W (device); with gW != nullptr also dL/dW (device,
+// 64 x V) and the convolution gradient (B.grad).
+double conv_loss_grad(iq_plan* pl, ConvBuf& B, const float* th, const float* W, int m, int nq, int ns, const float* hQ,
+                      const int32_t* hc, const double* hD, const double* htau, float* gW, bool forward = true) {
+  hipStream_t s = pl->ctx->stream;
+  const int V = pl->masks[m].V;
+  if (ns < 1 || ns > CVL_MAXS || nq < 1) bad(IQ_ERR_ARG, "convolutional encoder: 1..4096 candidates per query");
+  const int64_t nw = (int64_t)nq * ns, rows = nw + nq;
+  if (rows > 65535LL * 64) bad(IQ_ERR_UNSUPPORTED, "convolutional encoder: too many training pairs in one batch");
+  DMem t;
+  float* dQ = t.upload(hQ, (size_t)nq * V, s);
+  int32_t* dc = t.upload(hc, (size_t)nw, s);
+  double* dD = t.upload(hD, (size_t)nw, s);
+  double* dtau = t.upload(htau, (size_t)nq, s);
+  double* A2c = t.alloc<double>(nw);
+  hipLaunchKernelGGL(k_cv_a2, dim3(nblk(nw, 256)), dim3(256), 0, s, (const IqDev*)pl->dd, m, (const int32_t*)dc, nw, A2c);
+  if (forward) conv_forward(s, B, th);  // else B.f already holds the network's output for th
+  float* F = t.alloc<float>((size_t)rows * V);
+  hipLaunchKernelGGL(k_cv_gather, dim3((unsigned)rows), dim3(256), 0, s, (const IqDev*)pl->dd, m, (const float*)B.f,
+                     (const int32_t*)dc, (int)nw, (const float*)dQ, F);
+  HCL();
+  float* E = t.alloc<float>((size_t)rows * IQ_EMB);
+  sgemm(s, (int)rows, IQ_EMB, V, F, V, 1, W, V, 1, E, IQ_EMB);  // E = F W^T
+  float* G = t.alloc<float>((size_t)rows * IQ_EMB);
+  double* lq = t.alloc<double>(nq);
+  double* ls = t.alloc<double>(1);
+  hipLaunchKernelGGL(k_cv_rankloss, dim3(nq), dim3(256), 0, s, nq, ns, (const float*)E, (const double*)A2c,
+                     (const double*)dD, (const double*)dtau, 1.0 / nq, G, lq);
+  hipLaunchKernelGGL(k_cv_losssum, dim3(1), dim3(64), 0, s, (const double*)lq, nq, ls);
+  HCL();
+  if (gW) {
+    sgemm(s, IQ_EMB, V, (int)rows, G, 1, IQ_EMB, F, 1, V, gW, V);  // dL/dW = G^T F
+    float* U = t.alloc<float>((size_t)nw * V);
+    sgemm(s, (int)nw, V, IQ_EMB, G, IQ_EMB, 1, W, 1, V, U, V);      // dL/dF = G W (window rows)
+    hipLaunchKernelGGL((k_fill<int64_t>), dim3(1024), dim3(256), 0, s, (int64_t*)B.gfix, B.d.N, (int64_t)0);
+    hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(64), 0, s, (int32_t*)B.amax, (int64_t)1, 0);
+    hipLaunchKernelGGL(k_cv_absmax, dim3(1024), dim3(256), 0, s, (const float*)U, nw * V, B.amax);
+    hipLaunchKernelGGL(k_cv_scatter, dim3((unsigned)nw), dim3(256), 0, s, (const IqDev*)pl->dd, m, (const int32_t*)dc,
+                       (const float*)U, (int)nw, (const unsigned*)B.amax, B.gfix);
+    hipLaunchKernelGGL(k_cv_unfix, dim3(1024), dim3(256), 0, s, (const unsigned long long*)B.gfix, B.d.N,
+                       (const unsigned*)B.amax, (int)nw, B.gf);
+    HCL();
+    conv_backward(s, B, th);
+  }
+  double loss = 0.0;
+  HC(hipMemcpyAsync(&loss, ls, sizeof(double), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  return loss;
+}
+
+void adam(hipStream_t s, int64_t n, float* p, const float* g, float* m, float* v, float lr, int step) {
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - (float)std::pow(0.9, step), c2 = 1.0f - (float)std::pow(0.999, step);
+  hipLaunchKernelGGL(k_cv_adam, dim3(nblk(n, 256)), dim3(256), 0, s, n, p, g, m, v, lr, b1, b2, c1, c2);
+  HCL();
+}
+
+// Training pairs: per query, its values, its `topn` nearest candidates by
+// exact FP64 distance (preselected from the full scan's lower bounds) and
+// `nrand` random enabled candidates, with exact distances, and a
+// temperature (the spread of its eight nearest distances).
+struct ConvSet {
+  int ns = 0;
+  std::vector<int> mask;
+  std::vector<int64_t> qoff;
+  std::vector<float> Q;
+  std::vector<int32_t> cands;
+  std::vector<double> D, tau;
+};
+
+template <typename T>
+void conv_harvest(iq_plan* pl, int64_t nq_total, int topn, int nrand, std::mt19937_64& rng, ConvSet& set) {
+  hipStream_t s = pl->ctx->stream;
+  const IqGeo& g = pl->geo;
+  const IqDev* D = pl->dd;
+  // plain tile steps with an overlap mask, over every realization
+  std::vector<std::pair<int32_t, int32_t>> pool;  // (step, realization)
+  for (const HostGrid& G : pl->grids)
+    for (int64_t r = G.real0; r < G.real0 + G.nreal; r++)
+      for (int64_t i = 0; i < G.npath; i++) {
+        const IqStep& st = pl->steps[G.step0 + i];
+        if (st.active && st.mask >= 0 && !st.hard && pl->naux == 0) pool.push_back({(int32_t)(G.step0 + i), (int32_t)r});
+      }
+  if (pool.empty()) bad(IQ_ERR_ARG, "convolutional encoder: the plan has no plain tile step with an overlap");
+  std::shuffle(pool.begin(), pool.end(), rng);
+  if ((int64_t)pool.size() > nq_total) pool.resize((size_t)nq_total);
+  std::stable_sort(pool.begin(), pool.end(), [&](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) {
+    return pl->steps[a.first].mask < pl->steps[b.first].mask;
+  });
+  const int cap = 4 * topn;
+  set.ns = topn + nrand;
+  std::uniform_int_distribution<size_t> pick(0, pl->enabled_host.size() - 1);
+  const int nb = (int)std::min<int64_t>(pl->nslots, 256);
+  DMem t;
+  IqQuery* dq = t.alloc<IqQuery>(nb);
+  int32_t* hist = t.alloc<int32_t>((size_t)nb * CVT_BINS);
+  uint32_t* sel = t.alloc<uint32_t>((size_t)nb * 4);
+  int32_t* list = t.alloc<int32_t>((size_t)nb * cap);
+  int32_t* cnt = t.alloc<int32_t>(nb);
+  int32_t* dslot = t.alloc<int32_t>(nb);
+  int32_t* dcand = t.alloc<int32_t>((size_t)nb * (cap + nrand));
+  double* dex = t.alloc<double>((size_t)nb * (cap + nrand));
+  float* dQ = t.alloc<float>((size_t)nb * std::max(pl->lrn_vmax, 1));
+  for (size_t b0 = 0, b1 = 0; b0 < pool.size(); b0 = b1) {
+    // queries of one mask per batch (exact distances and values per mask)
+    const int m = pl->steps[pool[b0].first].mask;
+    b1 = b0;
+    while (b1 < pool.size() && b1 < b0 + (size_t)nb && pl->steps[pool[b1].first].mask == m) b1++;
+    const int nq = (int)(b1 - b0);
+    const int V = pl->masks[m].V;
+    std::vector<IqQuery> hq((size_t)nq);
+    std::vector<int32_t> hs((size_t)nq);
+    for (int i = 0; i < nq; i++) {
+      IqQuery q{};
+      q.step = pool[b0 + i].first;
+      q.real = pool[b0 + i].second;
+      q.slot = i;
+      q.multi = 0;
+      q.u = 0.5;
+      q.forced = -1;
+      hq[i] = q;
+      hs[i] = i;
+    }
+    HC(hipMemcpyAsync(dq, hq.data(), nq * sizeof(IqQuery), hipMemcpyHostToDevice, s));
+    HC(hipMemcpyAsync(dslot, hs.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    // the full scan's lower bounds of every candidate, per slot
+    hipLaunchKernelGGL((k_query_prep<T>), dim3(nq, 1), dim3(256), 0, s, D, (const IqQuery*)dq);
+    HCL();
+    if (pl->mfma) {
+      const int ntile = (nq + 31) / 32;
+      hipLaunchKernelGGL(k_mfma_bprep, dim3(pl->hd.ktab.nstep, ntile), dim3(64), 0, s, D, (const IqQuery*)dq, nq);
+      const int nw = pl->mfma_nw_forced ? pl->mfma_nw : pl->nw_tiles[ntile > 1 ? 2 : 1];
+      dim3 grid(nblk(g.dist[0], 32), nblk(g.dist[1], 4 * nw), (unsigned)(g.dist[2] * ntile));
+      if (pl->mfma_split)
+        hipLaunchKernelGGL((k_scan_mfma<true>), grid, dim3(64 * nw), pl->lds_nw[nw], s, D, (const IqQuery*)dq, nq,
+                           pl->mfma_gerr, pl->kofs_nw[nw]);
+      else
+        hipLaunchKernelGGL((k_scan_mfma<false>), grid, dim3(64 * nw), pl->lds_nw[nw], s, D, (const IqQuery*)dq, nq,
+                           pl->mfma_gerr, pl->kofs_nw[nw]);
+    } else {
+      launch_scan_batch(pl, dq, nq, s);
+    }
+    HCL();
+    // the topn smallest lower bounds per slot: two histogram passes, then the list
+    const unsigned hb = std::min<unsigned>(nblk(g.P, 256 * 16), 512);
+    hipLaunchKernelGGL((k_fill<int32_t>), dim3(256), dim3(256), 0, s, hist, (int64_t)nb * CVT_BINS, 0);
+    hipLaunchKernelGGL((k_cv_tophist<0>), dim3(hb, nq), dim3(256), 0, s, D, (const IqQuery*)dq, hist, (const uint32_t*)sel);
+    hipLaunchKernelGGL((k_cv_topsel<0>), dim3(nq), dim3(64), 0, s, (const IqQuery*)dq, (const int32_t*)hist, topn, sel);
+    hipLaunchKernelGGL((k_fill<int32_t>), dim3(256), dim3(256), 0, s, hist, (int64_t)nb * CVT_BINS, 0);
+    hipLaunchKernelGGL((k_cv_tophist<1>), dim3(hb, nq), dim3(256), 0, s, D, (const IqQuery*)dq, hist, (const uint32_t*)sel);
+    hipLaunchKernelGGL((k_cv_topsel<1>), dim3(nq), dim3(64), 0, s, (const IqQuery*)dq, (const int32_t*)hist, topn, sel);
+    hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(256), 0, s, cnt, (int64_t)nb, 0);
+    hipLaunchKernelGGL(k_cv_topcollect, dim3(hb, nq), dim3(256), 0, s, D, (const IqQuery*)dq, (const uint32_t*)sel, list,
+                       cnt, cap);
+    HCL();
+    std::vector<int32_t> hcnt((size_t)nq), hlist((size_t)nq * cap);
+    HC(hipMemcpyAsync(hcnt.data(), cnt, nq * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HC(hipMemcpyAsync(hlist.data(), list, (size_t)nq * cap * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HC(hipStreamSynchronize(s));
+    // listed and random candidates -> exact distances
+    const int w = cap + nrand;
+    std::vector<int32_t> hc((size_t)nq * w);
+    std::vector<int> nl((size_t)nq);
+    for (int i = 0; i < nq; i++) {
+      nl[i] = std::min(hcnt[i], cap);
+      std::vector<int32_t> li(hlist.begin() + (size_t)i * cap, hlist.begin() + (size_t)i * cap + nl[i]);
+      std::sort(li.begin(), li.end());
+      for (int j = 0; j < w; j++) hc[(size_t)i * w + j] = j < nl[i] ? li[j] : pl->enabled_host[0];  // padding
+      // random candidates outside the list (repeats allowed when the list
+      // holds most of the enabled candidates)
+      const bool distinct = pl->enabled_host.size() >= 2 * (size_t)(nl[i] + nrand);
+      for (int j = nl[i]; j < nl[i] + nrand;) {
+        const int32_t c = pl->enabled_host[pick(rng)];
+        if (distinct && std::binary_search(li.begin(), li.end(), c)) continue;
+        hc[(size_t)i * w + j++] = c;
+      }
+    }
+    HC(hipMemcpyAsync(dcand, hc.data(), hc.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_cv_exact, dim3(w, nq), dim3(64), 0, s, D, m, (const int32_t*)dslot, (const int32_t*)dcand, w, dex);
+    hipLaunchKernelGGL(k_cv_queries, dim3(nq), dim3(256), 0, s, D, V, (const int32_t*)dslot, dQ);
+    HCL();
+    std::vector<double> hd((size_t)nq * w);
+    std::vector<float> hQ((size_t)nq * V);
+    HC(hipMemcpyAsync(hd.data(), dex, hd.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HC(hipMemcpyAsync(hQ.data(), dQ, hQ.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    HC(hipStreamSynchronize(s));
+    for (int i = 0; i < nq; i++) {
+      if (nl[i] < topn) continue;  // fewer enabled candidates than topn
+      std::vector<int> ord((size_t)nl[i]);
+      for (int j = 0; j < nl[i]; j++) ord[j] = j;
+      const double* di = hd.data() + (size_t)i * w;
+      const int32_t* ci = hc.data() + (size_t)i * w;
+      std::sort(ord.begin(), ord.end(), [&](int a, int b) { return di[a] < di[b] || (di[a] == di[b] && ci[a] < ci[b]); });
+      set.mask.push_back(m);
+      set.qoff.push_back((int64_t)set.Q.size());
+      set.Q.insert(set.Q.end(), hQ.begin() + (size_t)i * V, hQ.begin() + (size_t)(i + 1) * V);
+      for (int j = 0; j < topn; j++) {
+        set.cands.push_back(ci[ord[j]]);
+        set.D.push_back(di[ord[j]]);
+      }
+      for (int j = 0; j < nrand; j++) {
+        set.cands.push_back(ci[nl[i] + j]);
+        set.D.push_back(di[nl[i] + j]);
+      }
+      // temperature: the band's width tol * min D (iqsim.jl:237), at least the
+      // spread of the eight nearest, so the target spreads over the band
+      const double d0 = di[ord[0]], d8 = di[ord[std::min(7, topn - 1)]];
+      set.tau.push_back(std::max({pl->tol * d0, d8 - d0, 1e-3 * d8, 1e-30}));
+    }
+  }
+  if (set.mask.empty()) bad(IQ_ERR_ARG, "convolutional encoder: no training query has enough enabled candidates");
+}
+
+// Harvest, train (Adam), then rebuild every mask's bank from the network's
+// output map with the trained heads.
+template <typename T>
+void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
+  hipStream_t s = pl->ctx->stream;
+  IqDev& H = pl->hd;
+  const IqGeo& g = pl->geo;
+  if (!pl->lrn_ready) bad(IQ_ERR_ARG, "convolutional encoder: learn the linear encoders first (iq_plan_learn)");
+  if (!pl->has_run) bad(IQ_ERR_ARG, "convolutional encoder: training queries come from a finished iq_plan_run");
+  if (o.nqueries < 1 || o.topn < 8 || o.nrandom < 0 || o.topn + o.nrandom > CVL_MAXS || o.steps < 0 || o.batch < 1 ||
+      !(o.lr >= 0.0f) || !(o.lr_head >= 0.0f))
+    bad(IQ_ERR_ARG, "convolutional encoder: invalid options");
+  const auto t0 = std::chrono::steady_clock::now();
+  std::mt19937_64 rng(o.seed);
+  ConvBuf B;
+  B.init(pl);
+  // network: He-normal hidden layers, zero output layer (f = x at the start)
+  std::vector<float> th(CV_NW, 0.0f);
+  std::normal_distribution<float> nrm(0.0f, 1.0f);
+  for (int i = 0; i < CV_C * 27; i++) th[CV_W1 + i] = nrm(rng) * std::sqrt(2.0f / 27.0f);
+  for (int i = 0; i < CV_C * CV_C * 27; i++) th[CV_W2 + i] = nrm(rng) * std::sqrt(2.0f / (27.0f * CV_C));
+  if (!pl->cv_theta) pl->cv_theta = pl->mem.alloc<float>(CV_NW);
+  HC(hipMemcpyAsync(pl->cv_theta, th.data(), CV_NW * sizeof(float), hipMemcpyHostToDevice, s));
+  ConvSet set;
+  conv_harvest<T>(pl, o.nqueries, o.topn, o.nrandom, rng, set);
+  const auto t1 = std::chrono::steady_clock::now();
+  // every fifth query is held out; batches of one mask each, visited in a
+  // shuffled order every epoch
+  std::map<int, std::vector<int>> bymask, valmask;
+  for (int i = 0; i < (int)set.mask.size(); i++) (i % 5 == 4 ? valmask : bymask)[set.mask[i]].push_back(i);
+  std::vector<std::vector<int>> batches, vbatches;
+  for (auto& kv : bymask)
+    for (size_t a = 0; a < kv.second.size(); a += (size_t)o.batch)
+      batches.emplace_back(kv.second.begin() + a, kv.second.begin() + std::min(kv.second.size(), a + (size_t)o.batch));
+  for (auto& kv : valmask)
+    for (size_t a = 0; a < kv.second.size(); a += 64)
+      vbatches.emplace_back(kv.second.begin() + a, kv.second.begin() + std::min(kv.second.size(), a + (size_t)64));
+  if (batches.empty()) bad(IQ_ERR_ARG, "convolutional encoder: too few training queries");
+  size_t wtot = 0;
+  for (const IqMask& m : pl->masks) wtot += (size_t)IQ_EMB * m.V;
+  DMem adm;
+  float* cm = adm.alloc<float>(CV_NW);
+  float* cv = adm.alloc<float>(CV_NW);
+  float* hm = adm.alloc<float>(wtot);
+  float* hv = adm.alloc<float>(wtot);
+  float* gW = adm.alloc<float>((size_t)IQ_EMB * pl->lrn_vmax);
+  hipLaunchKernelGGL((k_fill<float>), dim3(64), dim3(256), 0, s, cm, (int64_t)CV_NW, 0.0f);
+  hipLaunchKernelGGL((k_fill<float>), dim3(64), dim3(256), 0, s, cv, (int64_t)CV_NW, 0.0f);
+  hipLaunchKernelGGL((k_fill<float>), dim3(1024), dim3(256), 0, s, hm, (int64_t)wtot, 0.0f);
+  hipLaunchKernelGGL((k_fill<float>), dim3(1024), dim3(256), 0, s, hv, (int64_t)wtot, 0.0f);
+  HCL();
+  std::vector<int> hstep(pl->masks.size(), 0);
+  std::vector<double> losses;
+  std::vector<size_t> order(batches.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = i;
+  const int ns = set.ns;
+  // loss of the queries `bt` (one mask) for the current parameters
+  auto batch_loss = [&](const std::vector<int>& bt, float* grad_head, bool forward) {
+    const int m = set.mask[bt[0]];
+    const int V = pl->masks[m].V;
+    const int nq = (int)bt.size();
+    std::vector<float> hQ((size_t)nq * V);
+    std::vector<int32_t> hc((size_t)nq * ns);
+    std::vector<double> hD((size_t)nq * ns), htau((size_t)nq);
+    for (int i = 0; i < nq; i++) {
+      const int qi = bt[i];
+      std::copy(set.Q.begin() + set.qoff[qi], set.Q.begin() + set.qoff[qi] + V, hQ.begin() + (size_t)i * V);
+      std::copy(set.cands.begin() + (size_t)qi * ns, set.cands.begin() + (size_t)(qi + 1) * ns, hc.begin() + (size_t)i * ns);
+      std::copy(set.D.begin() + (size_t)qi * ns, set.D.begin() + (size_t)(qi + 1) * ns, hD.begin() + (size_t)i * ns);
+      htau[i] = set.tau[qi];
+    }
+    const float* Wm = (const float*)H.lrn.W + pl->lrn_wofs[m];
+    return conv_loss_grad(pl, B, pl->cv_theta, Wm, m, nq, ns, hQ.data(), hc.data(), hD.data(), htau.data(), grad_head,
+                          forward);
+  };
+  // held-out loss (mean over held-out queries); the parameters with the
+  // lowest one are kept, the starting point included (network = identity)
+  auto val_loss = [&]() {
+    double tot = 0.0;
+    int64_t n = 0;
+    for (size_t k = 0; k < vbatches.size(); k++) {
+      tot += batch_loss(vbatches[k], nullptr, k == 0) * (double)vbatches[k].size();
+      n += (int64_t)vbatches[k].size();
+    }
+    return n ? tot / (double)n : 0.0;
+  };
+  float* best_th = adm.alloc<float>(CV_NW);
+  float* best_W = adm.alloc<float>(wtot);
+  auto keep_best = [&]() {
+    HC(hipMemcpyAsync(best_th, pl->cv_theta, CV_NW * sizeof(float), hipMemcpyDeviceToDevice, s));
+    HC(hipMemcpyAsync(best_W, H.lrn.W, wtot * sizeof(float), hipMemcpyDeviceToDevice, s));
+  };
+  const int eval_every = std::max(1, std::min(25, o.steps / 4 + 1));
+  double best = vbatches.empty() ? 0.0 : val_loss();
+  const double val0 = best;
+  int best_step = 0;
+  keep_best();
+  for (int step = 0; step < o.steps; step++) {
+    if (step % (int)batches.size() == 0) std::shuffle(order.begin(), order.end(), rng);
+    const std::vector<int>& bt = batches[order[step % batches.size()]];
+    const int m = set.mask[bt[0]];
+    const int V = pl->masks[m].V;
+    float* Wm = (float*)H.lrn.W + pl->lrn_wofs[m];
+    losses.push_back(batch_loss(bt, gW, true));
+    adam(s, CV_NW, pl->cv_theta, B.grad, cm, cv, o.lr, step + 1);
+    if (o.lr_head > 0.0f)
+      adam(s, (int64_t)IQ_EMB * V, Wm, gW, hm + pl->lrn_wofs[m], hv + pl->lrn_wofs[m], o.lr_head, ++hstep[m]);
+    if (!vbatches.empty() && ((step + 1) % eval_every == 0 || step + 1 == o.steps)) {
+      const double v = val_loss();
+      if (v < best) {
+        best = v;
+        best_step = step + 1;
+        keep_best();
+      }
+    }
+  }
+  HC(hipMemcpyAsync(pl->cv_theta, best_th, CV_NW * sizeof(float), hipMemcpyDeviceToDevice, s));
+  HC(hipMemcpyAsync((float*)H.lrn.W, best_W, wtot * sizeof(float), hipMemcpyDeviceToDevice, s));
+  // banks from the trained network's output map
+  conv_forward(s, B, pl->cv_theta);
+  for (int m = 0; m < (int)pl->masks.size(); m++) {
+    hipLaunchKernelGGL(k_learn_bank, dim3(nblk(g.P, 256)), dim3(256), 0, s, (const IqDev*)pl->dd, m,
+                       (const float*)H.lrn.W + pl->lrn_wofs[m], (const float*)B.f, (uint16_t*)H.lrn.bank + pl->lrn_bofs[m]);
+    HCL();
+  }
+  HC(hipStreamSynchronize(s));
+  const auto t2 = std::chrono::steady_clock::now();
+  pl->cv_ready = true;
+  auto mean = [&](size_t a, size_t b) {
+    double x = 0.0;
+    for (size_t i = a; i < b; i++) x += losses[i];
+    return b > a ? x / (double)(b - a) : 0.0;
+  };
+  const size_t k = std::min<size_t>(10, losses.size());
+  pl->cv_stats[0] = std::chrono::duration<double>(t1 - t0).count();
+  pl->cv_stats[1] = std::chrono::duration<double>(t2 - t1).count();
+  pl->cv_stats[2] = mean(0, k);
+  pl->cv_stats[3] = mean(losses.size() - k, losses.size());
+  pl->cv_stats[4] = (double)set.mask.size() * ns;
+  pl->cv_stats[5] = (double)o.steps;
+  pl->cv_stats[6] = (double)set.mask.size();
+  pl->cv_stats[7] = (double)best_step;
+  pl->cv_val[0] = val0;
+  pl->cv_val[1] = best;
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1921,6 +2470,8 @@ int iq_plan_destroy(iq_plan* plan) {
   for (hipEvent_t e : plan->prof_ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : plan->cut_ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : plan->bank_ev) (void)hipEventDestroy(e);
+  for (hipGraphExec_t g : plan->gexec)
+    if (g) (void)hipGraphExecDestroy(g);
   if (plan->e0) (void)hipEventDestroy(plan->e0);
   if (plan->e1) (void)hipEventDestroy(plan->e1);
   plan->mem.release();
@@ -2010,6 +2561,85 @@ int iq_plan_learn(iq_ctx* ctx, iq_plan* plan, int64_t nsamples, int iters, uint6
   return guard(ctx, [&] {
     if (plan->ctx != ctx) bad(IQ_ERR_ARG, "plan belongs to another context");
     plan_learn(plan, nsamples, iters, seed);
+  });
+}
+
+int iq_plan_learn_conv(iq_ctx* ctx, iq_plan* plan, const iq_conv_opts* opts) {
+  if (!ctx || !plan || !opts) return fail(ctx, IQ_ERR_ARG, "null context, plan or options");
+  return guard(ctx, [&] {
+    if (plan->ctx != ctx) bad(IQ_ERR_ARG, "plan belongs to another context");
+    if (plan->dtype == IQ_F32) conv_learn<float>(plan, *opts);
+    else conv_learn<double>(plan, *opts);
+  });
+}
+
+int iq_plan_conv_stats(iq_plan* plan, double* out) {
+  if (!plan || !out) return fail(nullptr, IQ_ERR_ARG, "null plan or output");
+  return guard(plan->ctx, [&] {
+    if (!plan->cv_ready) bad(IQ_ERR_ARG, "call iq_plan_learn_conv first");
+    for (int k = 0; k < 8; k++) out[k] = plan->cv_stats[k];
+    out[8] = plan->cv_val[0];
+    out[9] = plan->cv_val[1];
+  });
+}
+
+int iq_plan_conv_params(iq_plan* plan, float* theta) {
+  if (!plan || !theta) return fail(nullptr, IQ_ERR_ARG, "null plan or output");
+  return guard(plan->ctx, [&] {
+    if (!plan->cv_theta) bad(IQ_ERR_ARG, "call iq_plan_learn_conv first");
+    HC(hipMemcpy(theta, plan->cv_theta, CV_NW * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int iq_plan_conv_grad(iq_ctx* ctx, iq_plan* plan, int mask, const float* theta, int64_t nq, int64_t ns, const float* Q,
+                      const int32_t* cands, const double* D, const double* tau, double* loss, float* grad) {
+  if (!ctx || !plan) return fail(ctx, IQ_ERR_ARG, "null context or plan");
+  return guard(ctx, [&] {
+    if (plan->ctx != ctx) bad(IQ_ERR_ARG, "plan belongs to another context");
+    if (!theta || !Q || !cands || !D || !tau || !loss) bad(IQ_ERR_ARG, "null argument");
+    if (mask < 0 || mask >= (int)plan->masks.size()) bad(IQ_ERR_ARG, "mask index out of range");
+    if (plan->masks[mask].a2 < 0) bad(IQ_ERR_ARG, "mask without an A2 map");
+    if (nq < 1 || nq > 65535 || ns < 1 || ns > CVL_MAXS) bad(IQ_ERR_ARG, "1..65535 queries of 1..4096 candidates");
+    for (int64_t i = 0; i < nq * ns; i++)
+      if (cands[i] < 0 || cands[i] >= plan->geo.P) bad(IQ_ERR_ARG, "candidate index out of range");
+    for (int64_t i = 0; i < nq; i++)
+      if (!(tau[i] > 0.0)) bad(IQ_ERR_ARG, "temperatures must be positive");
+    const int V = plan->masks[mask].V;
+    hipStream_t s = ctx->stream;
+    ConvBuf B;
+    B.init(plan);
+    DMem t;
+    float* th = t.upload(theta, CV_NW, s);
+    float* W = t.upload(theta + CV_NW, (size_t)IQ_EMB * V, s);
+    float* gW = grad ? t.alloc<float>((size_t)IQ_EMB * V) : nullptr;
+    *loss = conv_loss_grad(plan, B, th, W, mask, (int)nq, (int)ns, Q, cands, D, tau, gW);
+    if (grad) {
+      HC(hipMemcpyAsync(grad, B.grad, CV_NW * sizeof(float), hipMemcpyDeviceToHost, s));
+      HC(hipMemcpyAsync(grad + CV_NW, gW, (size_t)IQ_EMB * V * sizeof(float), hipMemcpyDeviceToHost, s));
+      HC(hipStreamSynchronize(s));
+    }
+  });
+}
+
+int iq_plan_bank_rows(iq_ctx* ctx, iq_plan* plan, int mask, const int32_t* cands, int64_t n, float* out) {
+  if (!ctx || !plan) return fail(ctx, IQ_ERR_ARG, "null context or plan");
+  return guard(ctx, [&] {
+    if (plan->ctx != ctx) bad(IQ_ERR_ARG, "plan belongs to another context");
+    if (!plan->lrn_ready) bad(IQ_ERR_ARG, "call iq_plan_learn first");
+    if (mask < 0 || mask >= (int)plan->masks.size()) bad(IQ_ERR_ARG, "mask index out of range");
+    if (!cands || !out || n < 0) bad(IQ_ERR_ARG, "invalid argument");
+    std::vector<uint16_t> row(IQ_EMB);
+    for (int64_t i = 0; i < n; i++) {
+      if (cands[i] < 0 || cands[i] >= plan->geo.P) bad(IQ_ERR_ARG, "candidate index out of range");
+      HC(hipMemcpy(row.data(), plan->hd.lrn.bank + plan->lrn_bofs[mask] + (int64_t)cands[i] * IQ_EMB,
+                   IQ_EMB * sizeof(uint16_t), hipMemcpyDeviceToHost));
+      for (int j = 0; j < IQ_EMB; j++) {
+        const uint32_t b = (uint32_t)row[j] << 16;
+        float f;
+        std::memcpy(&f, &b, 4);
+        out[i * IQ_EMB + j] = f;
+      }
+    }
   });
 }
 

@@ -97,7 +97,7 @@ struct IqLearn {
   float* eq;                  // per slot: the query's encoding
   uint4* bq;                  // B fragments of the query tiles, [tile][IQ_EMB / 16][64]
   uint32_t* brk;              // per slot: bracket lo, hi, bin shift
-  int32_t* hist;              // per slot: 2048-bin histogram over the bracket
+  uint4* binfo;               // per batch position: FP32 B2 bits, bracket lo, hi, shift (bank scan columns)
   int32_t* lcnt;              // per slot: candidates listed inside the bracket
   int2* list;                 // per slot: lcap (candidate, order key) pairs
   int64_t lcap;
@@ -190,6 +190,7 @@ struct IqDev {
   uint4* bh;                               // B fragments (hi), [ntile][nstep][64]
   uint4* bl;                               // B fragments (lo)
   IqLearn lrn;
+  int32_t rc_small;  // exact recheck: a wavefront per member up to this superset size, a work-item per member above
 };
 
 // Order-preserving 32-bit key of a non-NaN float (-0 and +0 share a key).

@@ -681,14 +681,16 @@ k_graphcut_wave(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 template __global__ void k_graphcut_wave<float>(const IqDev*, const IqQuery*);
 template __global__ void k_graphcut_wave<double>(const IqDev*, const IqQuery*);
 
-// Plans whose largest slab has CUT_BIG_MIN..CUT_BIG_MAX interior nodes: 512
-// work-items per slab, three register-resident nodes each (node state of a
-// 1,536-node slab fills the 160 KB of LDS, so one slab per compute unit runs
-// at a time and more lanes per slab shorten each round).
+// Plans whose largest slab has CUT_BIG_MIN..CUT_BIG_MAX interior nodes: TEAM
+// (1,024, or 512) work-items per slab, up to two (three) register-resident
+// nodes each (node state of a 1,536-node slab fills the 160 KB of LDS, so one
+// slab per compute unit runs at a time and more lanes per slab shorten each
+// round: config 5's 1,536-node cuts take 1,469 us per launch with 1,024
+// work-items against 1,780 us with 512).
 #define CB_BIG 512
 #define CUT_BIG_MAX 1536
-template <typename T>
-__global__ void __launch_bounds__(CB_BIG)
+template <typename T, int TEAM>
+__global__ void __launch_bounds__(TEAM)
 k_graphcut_big(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   extern __shared__ char cut_lds[];
   const IqQuery q = qs[blockIdx.y];
@@ -711,13 +713,17 @@ k_graphcut_big(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   B.st[0] = 1; B.st[1] = g.ti[0]; B.st[2] = (int64_t)g.ti[0] * g.ti[1];
   uint8_t* out = D->keep + (slot * IQ_MAX_SLAB + blockIdx.x) * g.tilevox;
   const int n = s[0] * s[1] * s[2] / s[sl.dim] * (s[sl.dim] > 2 ? s[sl.dim] - 2 : 0);
-  if (n <= CB_BIG) cut_slab_fast<T, 1, CB_BIG>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, D->err, D->rounds);
-  else if (n <= 2 * CB_BIG) cut_slab_fast<T, 2, CB_BIG>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, D->err, D->rounds);
-  else cut_slab_fast<T, 3, CB_BIG>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, D->err, D->rounds);
+  static_assert(3 * TEAM >= CUT_BIG_MAX, "team too small for the largest slab");
+  if (n <= TEAM) cut_slab_fast<T, 1, TEAM>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, D->err, D->rounds);
+  else if (n <= 2 * TEAM) cut_slab_fast<T, 2, TEAM>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, D->err, D->rounds);
+  else if constexpr (2 * TEAM < CUT_BIG_MAX)
+    cut_slab_fast<T, 3, TEAM>(A, B, s, sl.dim, (uint8_t)sl.side, out, cut_lds, D->err, D->rounds);
 }
 
-template __global__ void k_graphcut_big<float>(const IqDev*, const IqQuery*);
-template __global__ void k_graphcut_big<double>(const IqDev*, const IqQuery*);
+template __global__ void k_graphcut_big<float, 512>(const IqDev*, const IqQuery*);
+template __global__ void k_graphcut_big<double, 512>(const IqDev*, const IqQuery*);
+template __global__ void k_graphcut_big<float, 1024>(const IqDev*, const IqQuery*);
+template __global__ void k_graphcut_big<double, 1024>(const IqDev*, const IqQuery*);
 
 // Standalone cut of one slab pair (iq_graphcut): A and B both column-major of extents s.
 template <typename T>

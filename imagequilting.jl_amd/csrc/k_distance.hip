@@ -824,7 +824,7 @@ k_band_select(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 // Grid: (query, groups per query), 256 work-items per group.
 // ---------------------------------------------------------------------------
 #define RC_CHUNK 1024
-#define RC_SMALL 512
+#define RC_SMALL 512  // default of IqDev::rc_small
 #define RC_WCH 512
 #define RC_GROUPS 128  // member groups per query: 512 members in one pass, more loop
 __global__ void __launch_bounds__(256)
@@ -838,7 +838,8 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const IqQuery q = qs[blockIdx.x];
   const int grp = blockIdx.y, ngrp = gridDim.y;
   const int n = D->nsup[q.slot];
-  if (grp * (n <= RC_SMALL ? 4 : 256) >= n) return;
+  const int rcs = D->rc_small;
+  if (grp * (n <= rcs ? 4 : 256) >= n) return;
   const IqStep& st = D->steps[q.step];
   if (q.multi || st.mask < 0) return;
   const IqGeo& g = D->g;
@@ -851,7 +852,7 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const double b2 = D->b2[slot * (2 + D->naux)];
   double* dex = D->dexact + slot * g.P;
   unsigned long long best = 0x7ff0000000000000ull;
-  if (n <= RC_SMALL) {  // few members: a wavefront per member, its chain fed from LDS
+  if (n <= rcs) {  // few members: a wavefront per member, its chain fed from LDS
     // the wavefront stages RC_WCH window and query values per chunk and loads
     // the next chunk into registers while the chain runs over the current one
     double* tl = wl + (threadIdx.x >> 6) * (2 * RC_WCH);
@@ -905,36 +906,42 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (j0 + RC_WCH < V) fetch(j0 + RC_WCH);  // next chunk in flight during the chain
-        // eight LDS pairs of the next group in flight while the ordered fmas
-        // of the current group run
-        int k = 0;
-        double t[8], u[8];
-        if (nj >= 8) {
+        // the ordered chain runs on lane 0 alone (the other lanes' LDS reads
+        // would only repeat its values and take LDS bandwidth from other
+        // wavefronts); eight LDS pairs of the next group in flight while the
+        // fmas of the current group run
+        if (lane == 0) {
+          int k = 0;
+          double t[8], u[8];
+          if (nj >= 8) {
 #pragma unroll
-          for (int i = 0; i < 8; i++) { t[i] = tl[i]; u[i] = ql[i]; }
+            for (int i = 0; i < 8; i++) { t[i] = tl[i]; u[i] = ql[i]; }
+          }
+          for (; k + 8 <= nj; k += 8) {
+            double tn[8], un[8];
+            const int kn = k + 16 <= nj ? k + 8 : k;
+#pragma unroll
+            for (int i = 0; i < 8; i++) { tn[i] = tl[kn + i]; un[i] = ql[kn + i]; }
+#pragma unroll
+            for (int i = 0; i < 8; i++) ab = __builtin_fma(t[i], u[i], ab);
+#pragma unroll
+            for (int i = 0; i < 8; i++) { t[i] = tn[i]; u[i] = un[i]; }
+          }
+          for (; k < nj; k++) ab = __builtin_fma(tl[k], ql[k], ab);
         }
-        for (; k + 8 <= nj; k += 8) {
-          double tn[8], un[8];
-          const int kn = k + 16 <= nj ? k + 8 : k;
-#pragma unroll
-          for (int i = 0; i < 8; i++) { tn[i] = tl[kn + i]; un[i] = ql[kn + i]; }
-#pragma unroll
-          for (int i = 0; i < 8; i++) ab = __builtin_fma(t[i], u[i], ab);
-#pragma unroll
-          for (int i = 0; i < 8; i++) { t[i] = tn[i]; u[i] = un[i]; }
-        }
-        for (; k < nj; k++) ab = __builtin_fma(tl[k], ql[k], ab);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
-      const double d = __builtin_fabs((a2map[c] - 2.0 * ab) + b2);
-      if (lane == 0) dex[c] = d;
-      const unsigned long long bits = (unsigned long long)__double_as_longlong(d);
-      best = bits < best ? bits : best;
+      if (lane == 0) {
+        const double d = __builtin_fabs((a2map[c] - 2.0 * ab) + b2);
+        dex[c] = d;
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(d);
+        best = bits < best ? bits : best;
+      }
     }
   }
-  for (int e0 = grp * 256; n > RC_SMALL && e0 < n; e0 += ngrp * 256) {  // uniform over the group
+  for (int e0 = grp * 256; n > rcs && e0 < n; e0 += ngrp * 256) {  // uniform over the group
     const int e = e0 + (int)threadIdx.x;
     const bool live = e < n;
     int64_t c = 0, base = 0;
@@ -996,6 +1003,88 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     for (int i = 1; i < 4; i++) b = wbest[i] < b ? wbest[i] : b;
     if (b != 0x7ff0000000000000ull) atomicMin(&D->gmin64[slot], b);
   }
+}
+
+// Exact recheck with a lane per member: the same ordered FMA chain (so the
+// same FP64 value), each lane walking its member's window.  The query values
+// and window offsets are the same for every lane and stream through scalar
+// loads; each lane keeps RCL_B window values of the next block in flight
+// while its chain runs over the current block.  A wavefront serves 64
+// members; grid (query, RCL_GROUPS wavefronts), members beyond loop.
+#define RCL_B 32
+#define RCL_GROUPS 16
+template <bool F32>
+__device__ __forceinline__ void rcl_fetch(const void* ti, int64_t base, const __attribute__((address_space(4))) int32_t* toff,
+                                          int j0, int V, double (&t)[RCL_B]) {
+#pragma unroll
+  for (int i = 0; i < RCL_B; i++) {
+    const int j = j0 + i < V ? j0 + i : V - 1;
+    t[i] = F32 ? (double)((const float*)ti)[base + toff[j]] : ((const double*)ti)[base + toff[j]];
+  }
+}
+
+template <bool F32>
+__device__ __forceinline__ double rcl_chain(const IqDev* __restrict__ D, int64_t base,
+                                            const __attribute__((address_space(4))) int32_t* toff,
+                                            const __attribute__((address_space(4))) double* kv, int V) {
+  double cur[RCL_B], nxt[RCL_B];
+  rcl_fetch<F32>(D->ti, base, toff, 0, V, cur);
+  double ab = 0.0;
+  for (int j0 = 0; j0 < V; j0 += RCL_B) {
+    if (j0 + RCL_B < V) rcl_fetch<F32>(D->ti, base, toff, j0 + RCL_B, V, nxt);
+    if (j0 + RCL_B <= V) {
+#pragma unroll
+      for (int i = 0; i < RCL_B; i++) ab = __builtin_fma(cur[i], kv[j0 + i], ab);
+    } else {
+      for (int i = 0; i < V - j0; i++) ab = __builtin_fma(cur[i], kv[j0 + i], ab);
+    }
+#pragma unroll
+    for (int i = 0; i < RCL_B; i++) cur[i] = nxt[i];
+  }
+  return ab;
+}
+
+__global__ void __launch_bounds__(64)
+k_band_recheck_lanes(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  const IqQuery q = qs[blockIdx.x];
+  const int n = D->nsup[q.slot];
+  if ((int)blockIdx.y * 64 >= n) return;
+  const IqStep& st = D->steps[q.step];
+  if (q.multi || st.mask < 0) return;
+  const IqGeo& g = D->g;
+  const int64_t slot = q.slot;
+  const IqMask& m = D->masks[st.mask];
+  const int V = m.V;
+  const auto toff = iq_const(D->toff + m.off);
+  const auto kv = iq_const(D->kv + slot * D->kv_stride);
+  const double* a2map = D->a2maps[m.a2];
+  const double b2 = D->b2[slot * (2 + D->naux)];
+  double* dex = D->dexact + slot * g.P;
+  const int lane = threadIdx.x;
+  unsigned long long best = 0x7ff0000000000000ull;
+  for (int e0 = blockIdx.y * 64; e0 < n; e0 += gridDim.y * 64) {  // uniform over the wavefront
+    const int e = e0 + lane;
+    const bool live = e < n;
+    int64_t c = 0, base = 0;
+    if (live) {
+      c = D->sup[slot * g.P + e];
+      const int cx = (int)(c % g.dist[0]), cy = (int)((c / g.dist[0]) % g.dist[1]);
+      const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
+      base = iq_lin3(g.ti, cx, cy, cz);
+    }
+    const double ab = D->ti_f32 ? rcl_chain<true>(D, base, toff, kv, V) : rcl_chain<false>(D, base, toff, kv, V);
+    if (live) {
+      const double d = __builtin_fabs((a2map[c] - 2.0 * ab) + b2);
+      dex[c] = d;
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(d);
+      best = bits < best ? bits : best;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long other = __shfl_xor(best, o);
+    best = other < best ? other : best;
+  }
+  if (lane == 0 && best != 0x7ff0000000000000ull) atomicMin(&D->gmin64[slot], best);
 }
 
 // ---------------------------------------------------------------------------

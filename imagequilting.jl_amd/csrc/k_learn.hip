@@ -77,11 +77,14 @@ k_sgemm(int M, int N, int K, const float* __restrict__ A, int64_t sam, int64_t s
     }
 }
 
-// Bank of one mask: bank[c * IQ_EMB + j] = bf16(sum_v W[j][v] * TI[window c
-// at voxel v]) (one work-item per candidate; W staged through LDS 64 voxels
-// at a time; rows of 64 bf16 are the A operand of the MFMA shortlist scan).
+// Bank of one mask: bank[c * IQ_EMB + j] = bf16(sum_v W[j][v] * X[window c
+// at voxel v]) with X the training image, or the convolutional encoder's
+// output map `src` (training-image extents) when given (one work-item per
+// candidate; W staged through LDS 64 voxels at a time; rows of 64 bf16 are
+// the A operand of the MFMA shortlist scan).
 __global__ void __launch_bounds__(256)
-k_learn_bank(const IqDev* __restrict__ D, int mask, const float* __restrict__ W, uint16_t* __restrict__ bank) {
+k_learn_bank(const IqDev* __restrict__ D, int mask, const float* __restrict__ W, const float* __restrict__ src,
+             uint16_t* __restrict__ bank) {
   __shared__ float wc[64][IQ_EMB];
   const IqGeo& g = D->g;
   const IqMask& m = D->masks[mask];
@@ -107,7 +110,7 @@ k_learn_bank(const IqDev* __restrict__ D, int mask, const float* __restrict__ W,
     if (live) {
       for (int vv = 0; vv < nv; vv++) {
         const int64_t o = base + toff[v0 + vv];
-        const float t = D->ti_f32 ? ((const float*)D->ti)[o] : (float)((const double*)D->ti)[o];
+        const float t = src ? src[o] : D->ti_f32 ? ((const float*)D->ti)[o] : (float)((const double*)D->ti)[o];
 #pragma unroll
         for (int j = 0; j < IQ_EMB; j++) acc[j] = __builtin_fmaf(wc[vv][j], t, acc[j]);
       }
@@ -173,20 +176,23 @@ k_learn_bfrag(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const
   }
 }
 
-// Approximate distance of candidate c for a query, as the MFMA scan forms
-// it: A2[c] - 2 sum_j bf16(E[c][j]) bf16(e_q[j]) + B2.
-__device__ inline float learn_score(const IqDev* D, int mask, int64_t c, const float* eq, double b2) {
+// Approximate distance of candidate c for a query, in FP32 as the MFMA bank
+// scan forms it: A2[c] - 2 sum_j bf16(E[c][j]) bf16(e_q[j]) + B2 (A2 from
+// the FP32 copy of the mask's A2 map, NaN for a disabled window).
+__device__ inline float learn_score(const IqDev* D, int mask, int64_t c, const float* eq, float b2) {
   const uint16_t* row = D->lrn.bank + D->lrn.bofs[mask] + c * IQ_EMB;
   float acc = 0.0f;
   for (int j = 0; j < IQ_EMB; j++) acc = __builtin_fmaf(bf16_val(row[j]), bf16_val(iq_bf16_rne(eq[j])), acc);
-  return (float)((D->a2maps[D->masks[mask].a2][c] - 2.0 * (double)acc) + b2);
+  return __builtin_fmaf(-2.0f, acc, D->a2f[D->masks[mask].a2][c]) + b2;
 }
 
 // Per query: a bracket [lo, hi] of order keys around the K-th smallest
 // approximate distance from a strided sample of LKS candidates (sorted in
 // LDS; hi = the sample quantile at three times K/P), the bin width of a
-// 2048-bin histogram over the bracket, and fresh per-query state
-// (histogram, candidate list, shortlist, bitmaps).  Grid: nq blocks of 1024.
+// 2048-bin histogram over the bracket, fresh per-query state (candidate
+// list, shortlist, bitmaps), and the bank scan's per-column word of the
+// query at its batch position (FP32 B2, lo, hi, shift).  Grid: nq blocks of
+// 1024.
 #define LKS 4096
 __global__ void __launch_bounds__(1024)
 k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
@@ -198,7 +204,7 @@ k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const IqGeo& g = D->g;
   const int64_t slot = q.slot;
   const float* eq = D->lrn.eq + slot * IQ_EMB;
-  const double b2 = D->b2[slot * (2 + D->naux)];
+  const float b2 = (float)D->b2[slot * (2 + D->naux)];
   const int64_t K = D->lrn.K < D->nen ? D->lrn.K : D->nen;
   if (threadIdx.x == 0) best = ~0ull;
   __syncthreads();
@@ -207,8 +213,6 @@ k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     smp[i] = D->disabled[c] ? 0xffffffffu : iq_f32_key(learn_score(D, st.mask, c, eq, b2));
     atomicMin(&best, ((unsigned long long)smp[i] << 32) | (unsigned)i);
   }
-  int32_t* hist = D->lrn.hist + slot * 2048;
-  for (int i = threadIdx.x; i < 2048; i += blockDim.x) hist[i] = 0;
   unsigned long long* bm = D->bitmap + slot * D->nw64;
   unsigned long long* sb = D->lrn.sbits + slot * D->nw64;
   for (int64_t w = threadIdx.x; w < D->nw64; w += blockDim.x) { bm[w] = 0ull; sb[w] = 0ull; }
@@ -233,6 +237,7 @@ k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     while (shift < 32 && (span >> shift) >= 2048u) shift++;
     uint32_t* b = D->lrn.brk + slot * 4;
     b[0] = lo; b[1] = hi; b[2] = shift;
+    D->lrn.binfo[blockIdx.x] = make_uint4(__float_as_uint(b2), lo, hi, shift);
     // seed of the shortlist should no candidate fall inside the bracket: the
     // sample minimum, or the first enabled candidate when every sampled
     // window is disabled
@@ -245,10 +250,13 @@ k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 
 // Shortlist scan on MFMA (v_mfma_f32_32x32x16_bf16): approximate distances
 // of 32 candidates x 32 queries of a tile per wavefront from the bf16 bank
-// rows (A) and the tile's B fragments; the few inside a query's bracket are
-// counted in its histogram and appended to its candidate list (order key
-// kept).  Each wavefront owns 32 consecutive candidates and walks all tiles
-// (tiles of one mask are adjacent, so bank rows load once per mask).
+// rows (A) and the tile's B fragments, in FP32 (A2 from the mask's FP32 map,
+// whose NaN marks a disabled window); the few inside a query's bracket are
+// appended to its candidate list with their order keys (one reservation per
+// lane and hit set; the histogram is built from the list afterwards).  Each
+// wavefront owns 32 consecutive candidates and walks all tiles (tiles of one
+// mask are adjacent, so bank rows load once per mask); a column's query word
+// (B2, bracket) is one load at its batch position.
 // Grid: ceil(P / 128) blocks of 4 wavefronts.
 __global__ void __launch_bounds__(256)
 k_learn_mscan(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const int4* __restrict__ tiles, int ntiles) {
@@ -257,7 +265,7 @@ k_learn_mscan(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const
   const int64_t c0 = (int64_t)blockIdx.x * 128 + w * 32;
   const int kb = lane >> 5;
   iq_bf16x8 a[IQ_EMB / 16];
-  double a2[16];
+  float a2[16];
   uint32_t offm = 0;  // candidate rows of this lane that are disabled or past P
   int cur = -1;
   const int64_t LC = D->lrn.lcap;
@@ -273,35 +281,30 @@ k_learn_mscan(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const
         if (ca < g.P) v = *(const uint4*)(bank + ca * IQ_EMB + 16 * ks + 8 * kb);
         a[ks] = __builtin_bit_cast(iq_bf16x8, v);
       }
-      // A2 of the lane's 16 candidate rows and whether each is live: all 32
-      // loads in flight at once (the FP32 map's NaN marks a disabled window)
-      const int ma = D->masks[cur].a2;
-      const auto a2m = iq_glob(D->a2maps[ma]);
-      const auto a2f = iq_glob(D->a2f[ma]);
+      // A2 of the lane's 16 candidate rows (all 16 loads in flight) and
+      // whether each is live
+      const auto a2f = iq_glob(D->a2f[D->masks[cur].a2]);
       float fl[16];
-      double ad[16];
 #pragma unroll
       for (int v = 0; v < 16; v++) {
         const int64_t c = c0 + 8 * (v >> 2) + 4 * kb + (v & 3);
-        const int64_t cc = c < g.P ? c : g.P - 1;
-        fl[v] = a2f[cc];
-        ad[v] = a2m[cc];
+        fl[v] = a2f[c < g.P ? c : g.P - 1];
       }
       offm = 0;
 #pragma unroll
       for (int v = 0; v < 16; v++) {
         const int64_t c = c0 + 8 * (v >> 2) + 4 * kb + (v & 3);
         const bool ok = c < g.P && fl[v] == fl[v];
-        a2[v] = ok ? ad[v] : 0.0;
+        a2[v] = ok ? fl[v] : 0.0f;
         offm |= ok ? 0u : (1u << v);
       }
     }
-    // the lane's query (slot, B2, bracket) is fetched before the MFMAs run
-    const bool qlive = (lane & 31) < tl.z;
-    const int64_t slot = qs[tl.y + (qlive ? (lane & 31) : 0)].slot;
-    const double b2 = D->b2[slot * (2 + D->naux)];
-    const uint32_t* br = D->lrn.brk + slot * 4;
-    const uint32_t lo = br[0], hi = br[1], shift = br[2];
+    // the column's query word is fetched before the MFMAs run
+    const int col = lane & 31;
+    const bool qlive = col < tl.z;
+    const uint4 bi = D->lrn.binfo[tl.y + (qlive ? col : 0)];
+    const float b2 = __uint_as_float(bi.x);
+    const uint32_t hi = bi.z;
     iq_f32x16 acc;
 #pragma unroll
     for (int v = 0; v < 16; v++) acc[v] = 0.0f;
@@ -314,38 +317,52 @@ k_learn_mscan(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const
     uint32_t in = 0;
 #pragma unroll
     for (int v = 0; v < 16; v++) {
-      keys[v] = iq_f32_key((float)((a2[v] - 2.0 * (double)acc[v]) + b2));
+      keys[v] = iq_f32_key(__builtin_fmaf(-2.0f, acc[v], a2[v]) + b2);
       in |= (!(offm & (1u << v)) && keys[v] <= hi) ? (1u << v) : 0u;
     }
     if (!in) continue;
+    const int64_t slot = qs[tl.y + col].slot;
     int idx = atomicAdd(&D->lrn.lcnt[slot], __builtin_popcount(in));  // one reservation per lane
 #pragma unroll
     for (int v = 0; v < 16; v++) {
       if (!(in & (1u << v))) continue;
-      const uint32_t k = keys[v];
-      atomicAdd(&D->lrn.hist[slot * 2048 + (k <= lo ? 0 : (k - lo) >> shift)], 1);
-      if (idx < LC) D->lrn.list[slot * LC + idx] = make_int2((int)(c0 + 8 * (v >> 2) + 4 * kb + (v & 3)), (int)k);
+      if (idx < LC) D->lrn.list[slot * LC + idx] = make_int2((int)(c0 + 8 * (v >> 2) + 4 * kb + (v & 3)), (int)keys[v]);
       idx++;
     }
   }
 }
 
-// Per query: the threshold key where the histogram count reaches K (the
-// bracket's top when the sample underestimated it), then the shortlist: the
-// listed candidates at or below it (list and bitmap).  Grid: nq blocks of 1024.
+// Per query: a 2048-bin histogram (in LDS) of the listed order keys over the
+// bracket, the threshold key where its count reaches K (the bracket's top
+// when the sample underestimated it), then the shortlist: the listed
+// candidates at or below it (list and bitmap).  Grid: nq blocks of 1024.
 __global__ void __launch_bounds__(1024)
 k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   __shared__ uint32_t thr;
+  __shared__ int32_t hist[2048];
   const IqQuery q = qs[blockIdx.x];
   const IqStep& st = D->steps[q.step];
   if (!learn_applies(D, q, st)) return;
   const IqGeo& g = D->g;
   const int64_t slot = q.slot;
   const int64_t K = D->lrn.K < D->nen ? D->lrn.K : D->nen;
+  const int64_t LC = D->lrn.lcap;
+  const int64_t n = D->lrn.lcnt[slot] < LC ? D->lrn.lcnt[slot] : LC;
+  const int2* list = D->lrn.list + slot * LC;
+  {
+    const uint32_t* br = D->lrn.brk + slot * 4;
+    const uint32_t lo = br[0], shift = br[2];
+    for (int i = threadIdx.x; i < 2048; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+      const uint32_t k = (uint32_t)list[e].y;
+      atomicAdd(&hist[k <= lo ? 0 : (k - lo) >> shift], 1);
+    }
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     const uint32_t* br = D->lrn.brk + slot * 4;
     const uint32_t lo = br[0], hi = br[1], shift = br[2];
-    const int32_t* hist = D->lrn.hist + slot * 2048;
     int64_t run = 0;
     uint32_t th = hi;
     for (int b = 0; b < 2048; b++) {
@@ -359,9 +376,6 @@ k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     thr = th;
   }
   __syncthreads();
-  const int64_t LC = D->lrn.lcap;
-  const int64_t n = D->lrn.lcnt[slot] < LC ? D->lrn.lcnt[slot] : LC;
-  const int2* list = D->lrn.list + slot * LC;
   const int lane = threadIdx.x & 63;
   for (int64_t e0 = 0; e0 < n; e0 += blockDim.x) {
     const int64_t e = e0 + threadIdx.x;

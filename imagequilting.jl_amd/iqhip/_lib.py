@@ -35,7 +35,10 @@ EXPORTS = (
     "iq_simulate", "iq_prepare", "iq_prepare_soft", "iq_finddisabled",
     "iq_plan_learn", "iq_plan_set_proposal", "iq_plan_proposal_stats", "iq_plan_encoder", "iq_plan_select_stats",
     "iq_plan_validate", "iq_scan_error_bound", "iq_plan_bank_stats", "iq_graphcut_noalloc",
+    "iq_plan_learn_conv", "iq_plan_conv_stats", "iq_plan_conv_params", "iq_plan_conv_grad", "iq_plan_bank_rows",
 )
+
+IQ_CONV_NPARAM = 29441  # include/iqhip.h
 
 IQ_NOALLOC_MAX_NODES = 16384  # include/iqhip.h
 
@@ -69,6 +72,19 @@ class PlanDesc(ctypes.Structure):
     ]
 
 
+class ConvOpts(ctypes.Structure):
+    _fields_ = [
+        ("nqueries", ctypes.c_int64),
+        ("topn", ctypes.c_int),
+        ("nrandom", ctypes.c_int),
+        ("steps", ctypes.c_int),
+        ("batch", ctypes.c_int),
+        ("lr", ctypes.c_float),
+        ("lr_head", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -97,6 +113,11 @@ def load(path: str = LIB_PATH):
             "iq_sample": [vp, vp, i64, f64, vp],
             "iq_graphcut": [vp, vp, vp, c_int, vp, c_int, vp],
             "iq_graphcut_noalloc": [vp, vp, vp, c_int, vp, c_int, vp],
+            "iq_plan_learn_conv": [vp, vp, vp],
+            "iq_plan_conv_stats": [vp, vp],
+            "iq_plan_conv_params": [vp, vp],
+            "iq_plan_conv_grad": [vp, vp, c_int, vp, i64, i64, vp, vp, vp, vp, vp, vp],
+            "iq_plan_bank_rows": [vp, vp, c_int, vp, i64, vp],
             "iq_plan_create": [vp, vp, P(vp)],
             "iq_plan_create_multi": [vp, vp, c_int, P(vp)],
             "iq_plan_validate": [vp, c_int],

@@ -364,6 +364,62 @@ class Plan:
         self.ctx.check(self.ctx._L.iq_plan_encoder(self.handle, int(mask), W.ctypes.data, tv.ctypes.data, None, None))
         return W, tv
 
+    # -- convolutional encoder (no counterpart in the reference)
+    def learn_conv(self, nqueries: int = 640, topn: int = 256, nrandom: int = 64, steps: int = 200, batch: int = 16,
+                   lr: float = 1e-3, lr_head: float = 0.0, seed: int = 0) -> dict:
+        """Train the convolutional encoder on (query, exact distance) pairs of
+        the last run's grids and rebuild the banks from it (iq_plan_learn_conv);
+        returns its statistics."""
+        o = _lib.ConvOpts(int(nqueries), int(topn), int(nrandom), int(steps), int(batch), float(lr), float(lr_head),
+                          int(seed))
+        self.ctx.check(self.ctx._L.iq_plan_learn_conv(self.ctx.handle, self.handle, ctypes.byref(o)))
+        return self.conv_stats()
+
+    def conv_stats(self) -> dict:
+        v = np.zeros(10, dtype=np.float64)
+        self.ctx.check(self.ctx._L.iq_plan_conv_stats(self.handle, v.ctypes.data))
+        return dict(harvest_seconds=float(v[0]), train_seconds=float(v[1]), loss_first=float(v[2]),
+                    loss_last=float(v[3]), pairs=int(v[4]), steps=int(v[5]), queries=int(v[6]), kept_step=int(v[7]),
+                    heldout_loss_start=float(v[8]), heldout_loss_kept=float(v[9]))
+
+    def conv_params(self) -> np.ndarray:
+        """The network's parameters (IQ_CONV_NPARAM floats)."""
+        th = np.zeros(_lib.IQ_CONV_NPARAM, dtype=np.float32)
+        self.ctx.check(self.ctx._L.iq_plan_conv_params(self.handle, th.ctypes.data))
+        return th
+
+    def conv_grad(self, mask: int, theta, Q, cands, D, tau):
+        """(loss, gradient) of the ranking loss for parameters theta (network,
+        then the mask's head) on the given pairs; the plan is not changed."""
+        theta = np.ascontiguousarray(theta, dtype=np.float32)
+        Q = np.ascontiguousarray(Q, dtype=np.float32)
+        cands = np.ascontiguousarray(cands, dtype=np.int32)
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        tau = np.ascontiguousarray(tau, dtype=np.float64)
+        nq, ns = cands.shape
+        loss = ctypes.c_double(0.0)
+        grad = np.zeros_like(theta)
+        self.ctx.check(self.ctx._L.iq_plan_conv_grad(self.ctx.handle, self.handle, int(mask), theta.ctypes.data, nq, ns,
+                                                     Q.ctypes.data, cands.ctypes.data, D.ctypes.data, tau.ctypes.data,
+                                                     ctypes.byref(loss), grad.ctypes.data))
+        return loss.value, grad
+
+    def bank_rows(self, mask: int, cands) -> np.ndarray:
+        """Bank rows (n x 64, bf16 values as float) of the given candidates."""
+        cands = np.ascontiguousarray(cands, dtype=np.int32)
+        out = np.zeros((cands.size, 64), dtype=np.float32)
+        self.ctx.check(self.ctx._L.iq_plan_bank_rows(self.ctx.handle, self.handle, int(mask), cands.ctypes.data,
+                                                     cands.size, out.ctypes.data))
+        return out
+
+    def mask_voxels(self, mask: int) -> np.ndarray:
+        """Column-major tile offsets (ascending) of one overlap-mask shape."""
+        V = ctypes.c_int64(0)
+        self.ctx.check(self.ctx._L.iq_plan_encoder(self.handle, int(mask), None, None, ctypes.byref(V), None))
+        tv = np.zeros(V.value, dtype=np.int32)
+        self.ctx.check(self.ctx._L.iq_plan_encoder(self.handle, int(mask), None, tv.ctypes.data, None, None))
+        return tv
+
     def nmasks(self) -> int:
         nm = ctypes.c_int(0)
         self.ctx._L.iq_plan_encoder(self.handle, 0, None, None, None, ctypes.byref(nm))

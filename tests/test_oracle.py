@@ -539,3 +539,57 @@ def test_relaxation_of_all_infinite_distances_is_refused():
     # relaxation.jl:7-36 never terminates when no distance is finite
     with pytest.raises(RuntimeError, match="-3"):
         iqref.relaxation(np.full(9, np.inf), [np.full(9, np.inf)], 0.1)
+
+
+# ------------------------------------------------ learned encoder (no reference counterpart)
+def test_learnref_gradient_matches_central_differences():
+    """The conv encoder oracle's analytic gradient (every parameter group)
+    against central differences of its own loss, FP64."""
+    import learnref as L
+
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((7, 6, 5))
+    toff = np.array([(i, j, k) for k in range(2) for j in range(3) for i in range(3) if i < 1 or j < 1])
+    V = len(toff)
+    th = rng.standard_normal(L.nparams(V)) * 0.3
+    nq, ns, dist = 2, 5, (5, 4, 4)
+    cands = np.stack([np.stack([rng.integers(0, d, ns) for d in dist], -1) for _ in range(nq)])
+    Q = rng.standard_normal((nq, V))
+    D = rng.random((nq, ns)) * 3
+    A2 = rng.random((nq, ns)) * 2
+    tau = np.array([0.7, 1.3])
+    loss, g, _ = L.loss_and_grad(x, th, toff, A2, Q, cands, D, tau)
+    assert abs(loss - L.loss_only(x, th, toff, A2, Q, cands, D, tau)) < 1e-12 * abs(loss)
+    C = L.C
+    starts = [0, C * 27, C * 27 + C, C * 27 + C + C * C * 27, C * 27 + 2 * C + C * C * 27,
+              L.NCONV - 1, L.NCONV]
+    idx = np.concatenate([rng.choice(L.nparams(V), 40, replace=False), starts, [L.nparams(V) - 1]])
+    h = 1e-6
+    for i in idx:
+        e = np.zeros_like(th)
+        e[i] = h
+        fd = (L.loss_only(x, th + e, toff, A2, Q, cands, D, tau) - L.loss_only(x, th - e, toff, A2, Q, cands, D, tau)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * np.abs(g).max(), i
+
+
+def test_learnref_conv_adjoints():
+    """<g, conv(a, w)> = <conv_adjoint_input(g, w), a> = <conv_adjoint_weight(g, a), w>,
+    and a single centre tap is the identity map."""
+    import learnref as L
+
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((3, 5, 4, 3))
+    w = rng.standard_normal((2, 3, 27))
+    g = rng.standard_normal((2, 5, 4, 3))
+    lhs = (g * L.conv(a, w)).sum()
+    assert abs(lhs - (L.conv_adjoint_input(g, w) * a).sum()) < 1e-10 * abs(lhs)
+    assert abs(lhs - (L.conv_adjoint_weight(g, a) * w).sum()) < 1e-10 * abs(lhs)
+    eye = np.zeros((3, 3, 27))
+    for c in range(3):
+        eye[c, c, 13] = 1.0
+    assert np.array_equal(L.conv(a, eye), a)
+    # an x shift by one voxel, zero at the border
+    sh = np.zeros((1, 1, 27))
+    sh[0, 0, 14] = 1.0  # d = (+1, 0, 0)
+    out = L.conv(a[:1], sh)[0]
+    assert np.array_equal(out[:-1], a[0, 1:]) and not out[-1].any()
