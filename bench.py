@@ -279,11 +279,11 @@ def main():
             plan.run(U[0])
             st_lin = plan.proposal_stats()
             plan.set_proposal(False)
-            # every batch was decided by the full scan: the grids are the plain
-            # method's, and the training queries come from them
+            # training pairs: the encoder runs the full scan itself and takes
+            # tile steps' queries as they run
             t0 = time.perf_counter()
             conv = plan.learn_conv(nqueries=640, topn=256, nrandom=64, steps=args.conv_steps, batch=16, lr=1e-3,
-                                   lr_head=0.0, seed=1)
+                                   lr_head=0.0, seed=1, val_k=args.K)
             conv["seconds"] = time.perf_counter() - t0
             plan.set_proposal(True, K=args.K, check_every=1, min_recall=0.0)
             plan.run(U[0])

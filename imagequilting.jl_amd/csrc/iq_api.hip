@@ -371,11 +371,10 @@ struct iq_plan {
   std::vector<int64_t> lrn_wofs, lrn_bofs;
   int lrn_vmax = 0;
   // convolutional encoder of the learned proposal
-  bool has_run = false;         // grids hold a finished run (training queries come from them)
   bool cv_ready = false;        // banks built from the trained network's output map
   float* cv_theta = nullptr;    // CV_NW convolution parameters
   double cv_stats[8] = {};      // see iq_plan_conv_stats
-  double cv_val[2] = {};        // held-out loss at the start and of the kept parameters
+  double cv_val[4] = {};        // held-out loss and shortlist recall at the start and of the kept parameters
   float last_ms = 0.0f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // a run's launch sequence captured once and replayed (runs that take no
@@ -1302,6 +1301,63 @@ void plan_run_launches(iq_plan* pl, const std::vector<IqQuery>& qs, const std::v
                        const std::vector<int4>& tiles, bool learned);
 void plan_run_finish(iq_plan* pl);
 
+// The query table of a run: wave k of every grid, all realizations, in
+// batches of at most nslots queries (each query's slot is its batch
+// position); with the learned proposal, queries of one mask side by side and
+// the shortlist tiles of each batch.
+void plan_tables(iq_plan* pl, const double* uniforms, const int64_t* forced, bool learned, std::vector<IqQuery>& qs,
+                 std::vector<int4>& tiles, std::vector<RunBatch>& batches) {
+  for (size_t k = 0; k < pl->nwaves; k++) {
+    // lockstep: wave k of every grid, all realizations
+    std::vector<IqQuery> wq;
+    for (const HostGrid& G : pl->grids) {
+      if (k >= G.waves.size()) continue;
+      for (int64_t r = G.real0; r < G.real0 + G.nreal; r++)
+        for (int64_t i : G.waves[k]) {
+          const int64_t st = G.step0 + i, lo = pl->reals[r].log_off + i;
+          IqQuery q{};
+          q.step = (int32_t)st;
+          q.real = (int32_t)r;
+          q.multi = (pl->steps[st].hard || pl->naux > 0) ? 1 : 0;
+          q.u = uniforms ? uniforms[lo] : 0.5;
+          q.forced = forced ? forced[lo] : -1;
+          wq.push_back(q);
+        }
+    }
+    if (learned)  // queries of one mask side by side: one bank row load serves them
+      std::stable_sort(wq.begin(), wq.end(), [&](const IqQuery& a, const IqQuery& b) {
+        return pl->steps[a.step].mask < pl->steps[b.step].mask;
+      });
+    for (size_t b0 = 0; b0 < wq.size(); b0 += (size_t)pl->nslots) {
+      RunBatch bt{(int64_t)qs.size(), 0, false, false, 0, 0};
+      for (size_t k = b0; k < wq.size() && k < b0 + (size_t)pl->nslots; k++) {
+        IqQuery q = wq[k];
+        q.slot = (int32_t)(k - b0);
+        bt.plain |= !q.multi;
+        bt.multi |= q.multi != 0;
+        qs.push_back(q);
+        bt.n++;
+      }
+      if (learned) {
+        // tiles of up to 32 consecutive plain queries of one overlap mask
+        bt.toff = (int64_t)tiles.size();
+        for (int i = 0; i < bt.n; i++) {
+          const IqQuery& q = qs[bt.off + i];
+          const int mk = pl->steps[q.step].mask;
+          if (q.multi || mk < 0) continue;
+          if (bt.nt > 0 && tiles.back().x == mk && tiles.back().z < 32 && tiles.back().y + tiles.back().z == i) {
+            tiles.back().z++;
+          } else {
+            tiles.push_back(make_int4(mk, i, 1, 0));
+            bt.nt++;
+          }
+        }
+      }
+      batches.push_back(bt);
+    }
+  }
+}
+
 template <typename T>
 void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   hipStream_t s = pl->ctx->stream;
@@ -1324,55 +1380,7 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   std::vector<IqQuery> qs;
   std::vector<int4> tiles;
   std::vector<RunBatch> batches;
-  for (size_t k = 0; k < pl->nwaves; k++) {
-    // lockstep: wave k of every grid, all realizations
-    std::vector<IqQuery> wq;
-    for (const HostGrid& G : pl->grids) {
-      if (k >= G.waves.size()) continue;
-      for (int64_t r = G.real0; r < G.real0 + G.nreal; r++)
-        for (int64_t i : G.waves[k]) {
-          const int64_t st = G.step0 + i, lo = pl->reals[r].log_off + i;
-          IqQuery q{};
-          q.step = (int32_t)st;
-          q.real = (int32_t)r;
-          q.multi = (pl->steps[st].hard || pl->naux > 0) ? 1 : 0;
-          q.u = uniforms ? uniforms[lo] : 0.5;
-          q.forced = forced ? forced[lo] : -1;
-          wq.push_back(q);
-        }
-    }
-    if (pl->lrn_ready && pl->lrn_on)  // queries of one mask side by side: one bank row load serves them
-      std::stable_sort(wq.begin(), wq.end(), [&](const IqQuery& a, const IqQuery& b) {
-        return pl->steps[a.step].mask < pl->steps[b.step].mask;
-      });
-    for (size_t b0 = 0; b0 < wq.size(); b0 += (size_t)pl->nslots) {
-      RunBatch bt{(int64_t)qs.size(), 0, false, false, 0, 0};
-      for (size_t k = b0; k < wq.size() && k < b0 + (size_t)pl->nslots; k++) {
-        IqQuery q = wq[k];
-        q.slot = (int32_t)(k - b0);
-        bt.plain |= !q.multi;
-        bt.multi |= q.multi != 0;
-        qs.push_back(q);
-        bt.n++;
-      }
-      if (pl->lrn_ready && pl->lrn_on) {
-        // tiles of up to 32 consecutive plain queries of one overlap mask
-        bt.toff = (int64_t)tiles.size();
-        for (int i = 0; i < bt.n; i++) {
-          const IqQuery& q = qs[bt.off + i];
-          const int mk = pl->steps[q.step].mask;
-          if (q.multi || mk < 0) continue;
-          if (bt.nt > 0 && tiles.back().x == mk && tiles.back().z < 32 && tiles.back().y + tiles.back().z == i) {
-            tiles.back().z++;
-          } else {
-            tiles.push_back(make_int4(mk, i, 1, 0));
-            bt.nt++;
-          }
-        }
-      }
-      batches.push_back(bt);
-    }
-  }
+  plan_tables(pl, uniforms, forced, pl->lrn_ready && pl->lrn_on, qs, tiles, batches);
   if ((int64_t)qs.size() > pl->dq_cap) {
     pl->dq = pl->mem.alloc<IqQuery>(qs.size());
     pl->dq_cap = (int64_t)qs.size();
@@ -1431,11 +1439,9 @@ void plan_run(iq_plan* pl, const double* uniforms, const int64_t* forced) {
   plan_run_finish(pl);
 }
 
-// Every launch of one run, in order: resets, then the batches.  Host decisions
-// between batches (the recall checks' fallback) read back device counters.
+// State a run starts from: error word, counters, pick log, empty grids.
 template <typename T>
-void plan_run_launches(iq_plan* pl, const std::vector<IqQuery>& qs, const std::vector<RunBatch>& batches,
-                       const std::vector<int4>& tiles, bool learned) {
+void plan_resets(iq_plan* pl) {
   hipStream_t s = pl->ctx->stream;
   IqDev& H = pl->hd;
   hipLaunchKernelGGL((k_fill<int32_t>), dim3(1), dim3(64), 0, s, H.err, (int64_t)1, 0);
@@ -1445,6 +1451,16 @@ void plan_run_launches(iq_plan* pl, const std::vector<IqQuery>& qs, const std::v
   hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s, H.cuts, pl->nvox, (uint8_t)0);
   hipLaunchKernelGGL((k_fill<int64_t>), dim3(16), dim3(256), 0, s, H.pick, pl->nslots, (int64_t)-1);
   HCL();
+}
+
+// Every launch of one run, in order: resets, then the batches.  Host decisions
+// between batches (the recall checks' fallback) read back device counters.
+template <typename T>
+void plan_run_launches(iq_plan* pl, const std::vector<IqQuery>& qs, const std::vector<RunBatch>& batches,
+                       const std::vector<int4>& tiles, bool learned) {
+  hipStream_t s = pl->ctx->stream;
+  IqDev& H = pl->hd;
+  plan_resets<T>(pl);
   pl->lrn_fell_back = false;
   for (int k = 0; k < 4; k++) pl->lrn_stats[k] = 0;
   if (learned) {
@@ -1498,7 +1514,6 @@ void plan_run_finish(iq_plan* pl) {
   int32_t code = 0;
   HC(hipMemcpy(&code, H.err, sizeof(int32_t), hipMemcpyDeviceToHost));
   HC(hipMemcpy(pl->cut_stats, H.rounds, 6 * sizeof(int64_t), hipMemcpyDeviceToHost));
-  pl->has_run = true;
   if (pl->lrn_ready && pl->lrn_on) {
     unsigned long long st[4];
     HC(hipMemcpy(st, H.lrn.stats, sizeof(st), hipMemcpyDeviceToHost));
@@ -1781,45 +1796,57 @@ void adam(hipStream_t s, int64_t n, float* p, const float* g, float* m, float* v
 struct ConvSet {
   int ns = 0;
   std::vector<int> mask;
-  std::vector<int64_t> qoff;
-  std::vector<float> Q;
+  std::vector<int32_t> step, real;  // the tile step and realization of each query
+  std::vector<int64_t> qoff;  // each query's values start at kv[qoff]
+  std::vector<double> kv, b2;  // query values (FP64, V of its mask) and B2
   std::vector<int32_t> cands;
   std::vector<double> D, tau;
 };
 
+// Device scratch of the training-pair harvest (one batch of up to nb queries).
+struct HarvestWs {
+  DMem t;
+  int nb = 0, cap = 0, nrand = 0;
+  IqQuery* dq = nullptr;
+  int32_t *hist = nullptr, *list = nullptr, *cnt = nullptr, *dslot = nullptr, *dcand = nullptr;
+  uint32_t* sel = nullptr;
+  double *dex = nullptr, *dkv = nullptr;
+  void init(iq_plan* pl, int topn, int nrand_) {
+    nb = (int)std::min<int64_t>(pl->nslots, 256);
+    cap = 4 * topn;
+    nrand = nrand_;
+    dq = t.alloc<IqQuery>(nb);
+    hist = t.alloc<int32_t>((size_t)nb * CVT_BINS);
+    sel = t.alloc<uint32_t>((size_t)nb * 4);
+    list = t.alloc<int32_t>((size_t)nb * cap);
+    cnt = t.alloc<int32_t>(nb);
+    dslot = t.alloc<int32_t>(nb);
+    dcand = t.alloc<int32_t>((size_t)nb * (cap + nrand));
+    dex = t.alloc<double>((size_t)nb * (cap + nrand));
+  }
+};
+
+// Training pairs of the plain tile steps `pool` (step, realization), read from
+// the grids' current state (so a query is what its tile step sees when it
+// runs): the query values (FP64) and B2, the `topn` nearest candidates by
+// exact FP64 distance among those the full scan's lower bounds put first,
+// and `nrand` random enabled candidates with their exact distances.
 template <typename T>
-void conv_harvest(iq_plan* pl, int64_t nq_total, int topn, int nrand, std::mt19937_64& rng, ConvSet& set) {
+void harvest_queries(iq_plan* pl, std::vector<std::pair<int32_t, int32_t>> pool, int topn, int nrand,
+                     std::mt19937_64& rng, HarvestWs& ws, ConvSet& set) {
   hipStream_t s = pl->ctx->stream;
   const IqGeo& g = pl->geo;
   const IqDev* D = pl->dd;
-  // plain tile steps with an overlap mask, over every realization
-  std::vector<std::pair<int32_t, int32_t>> pool;  // (step, realization)
-  for (const HostGrid& G : pl->grids)
-    for (int64_t r = G.real0; r < G.real0 + G.nreal; r++)
-      for (int64_t i = 0; i < G.npath; i++) {
-        const IqStep& st = pl->steps[G.step0 + i];
-        if (st.active && st.mask >= 0 && !st.hard && pl->naux == 0) pool.push_back({(int32_t)(G.step0 + i), (int32_t)r});
-      }
-  if (pool.empty()) bad(IQ_ERR_ARG, "convolutional encoder: the plan has no plain tile step with an overlap");
-  std::shuffle(pool.begin(), pool.end(), rng);
-  if ((int64_t)pool.size() > nq_total) pool.resize((size_t)nq_total);
   std::stable_sort(pool.begin(), pool.end(), [&](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) {
     return pl->steps[a.first].mask < pl->steps[b.first].mask;
   });
-  const int cap = 4 * topn;
+  const int cap = ws.cap, nb = ws.nb;
   set.ns = topn + nrand;
   std::uniform_int_distribution<size_t> pick(0, pl->enabled_host.size() - 1);
-  const int nb = (int)std::min<int64_t>(pl->nslots, 256);
-  DMem t;
-  IqQuery* dq = t.alloc<IqQuery>(nb);
-  int32_t* hist = t.alloc<int32_t>((size_t)nb * CVT_BINS);
-  uint32_t* sel = t.alloc<uint32_t>((size_t)nb * 4);
-  int32_t* list = t.alloc<int32_t>((size_t)nb * cap);
-  int32_t* cnt = t.alloc<int32_t>(nb);
-  int32_t* dslot = t.alloc<int32_t>(nb);
-  int32_t* dcand = t.alloc<int32_t>((size_t)nb * (cap + nrand));
-  double* dex = t.alloc<double>((size_t)nb * (cap + nrand));
-  float* dQ = t.alloc<float>((size_t)nb * std::max(pl->lrn_vmax, 1));
+  IqQuery* dq = ws.dq;
+  int32_t *hist = ws.hist, *list = ws.list, *cnt = ws.cnt, *dslot = ws.dslot, *dcand = ws.dcand;
+  uint32_t* sel = ws.sel;
+  double* dex = ws.dex;
   for (size_t b0 = 0, b1 = 0; b0 < pool.size(); b0 = b1) {
     // queries of one mask per batch (exact distances and values per mask)
     const int m = pl->steps[pool[b0].first].mask;
@@ -1896,12 +1923,15 @@ void conv_harvest(iq_plan* pl, int64_t nq_total, int topn, int nrand, std::mt199
     }
     HC(hipMemcpyAsync(dcand, hc.data(), hc.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_cv_exact, dim3(w, nq), dim3(64), 0, s, D, m, (const int32_t*)dslot, (const int32_t*)dcand, w, dex);
-    hipLaunchKernelGGL(k_cv_queries, dim3(nq), dim3(256), 0, s, D, V, (const int32_t*)dslot, dQ);
     HCL();
     std::vector<double> hd((size_t)nq * w);
-    std::vector<float> hQ((size_t)nq * V);
+    std::vector<double> hkv((size_t)nq * V), hb2((size_t)nq);
     HC(hipMemcpyAsync(hd.data(), dex, hd.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    HC(hipMemcpyAsync(hQ.data(), dQ, hQ.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    for (int i = 0; i < nq; i++) {
+      HC(hipMemcpyAsync(hkv.data() + (size_t)i * V, pl->hd.kv + (int64_t)i * pl->hd.kv_stride, V * sizeof(double),
+                        hipMemcpyDeviceToHost, s));
+      HC(hipMemcpyAsync(hb2.data() + i, pl->hd.b2 + (int64_t)i * (2 + pl->naux), sizeof(double), hipMemcpyDeviceToHost, s));
+    }
     HC(hipStreamSynchronize(s));
     for (int i = 0; i < nq; i++) {
       if (nl[i] < topn) continue;  // fewer enabled candidates than topn
@@ -1911,8 +1941,11 @@ void conv_harvest(iq_plan* pl, int64_t nq_total, int topn, int nrand, std::mt199
       const int32_t* ci = hc.data() + (size_t)i * w;
       std::sort(ord.begin(), ord.end(), [&](int a, int b) { return di[a] < di[b] || (di[a] == di[b] && ci[a] < ci[b]); });
       set.mask.push_back(m);
-      set.qoff.push_back((int64_t)set.Q.size());
-      set.Q.insert(set.Q.end(), hQ.begin() + (size_t)i * V, hQ.begin() + (size_t)(i + 1) * V);
+      set.step.push_back(hq[i].step);
+      set.real.push_back(hq[i].real);
+      set.qoff.push_back((int64_t)set.kv.size());
+      set.kv.insert(set.kv.end(), hkv.begin() + (size_t)i * V, hkv.begin() + (size_t)(i + 1) * V);
+      set.b2.push_back(hb2[i]);
       for (int j = 0; j < topn; j++) {
         set.cands.push_back(ci[ord[j]]);
         set.D.push_back(di[ord[j]]);
@@ -1927,7 +1960,155 @@ void conv_harvest(iq_plan* pl, int64_t nq_total, int topn, int nrand, std::mt199
       set.tau.push_back(std::max({pl->tol * d0, d8 - d0, 1e-3 * d8, 1e-30}));
     }
   }
+}
+
+
+// Training pairs from the plain method at work: a full-scan run of every
+// realization (uniforms from rng; the grids then hold its result) in which
+// about nq_total plain tile steps, spread over up to 48 batches, are
+// harvested right before their batch runs.
+template <typename T>
+void conv_harvest(iq_plan* pl, int64_t nq_total, int topn, int nrand, std::mt19937_64& rng, ConvSet& set) {
+  hipStream_t s = pl->ctx->stream;
+  std::vector<double> U((size_t)pl->nlog);
+  std::uniform_real_distribution<double> uni(0.0, 1.0);
+  for (double& u : U) u = uni(rng);
+  std::vector<IqQuery> qs;
+  std::vector<int4> tiles;
+  std::vector<RunBatch> batches;
+  plan_tables(pl, U.data(), nullptr, false, qs, tiles, batches);
+  auto eligible = [&](const IqQuery& q) {
+    const IqStep& st = pl->steps[q.step];
+    return !q.multi && st.active && st.mask >= 0;
+  };
+  std::vector<int> hb;  // batches with plain queries that have an overlap
+  for (int b = 0; b < (int)batches.size(); b++)
+    for (int i = 0; i < batches[b].n; i++)
+      if (eligible(qs[batches[b].off + i])) { hb.push_back(b); break; }
+  if (hb.empty()) bad(IQ_ERR_ARG, "convolutional encoder: the plan has no plain tile step with an overlap");
+  const int npts = (int)std::min<size_t>(48, hb.size());
+  std::vector<int> pts;  // evenly spread over the run
+  for (int k = 0; k < npts; k++) pts.push_back(hb[(size_t)k * hb.size() / npts]);
+  const int64_t per = (nq_total + npts - 1) / npts;
+  if ((int64_t)qs.size() > pl->dq_cap) {
+    pl->dq = pl->mem.alloc<IqQuery>(qs.size());
+    pl->dq_cap = (int64_t)qs.size();
+  }
+  HC(hipMemcpyAsync(pl->dq, qs.data(), qs.size() * sizeof(IqQuery), hipMemcpyHostToDevice, s));
+  HarvestWs ws;
+  ws.init(pl, topn, nrand);
+  plan_resets<T>(pl);
+  size_t next = 0;
+  int64_t left = nq_total;
+  for (int b = 0; b < (int)batches.size(); b++) {
+    const RunBatch& bt = batches[b];
+    if (next < pts.size() && pts[next] == b) {
+      next++;
+      std::vector<std::pair<int32_t, int32_t>> pool;
+      for (int i = 0; i < bt.n; i++)
+        if (eligible(qs[bt.off + i])) pool.push_back({qs[bt.off + i].step, qs[bt.off + i].real});
+      std::shuffle(pool.begin(), pool.end(), rng);
+      const int64_t take = std::min<int64_t>({(int64_t)pool.size(), per, left, (int64_t)ws.nb});
+      pool.resize((size_t)take);
+      left -= take;
+      if (!pool.empty()) harvest_queries<T>(pl, pool, topn, nrand, rng, ws, set);
+    }
+    run_batch<T>(pl, qs.data() + bt.off, pl->dq + bt.off, bt.n, bt.plain, bt.multi, SEL_SCAN, nullptr, nullptr, 0);
+  }
+  HC(hipStreamSynchronize(s));
+  int32_t code = 0;
+  HC(hipMemcpy(&code, pl->hd.err, sizeof(int32_t), hipMemcpyDeviceToHost));
+  check_kernel_error(code);
   if (set.mask.empty()) bad(IQ_ERR_ARG, "convolutional encoder: no training query has enough enabled candidates");
+}
+
+// Banks of every mask from the network's output map B.f and the current heads.
+void conv_banks(iq_plan* pl, ConvBuf& B) {
+  hipStream_t s = pl->ctx->stream;
+  IqDev& H = pl->hd;
+  for (int m = 0; m < (int)pl->masks.size(); m++) {
+    hipLaunchKernelGGL(k_learn_bank, dim3(nblk(pl->geo.P, 256)), dim3(256), 0, s, (const IqDev*)pl->dd, m,
+                       (const float*)H.lrn.W + pl->lrn_wofs[m], (const float*)B.f, (uint16_t*)H.lrn.bank + pl->lrn_bofs[m]);
+    HCL();
+  }
+}
+
+// Recall of the shortlist of K (the deployed bank scan on the current banks)
+// on the exact bands of held-out queries: each query's harvested values (what
+// its tile step saw) are shortlisted exactly as iq_plan_run does; its band is
+// the members of its harvested nearest list within (1 + tol) of the nearest
+// (iqsim.jl:237).  Returns (band members in the shortlist) / (band members).
+template <typename T>
+double conv_val_recall(iq_plan* pl, const ConvSet& set, const std::vector<int>& val, int topn) {
+  hipStream_t s = pl->ctx->stream;
+  const IqDev* D = pl->dd;
+  int64_t hits = 0, total = 0;
+  const int nb = (int)std::min<int64_t>(pl->nslots, 256);
+  DMem t;
+  IqQuery* dq = t.alloc<IqQuery>(nb);
+  int4* dt = t.alloc<int4>(nb);
+  int32_t* dband = t.alloc<int32_t>((size_t)nb * topn);
+  int32_t* dnb = t.alloc<int32_t>(nb);
+  unsigned long long* dhits = t.alloc<unsigned long long>(1);
+  for (size_t b0 = 0; b0 < val.size(); b0 += (size_t)nb) {
+    const int nq = (int)std::min<size_t>(nb, val.size() - b0);
+    std::vector<IqQuery> hq((size_t)nq);
+    std::vector<int4> tiles;
+    std::vector<int32_t> hband((size_t)nq * topn, 0), hnb((size_t)nq, 0);
+    for (int i = 0; i < nq; i++) {
+      const int qi = val[b0 + i];
+      IqQuery q{};
+      q.step = set.step[qi];
+      q.real = set.real[qi];
+      q.slot = i;
+      q.u = 0.5;
+      q.forced = -1;
+      hq[i] = q;
+      const int mk = set.mask[qi];
+      if (!tiles.empty() && tiles.back().x == mk && tiles.back().z < 32 && tiles.back().y + tiles.back().z == i)
+        tiles.back().z++;
+      else
+        tiles.push_back(make_int4(mk, i, 1, 0));
+      const double* Di = set.D.data() + (size_t)qi * set.ns;
+      const int32_t* ci = set.cands.data() + (size_t)qi * set.ns;
+      const double thr = (1.0 + pl->tol) * Di[0];  // the topn list is ascending in D
+      int k = 0;
+      while (k < topn && Di[k] <= thr) {
+        hband[(size_t)i * topn + k] = ci[k];
+        k++;
+      }
+      hnb[i] = k;
+      total += k;
+    }
+    const int nt = (int)tiles.size();
+    HC(hipMemcpyAsync(dq, hq.data(), nq * sizeof(IqQuery), hipMemcpyHostToDevice, s));
+    HC(hipMemcpyAsync(dt, tiles.data(), nt * sizeof(int4), hipMemcpyHostToDevice, s));
+    HC(hipMemcpyAsync(dband, hband.data(), hband.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HC(hipMemcpyAsync(dnb, hnb.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, (int64_t*)dhits, (int64_t)1, (int64_t)0);
+    for (int i = 0; i < nq; i++) {  // the query values its tile step saw, and B2
+      const int qi = val[b0 + i];
+      const int V = pl->masks[set.mask[qi]].V;
+      HC(hipMemcpyAsync(pl->hd.kv + (int64_t)i * pl->hd.kv_stride, set.kv.data() + set.qoff[qi], V * sizeof(double),
+                        hipMemcpyHostToDevice, s));
+      HC(hipMemcpyAsync(pl->hd.b2 + (int64_t)i * (2 + pl->naux), set.b2.data() + qi, sizeof(double),
+                        hipMemcpyHostToDevice, s));
+    }
+    hipLaunchKernelGGL(k_learn_embed, dim3(nq), dim3(256), 0, s, D, (const IqQuery*)dq);
+    hipLaunchKernelGGL(k_learn_bfrag, dim3(nt), dim3(64), 0, s, D, (const IqQuery*)dq, (const int4*)dt);
+    hipLaunchKernelGGL(k_learn_bracket, dim3(nq), dim3(1024), 0, s, D, (const IqQuery*)dq);
+    hipLaunchKernelGGL(k_learn_mscan, dim3(nblk(pl->geo.P, 128)), dim3(256), 0, s, D, (const IqQuery*)dq,
+                       (const int4*)dt, nt);
+    hipLaunchKernelGGL(k_learn_pick, dim3(nq), dim3(1024), 0, s, D, (const IqQuery*)dq);
+    hipLaunchKernelGGL(k_cv_bandhits, dim3(nq), dim3(256), 0, s, D, (const IqQuery*)dq, (const int32_t*)dband,
+                       (const int32_t*)dnb, topn, dhits);
+    HCL();
+    unsigned long long h = 0;
+    HC(hipMemcpyAsync(&h, dhits, sizeof(h), hipMemcpyDeviceToHost, s));
+    HC(hipStreamSynchronize(s));
+    hits += (int64_t)h;
+  }
+  return total ? (double)hits / (double)total : 1.0;
 }
 
 // Harvest, train (Adam), then rebuild every mask's bank from the network's
@@ -1938,7 +2119,6 @@ void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
   IqDev& H = pl->hd;
   const IqGeo& g = pl->geo;
   if (!pl->lrn_ready) bad(IQ_ERR_ARG, "convolutional encoder: learn the linear encoders first (iq_plan_learn)");
-  if (!pl->has_run) bad(IQ_ERR_ARG, "convolutional encoder: training queries come from a finished iq_plan_run");
   if (o.nqueries < 1 || o.topn < 8 || o.nrandom < 0 || o.topn + o.nrandom > CVL_MAXS || o.steps < 0 || o.batch < 1 ||
       !(o.lr >= 0.0f) || !(o.lr_head >= 0.0f))
     bad(IQ_ERR_ARG, "convolutional encoder: invalid options");
@@ -1946,6 +2126,15 @@ void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
   std::mt19937_64 rng(o.seed);
   ConvBuf B;
   B.init(pl);
+  // shortlist size of the held-out recall check, and room for its lists
+  if (o.val_k > 0) H.lrn.K = (int32_t)std::min<int64_t>(o.val_k, g.P);
+  else if (H.lrn.K <= 0) H.lrn.K = (int32_t)std::min<int64_t>(2048, g.P);
+  const int64_t lcap = std::min<int64_t>(g.P, std::max<int64_t>(16 * (int64_t)H.lrn.K, 65536));
+  if (lcap > H.lrn.lcap) {
+    H.lrn.list = pl->mem.alloc<int2>((size_t)pl->nslots * lcap);
+    H.lrn.lcap = lcap;
+  }
+  HC(hipMemcpyAsync(pl->dd, &pl->hd, sizeof(IqDev), hipMemcpyHostToDevice, s));
   // network: He-normal hidden layers, zero output layer (f = x at the start)
   std::vector<float> th(CV_NW, 0.0f);
   std::normal_distribution<float> nrm(0.0f, 1.0f);
@@ -1996,7 +2185,7 @@ void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
     std::vector<double> hD((size_t)nq * ns), htau((size_t)nq);
     for (int i = 0; i < nq; i++) {
       const int qi = bt[i];
-      std::copy(set.Q.begin() + set.qoff[qi], set.Q.begin() + set.qoff[qi] + V, hQ.begin() + (size_t)i * V);
+      for (int v = 0; v < V; v++) hQ[(size_t)i * V + v] = (float)set.kv[set.qoff[qi] + v];
       std::copy(set.cands.begin() + (size_t)qi * ns, set.cands.begin() + (size_t)(qi + 1) * ns, hc.begin() + (size_t)i * ns);
       std::copy(set.D.begin() + (size_t)qi * ns, set.D.begin() + (size_t)(qi + 1) * ns, hD.begin() + (size_t)i * ns);
       htau[i] = set.tau[qi];
@@ -2005,8 +2194,10 @@ void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
     return conv_loss_grad(pl, B, pl->cv_theta, Wm, m, nq, ns, hQ.data(), hc.data(), hD.data(), htau.data(), grad_head,
                           forward);
   };
-  // held-out loss (mean over held-out queries); the parameters with the
-  // lowest one are kept, the starting point included (network = identity)
+  // held-out queries (all masks, in mask order) for the recall check
+  std::vector<int> vq;
+  for (auto& kv : valmask) vq.insert(vq.end(), kv.second.begin(), kv.second.end());
+  // held-out loss (mean over held-out queries)
   auto val_loss = [&]() {
     double tot = 0.0;
     int64_t n = 0;
@@ -2022,9 +2213,22 @@ void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
     HC(hipMemcpyAsync(best_th, pl->cv_theta, CV_NW * sizeof(float), hipMemcpyDeviceToDevice, s));
     HC(hipMemcpyAsync(best_W, H.lrn.W, wtot * sizeof(float), hipMemcpyDeviceToDevice, s));
   };
+  // the kept parameters: highest held-out recall of the deployed shortlist
+  // (banks rebuilt from the network's output), lower held-out loss on ties;
+  // the starting point (network = identity) is a candidate
   const int eval_every = std::max(1, std::min(25, o.steps / 4 + 1));
-  double best = vbatches.empty() ? 0.0 : val_loss();
-  const double val0 = best;
+  auto evaluate = [&](double& rec, double& vl) {
+    vl = vbatches.empty() ? 0.0 : val_loss();  // leaves B.f = the network's output
+    rec = 0.0;
+    if (!vq.empty()) {
+      conv_banks(pl, B);
+      rec = conv_val_recall<T>(pl, set, vq, o.topn);
+    }
+  };
+  double best_rec = 0.0, best = 0.0;
+  if (vbatches.empty()) conv_forward(s, B, pl->cv_theta);
+  evaluate(best_rec, best);
+  const double val0 = best, rec0 = best_rec;
   int best_step = 0;
   keep_best();
   for (int step = 0; step < o.steps; step++) {
@@ -2038,8 +2242,10 @@ void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
     if (o.lr_head > 0.0f)
       adam(s, (int64_t)IQ_EMB * V, Wm, gW, hm + pl->lrn_wofs[m], hv + pl->lrn_wofs[m], o.lr_head, ++hstep[m]);
     if (!vbatches.empty() && ((step + 1) % eval_every == 0 || step + 1 == o.steps)) {
-      const double v = val_loss();
-      if (v < best) {
+      double r, v;
+      evaluate(r, v);
+      if (r > best_rec || (r == best_rec && v < best)) {
+        best_rec = r;
         best = v;
         best_step = step + 1;
         keep_best();
@@ -2074,6 +2280,8 @@ void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
   pl->cv_stats[7] = (double)best_step;
   pl->cv_val[0] = val0;
   pl->cv_val[1] = best;
+  pl->cv_val[2] = rec0;
+  pl->cv_val[3] = best_rec;
 }
 }  // namespace
 
@@ -2580,6 +2788,8 @@ int iq_plan_conv_stats(iq_plan* plan, double* out) {
     for (int k = 0; k < 8; k++) out[k] = plan->cv_stats[k];
     out[8] = plan->cv_val[0];
     out[9] = plan->cv_val[1];
+    out[10] = plan->cv_val[2];
+    out[11] = plan->cv_val[3];
   });
 }
 

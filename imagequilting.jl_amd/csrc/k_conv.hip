@@ -589,16 +589,27 @@ k_cv_exact(const IqDev* __restrict__ D, int mask, const int32_t* __restrict__ sl
   }
 }
 
-// Query values of slot slots[i] as FP32 rows.  Grid: nq groups.
-__global__ void __launch_bounds__(256)
-k_cv_queries(const IqDev* __restrict__ D, int V, const int32_t* __restrict__ slots, float* __restrict__ Q) {
-  const double* kv = D->kv + (int64_t)slots[blockIdx.x] * D->kv_stride;
-  for (int v = threadIdx.x; v < V; v += 256) Q[(int64_t)blockIdx.x * V + v] = (float)kv[v];
-}
-
 // A2 of listed candidates (FP64 maps of the mask).
 __global__ void __launch_bounds__(256)
 k_cv_a2(const IqDev* __restrict__ D, int mask, const int32_t* __restrict__ cands, int64_t n, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) out[i] = D->a2maps[D->masks[mask].a2][cands[i]];
+}
+
+// Band members of each held-out query that its shortlist holds:
+// hits += #{b in band[i][0..nband[i]) : the slot's shortlist bitmap has b}.
+// Grid: nq groups.
+__global__ void __launch_bounds__(256)
+k_cv_bandhits(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const int32_t* __restrict__ band,
+              const int32_t* __restrict__ nband, int stride, unsigned long long* __restrict__ hits) {
+  const int i = blockIdx.x;
+  const int64_t slot = qs[i].slot;
+  const unsigned long long* sb = D->lrn.sbits + slot * D->nw64;
+  int h = 0;
+  for (int k = threadIdx.x; k < nband[i]; k += 256) {
+    const int32_t c = band[(int64_t)i * stride + k];
+    h += (int)((sb[c >> 6] >> (c & 63)) & 1ull);
+  }
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
+  if ((threadIdx.x & 63) == 0 && h) atomicAdd(hits, (unsigned long long)h);
 }

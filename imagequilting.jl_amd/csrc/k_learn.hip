@@ -450,26 +450,26 @@ k_short_bound(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     const int cx = (int)(c % g.dist[0]), cy = (int)((c / g.dist[0]) % g.dist[1]);
     const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
     const int64_t base = iq_lin3(g.ti, cx, cy, cz);
-    // each lane sums its strided voxels in ascending order, eight window
+    // each lane sums its strided voxels in ascending order, sixteen window
     // values in flight at a time
     float acc = 0.0f;
     int v = lane;
     if (f32) {
-      for (; v + 64 * 7 < V; v += 64 * 8) {
-        float t[8];
+      for (; v + 64 * 15 < V; v += 64 * 16) {
+        float t[16];
 #pragma unroll
-        for (int u = 0; u < 8; u++) t[u] = ti32[base + tl[v + 64 * u]];
+        for (int u = 0; u < 16; u++) t[u] = ti32[base + tl[v + 64 * u]];
 #pragma unroll
-        for (int u = 0; u < 8; u++) acc = __builtin_fmaf(t[u], qv[v + 64 * u], acc);
+        for (int u = 0; u < 16; u++) acc = __builtin_fmaf(t[u], qv[v + 64 * u], acc);
       }
       for (; v < V; v += 64) acc = __builtin_fmaf(ti32[base + tl[v]], qv[v], acc);
     } else {
-      for (; v + 64 * 7 < V; v += 64 * 8) {
-        float t[8];
+      for (; v + 64 * 15 < V; v += 64 * 16) {
+        float t[16];
 #pragma unroll
-        for (int u = 0; u < 8; u++) t[u] = (float)ti64[base + tl[v + 64 * u]];
+        for (int u = 0; u < 16; u++) t[u] = (float)ti64[base + tl[v + 64 * u]];
 #pragma unroll
-        for (int u = 0; u < 8; u++) acc = __builtin_fmaf(t[u], qv[v + 64 * u], acc);
+        for (int u = 0; u < 16; u++) acc = __builtin_fmaf(t[u], qv[v + 64 * u], acc);
       }
       for (; v < V; v += 64) acc = __builtin_fmaf((float)ti64[base + tl[v]], qv[v], acc);
     }

@@ -82,6 +82,7 @@ class ConvOpts(ctypes.Structure):
         ("lr", ctypes.c_float),
         ("lr_head", ctypes.c_float),
         ("seed", ctypes.c_uint64),
+        ("val_k", ctypes.c_int64),
     ]
 
 

@@ -366,21 +366,22 @@ class Plan:
 
     # -- convolutional encoder (no counterpart in the reference)
     def learn_conv(self, nqueries: int = 640, topn: int = 256, nrandom: int = 64, steps: int = 200, batch: int = 16,
-                   lr: float = 1e-3, lr_head: float = 0.0, seed: int = 0) -> dict:
+                   lr: float = 1e-3, lr_head: float = 0.0, seed: int = 0, val_k: int = 0) -> dict:
         """Train the convolutional encoder on (query, exact distance) pairs of
         the last run's grids and rebuild the banks from it (iq_plan_learn_conv);
         returns its statistics."""
         o = _lib.ConvOpts(int(nqueries), int(topn), int(nrandom), int(steps), int(batch), float(lr), float(lr_head),
-                          int(seed))
+                          int(seed), int(val_k))
         self.ctx.check(self.ctx._L.iq_plan_learn_conv(self.ctx.handle, self.handle, ctypes.byref(o)))
         return self.conv_stats()
 
     def conv_stats(self) -> dict:
-        v = np.zeros(10, dtype=np.float64)
+        v = np.zeros(12, dtype=np.float64)
         self.ctx.check(self.ctx._L.iq_plan_conv_stats(self.handle, v.ctypes.data))
         return dict(harvest_seconds=float(v[0]), train_seconds=float(v[1]), loss_first=float(v[2]),
                     loss_last=float(v[3]), pairs=int(v[4]), steps=int(v[5]), queries=int(v[6]), kept_step=int(v[7]),
-                    heldout_loss_start=float(v[8]), heldout_loss_kept=float(v[9]))
+                    heldout_loss_start=float(v[8]), heldout_loss_kept=float(v[9]),
+                    heldout_recall_start=float(v[10]), heldout_recall_kept=float(v[11]))
 
     def conv_params(self) -> np.ndarray:
         """The network's parameters (IQ_CONV_NPARAM floats)."""

@@ -284,20 +284,25 @@ int iq_plan_encoder(iq_plan* plan, int mask, float* W, int32_t* tvox, int64_t* V
  * f = x + conv3(...) (zero outside the image), feeds each mask shape's head
  * W (64 x V): e_c = W f[window c] for the bank, e_q = W q for a query.
  * iq_plan_learn_conv trains it on the GPU (forward and backward kernels,
- * Adam) from pairs of the plain method: `nqueries` overlap regions at the
- * tile steps of the grids of the plan's last iq_plan_run, each with its
- * `topn` nearest candidates by exact FP64 distance (preselected by the full
- * scan) and `nrandom` random candidates; the loss of a query is
+ * Adam) from pairs of the plain method: it runs the full scan over every
+ * realization (its own uniforms; the grids then hold that run) and takes
+ * about `nqueries` plain tile steps, spread over the run, as they run: the
+ * overlap values the step sees, its `topn` nearest candidates by exact FP64
+ * distance (preselected by the full scan's lower bounds) and `nrandom`
+ * random candidates; the loss of a query is
  * KL(p* || p~) with p* ~ exp(-(D - min D)/tau) over the exact distances,
  * p~ ~ exp(-(dd - min dd)/tau) over the bank scan's approximate distances
  * dd = A2 - 2 e_c.e_q, tau the larger of the band width tol * min D and the
  * spread of its eight nearest exact distances.  The heads start from
  * iq_plan_learn's linear encoders (call it first) and are trained with the
- * network when lr_head > 0.  Every fifth query is held out: the parameters
- * with the lowest held-out loss seen (checked at the start, where the
- * network is the identity, and every few steps) are kept, so training never
- * leaves a network that scores worse on held-out pairs than the linear
- * encoders alone.  Afterwards every mask's bank holds the encodings of f.  Parameter vector (float): w1[32][27], b1[32],
+ * network when lr_head > 0.  Every fifth query is held out: at the start
+ * (where the network is the identity, i.e. the linear encoders alone) and
+ * every few steps, the banks are rebuilt and the held-out tile steps are
+ * shortlisted exactly as iq_plan_run does; the parameters with the highest
+ * recall of the held-out exact bands (lower held-out loss on ties) are kept,
+ * so training never leaves an encoder that recalls fewer held-out band
+ * members than the linear encoders.  Afterwards every mask's bank holds the
+ * encodings of f.  Parameter vector (float): w1[32][27], b1[32],
  * w2[32][32][27], b2[32], w3[32][27], b3, i.e. IQ_CONV_NPARAM values, tap
  * k = (dx+1) + 3(dy+1) + 9(dz+1); a mask's head W[64][V] follows when a
  * function takes one.
@@ -305,7 +310,8 @@ int iq_plan_encoder(iq_plan* plan, int mask, float* W, int32_t* tvox, int64_t* V
  * training and rebuilding the banks, out[2] / out[3] mean loss of the first /
  * last ten steps, out[4] training pairs, out[5] steps, out[6] queries, out[7]
  * the step whose parameters were kept (0: the starting point), out[8] /
- * out[9] held-out loss at the start / of the kept parameters.
+ * out[9] held-out loss at the start / of the kept parameters, out[10] /
+ * out[11] held-out shortlist recall at the start / of the kept parameters.
  * iq_plan_conv_grad: loss and gradient (network, then head) for given
  * parameters on given pairs of mask shape `mask` (Q: nq x V values; cands,
  * D: nq x ns; tau: nq), without changing the plan.  iq_plan_bank_rows: bank
@@ -318,11 +324,12 @@ typedef struct iq_conv_opts {
   int steps;         /* Adam steps (e.g. 200) */
   int batch;         /* queries per step (e.g. 16) */
   float lr;          /* learning rate of the network (e.g. 1e-3) */
-  float lr_head;     /* learning rate of the heads (e.g. 1e-4) */
+  float lr_head;     /* learning rate of the heads (0: the linear encoders' heads stay) */
   uint64_t seed;
+  int64_t val_k;     /* shortlist size of the held-out recall check (0: the plan's K, else 2048) */
 } iq_conv_opts;
 int iq_plan_learn_conv(iq_ctx* ctx, iq_plan* plan, const iq_conv_opts* opts);
-int iq_plan_conv_stats(iq_plan* plan, double out[10]);
+int iq_plan_conv_stats(iq_plan* plan, double out[12]);
 int iq_plan_conv_params(iq_plan* plan, float* theta);
 int iq_plan_conv_grad(iq_ctx* ctx, iq_plan* plan, int mask, const float* theta, int64_t nq, int64_t ns, const float* Q,
                       const int32_t* cands, const double* D, const double* tau, double* loss, float* grad);
