@@ -193,11 +193,12 @@ def main():
     ap.add_argument("--workload", type=int, default=5, choices=sorted(WORKLOADS),
                     help="5 (default): learned + batched vs the full scan; 1-4: the full scan alone")
     ap.add_argument("--K", type=int, default=2048, help="shortlist size of the learned proposal")
-    ap.add_argument("--check-every", type=int, default=16, help="full-scan recall check every N-th batch")
+    ap.add_argument("--check-every", type=int, default=32, help="full-scan recall check every N-th batch")
     ap.add_argument("--full-steps", type=int, default=-1, help="timed steps of the full-scan leg (default: --steps)")
     ap.add_argument("--encoder", choices=("conv", "linear"), default="conv",
                     help="learned leg's encoder: the trained convolutional network (default) or the linear one alone")
     ap.add_argument("--conv-steps", type=int, default=200, help="Adam steps of the convolutional encoder")
+    ap.add_argument("--conv-lr-head", type=float, default=0.0, help="learning rate of the heads (0: linear heads stay)")
     args = ap.parse_args()
     if args.copy_mb > 0:
         copy_calibration(args.copy_mb)
@@ -282,8 +283,8 @@ def main():
             # training pairs: the encoder runs the full scan itself and takes
             # tile steps' queries as they run
             t0 = time.perf_counter()
-            conv = plan.learn_conv(nqueries=640, topn=256, nrandom=64, steps=args.conv_steps, batch=16, lr=1e-3,
-                                   lr_head=0.0, seed=1, val_k=args.K)
+            conv = plan.learn_conv(nqueries=640, topn=256, nrandom=64, nhard=64, steps=args.conv_steps, batch=16,
+                                   lr=1e-3, lr_head=args.conv_lr_head, seed=1, val_k=args.K)
             conv["seconds"] = time.perf_counter() - t0
             plan.set_proposal(True, K=args.K, check_every=1, min_recall=0.0)
             plan.run(U[0])

@@ -50,6 +50,7 @@ struct IqError {
 
 struct iq_ctx {
   int device = 0;
+  int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   std::string err;
 };
@@ -360,7 +361,6 @@ struct iq_plan {
   int multi_bx = 16;       // candidate block width of the tiled maps (32 or 16; 16 rows)
   bool cut_big = false;    // largest slab needs the 512/1024-lane register-resident cut
   int cut_team = 1024;     // its work-items per slab (IQHIP_CUT_TEAM=512: 512)
-  bool recheck_lanes = true;  // exact recheck with a lane per member (IQHIP_RECHECK=groups: older kernel)
   bool cut_wave = false;   // every slab fits the one-wavefront cut
   std::vector<int32_t> enabled_host;
   // learned proposal
@@ -790,7 +790,10 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
       hipLaunchKernelGGL((k_exact_map<double, 2>), dim3(nblk(g.P, 256)), dim3(256), 0, s, (const double*)H.ti, g,
                          H.toff + m.off, (const double*)nullptr, m.V, (const double*)nullptr, a2);
     HCL();
-    float* a2f = pl->mem.alloc<float>(g.P);
+    // padded to whole 32-candidate chunks of the bank scan, the pad NaN
+    const int64_t P32 = (g.P + 31) / 32 * 32;
+    float* a2f = pl->mem.alloc<float>(P32);
+    if (P32 > g.P) HC(hipMemsetAsync(a2f + g.P, 0xff, (P32 - g.P) * sizeof(float), s));
     hipLaunchKernelGGL(k_a2_f32, dim3(std::min<int64_t>(nblk(g.P, 256), 4096)), dim3(256), 0, s, (const double*)a2,
                        (const uint8_t*)H.disabled, g.P, a2f);
     HCL();
@@ -1064,7 +1067,14 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   if (const char* erc = std::getenv("IQHIP_RC_SMALL")) H.rc_small = std::max(0, std::atoi(erc));
   if (const char* eg = std::getenv("IQHIP_GRAPH")) pl->graphs = std::atoi(eg) != 0;
   if (const char* et = std::getenv("IQHIP_CUT_TEAM")) pl->cut_team = std::atoi(et) == 512 ? 512 : 1024;
-  if (const char* er = std::getenv("IQHIP_RECHECK")) pl->recheck_lanes = std::string(er) != "groups";
+  // the exact recheck runs a lane per superset member (k_band_recheck_lanes;
+  // measured faster than a wavefront per member at 720- and 4,736-voxel
+  // masks); IQHIP_RECHECK=groups selects the wavefront-per-member kernel
+  H.rc_lanes = 0;
+  if (const char* er = std::getenv("IQHIP_RECHECK")) {
+    if (std::string(er) == "groups") H.rc_lanes = 1 << 30;
+    if (std::string(er) == "lanes") H.rc_lanes = 0;
+  }
   H.rounds = pl->mem.alloc<int64_t>(6);  // push rounds, BFS levels, cuts; plain queries, superset, band members
 
   // ---- results
@@ -1088,10 +1098,24 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   HC(hipStreamSynchronize(s));
 }
 
-// exact FP64 recheck of the band supersets of a batch: a lane per member
-// (default), or the wavefront / work-item kernel (IQHIP_RECHECK=groups)
+// bank scan: persistent blocks, as many as are resident at once on every
+// CU (each owns a staging region), none beyond the chunks
+void launch_mscan(iq_plan* pl, const IqQuery* q, const int4* tiles, int ntiles, hipStream_t s) {
+  static thread_local int per_cu = 0;
+  if (per_cu <= 0) {
+    HC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_learn_mscan, 64 * LMS_WAVES, 0));
+    per_cu = std::min(std::max(per_cu, 1), LMS_BLOCKS_PER_CU);
+  }
+  const int64_t nchunk = (pl->geo.P + 31) / 32;
+  const int64_t nb = std::min<int64_t>(nblk(nchunk, LMS_WAVES), (int64_t)per_cu * pl->ctx->ncu);
+  hipLaunchKernelGGL(k_learn_mscan, dim3((unsigned)nb), dim3(64 * LMS_WAVES), 0, s, (const IqDev*)pl->dd, q, tiles, ntiles);
+  HCL();
+}
+
+// exact FP64 recheck of the band supersets of a batch: with a lane per
+// member (plans with long chains), or a wavefront per member
 void launch_recheck(iq_plan* pl, const IqQuery* q, int nq, hipStream_t s) {
-  if (pl->recheck_lanes)
+  if (pl->hd.rc_lanes == 0)
     hipLaunchKernelGGL(k_band_recheck_lanes, dim3(nq, RCL_GROUPS), dim3(64), 0, s, (const IqDev*)pl->dd, q);
   else
     hipLaunchKernelGGL(k_band_recheck, dim3(nq, RC_GROUPS), dim3(256), 0, s, (const IqDev*)pl->dd, q);
@@ -1141,7 +1165,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
   if (has_plain && sel != SEL_SCAN && ntiles > 0) {
     // shortlist: query encodings -> bracket from a sample -> MFMA scan of the
     // bank keeping what falls in the bracket -> the K best
-    hipLaunchKernelGGL(k_learn_embed, dim3(nq), dim3(256), 0, s, D, q);
+    hipLaunchKernelGGL(k_learn_embed, dim3(nq), dim3(1024), 0, s, D, q);
     HCL();
     hipLaunchKernelGGL(k_learn_bfrag, dim3(ntiles), dim3(64), 0, s, D, q, tiles);
     HCL();
@@ -1162,7 +1186,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
       for (int t = 0; t < ntiles; t++) nmk += (t == 0 || htiles[t].x != htiles[t - 1].x) ? 1 : 0;
       pl->bank_bytes_run += (double)nmk * (double)P * (IQ_EMB * 2 + 4);
     }
-    hipLaunchKernelGGL(k_learn_mscan, dim3(nblk(P, 128)), dim3(256), 0, s, D, q, tiles, ntiles);
+    launch_mscan(pl, q, tiles, ntiles, s);
     HCL();
     if (pl->prof) {
       HC(hipEventRecord(pl->bank_ev[pl->bank_used + 1], s));
@@ -1173,7 +1197,7 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
   }
   if (has_plain && sel == SEL_LEARNED) {
     // shortlist -> FP32 bounds of its members -> band superset -> exact recheck
-    hipLaunchKernelGGL(k_short_bound, dim3(64, nq), dim3(256), (size_t)pl->lrn_vmax * 8, s, D, q);
+    hipLaunchKernelGGL(k_short_bound, dim3(32, nq), dim3(512), (size_t)pl->lrn_vmax * 8, s, D, q);
     HCL();
     hipLaunchKernelGGL(k_short_select, dim3(16, nq), dim3(256), 0, s, D, q);
     HCL();
@@ -1529,6 +1553,17 @@ void sgemm(hipStream_t s, int M, int N, int K, const float* A, int64_t sam, int6
   HCL();
 }
 
+// Transposed heads of every mask (read by the query encodings).
+void refresh_wt(iq_plan* pl) {
+  hipStream_t s = pl->ctx->stream;
+  for (int m = 0; m < (int)pl->masks.size(); m++) {
+    const int V = pl->masks[m].V;
+    hipLaunchKernelGGL(k_learn_transpose, dim3(V), dim3(IQ_EMB), 0, s, pl->hd.lrn.W + pl->lrn_wofs[m], V,
+                       (float*)pl->hd.lrn.WT + pl->lrn_wofs[m]);
+    HCL();
+  }
+}
+
 // Rows of Y (64 x V) made orthonormal into W: W = L^-1 Y with Y Y^T = L L^T
 // (Cholesky in FP64 on the host), applied twice.
 void orthonormalize(hipStream_t s, DMem& tmp, const float* Y, float* W, int V) {
@@ -1587,21 +1622,27 @@ void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
   if (pl->enabled_host.empty()) bad(IQ_ERR_NOCAND, "every candidate pattern is disabled");
   if (!pl->fast) bad(IQ_ERR_UNSUPPORTED, "learned proposal: training image values must be exact in FP32");
   if (!pl->lrn_ready) {
+    const int64_t P32 = (g.P + 31) / 32 * 32;
     std::vector<int64_t> wofs(nm), bofs(nm);
     size_t wtot = 0;
     for (int m = 0; m < nm; m++) {
       wofs[m] = (int64_t)wtot;
       wtot += (size_t)IQ_EMB * pl->masks[m].V;
-      bofs[m] = (int64_t)m * IQ_EMB * g.P;
+      bofs[m] = (int64_t)m * IQ_EMB * P32;
     }
     H.lrn.W = pl->mem.alloc<float>(wtot);
-    H.lrn.bank = pl->mem.alloc<uint16_t>((size_t)nm * IQ_EMB * g.P);
+    H.lrn.WT = pl->mem.alloc<float>(wtot);
+    // each mask's rows padded to whole 32-candidate chunks (the pad rows zero)
+    H.lrn.bank = pl->mem.alloc<uint16_t>((size_t)nm * IQ_EMB * P32);
+    HC(hipMemsetAsync((void*)H.lrn.bank, 0, (size_t)nm * IQ_EMB * P32 * sizeof(uint16_t), s));
     H.lrn.wofs = pl->mem.upload(wofs.data(), wofs.size(), s);
     H.lrn.bofs = pl->mem.upload(bofs.data(), bofs.size(), s);
     H.lrn.eq = pl->mem.alloc<float>((size_t)pl->nslots * IQ_EMB);
     H.lrn.bq = pl->mem.alloc<uint4>((size_t)pl->nslots * (IQ_EMB / 16) * 64);
     H.lrn.brk = pl->mem.alloc<uint32_t>((size_t)pl->nslots * 4);
     H.lrn.binfo = pl->mem.alloc<uint4>((size_t)pl->nslots);
+    H.lrn.stage_cap = LMS_STAGE;
+    H.lrn.stage = pl->mem.alloc<int4>((size_t)LMS_BLOCKS_PER_CU * pl->ctx->ncu * LMS_STAGE);
     H.lrn.lcnt = pl->mem.alloc<int32_t>(pl->nslots);
     H.lrn.sbits = pl->mem.alloc<unsigned long long>((size_t)pl->nslots * H.nw64);
     H.lrn.stats = pl->mem.alloc<unsigned long long>(4);
@@ -1646,6 +1687,7 @@ void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
     HCL();
     HC(hipStreamSynchronize(s));
   }
+  refresh_wt(pl);
   pl->lrn_ready = true;
   pl->cv_ready = false;  // banks now hold the linear encodings of the training image
   HC(hipMemcpyAsync(pl->dd, &pl->hd, sizeof(IqDev), hipMemcpyHostToDevice, s));
@@ -1803,26 +1845,55 @@ struct ConvSet {
   std::vector<double> D, tau;
 };
 
+// The learned shortlist of the queries hq (device copy dq; slots 0..nq-1,
+// query values and B2 already in their slots, queries of one mask adjacent),
+// launched as iq_plan_run launches it: encodings, bracket, bank scan, pick.
+// dt: room for nq tiles.
+void launch_shortlist(iq_plan* pl, const std::vector<IqQuery>& hq, const IqQuery* dq, int4* dt) {
+  hipStream_t s = pl->ctx->stream;
+  const IqDev* D = pl->dd;
+  const int nq = (int)hq.size();
+  std::vector<int4> tiles;
+  for (int i = 0; i < nq; i++) {
+    const int mk = pl->steps[hq[i].step].mask;
+    if (!tiles.empty() && tiles.back().x == mk && tiles.back().z < 32 && tiles.back().y + tiles.back().z == i)
+      tiles.back().z++;
+    else
+      tiles.push_back(make_int4(mk, i, 1, 0));
+  }
+  const int nt = (int)tiles.size();
+  HC(hipMemcpyAsync(dt, tiles.data(), nt * sizeof(int4), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_learn_embed, dim3(nq), dim3(1024), 0, s, D, dq);
+  hipLaunchKernelGGL(k_learn_bfrag, dim3(nt), dim3(64), 0, s, D, dq, (const int4*)dt);
+  hipLaunchKernelGGL(k_learn_bracket, dim3(nq), dim3(1024), 0, s, D, dq);
+  launch_mscan(pl, dq, (const int4*)dt, nt, s);
+  hipLaunchKernelGGL(k_learn_pick, dim3(nq), dim3(1024), 0, s, D, dq);
+  HCL();
+}
+
 // Device scratch of the training-pair harvest (one batch of up to nb queries).
 struct HarvestWs {
   DMem t;
-  int nb = 0, cap = 0, nrand = 0;
+  int nb = 0, cap = 0, nrand = 0, nhard = 0;
   IqQuery* dq = nullptr;
+  int4* dt = nullptr;
   int32_t *hist = nullptr, *list = nullptr, *cnt = nullptr, *dslot = nullptr, *dcand = nullptr;
   uint32_t* sel = nullptr;
   double *dex = nullptr, *dkv = nullptr;
-  void init(iq_plan* pl, int topn, int nrand_) {
+  void init(iq_plan* pl, int topn, int nrand_, int nhard_) {
     nb = (int)std::min<int64_t>(pl->nslots, 256);
     cap = 4 * topn;
     nrand = nrand_;
+    nhard = nhard_;
     dq = t.alloc<IqQuery>(nb);
+    dt = t.alloc<int4>(nb);
     hist = t.alloc<int32_t>((size_t)nb * CVT_BINS);
     sel = t.alloc<uint32_t>((size_t)nb * 4);
     list = t.alloc<int32_t>((size_t)nb * cap);
     cnt = t.alloc<int32_t>(nb);
     dslot = t.alloc<int32_t>(nb);
-    dcand = t.alloc<int32_t>((size_t)nb * (cap + nrand));
-    dex = t.alloc<double>((size_t)nb * (cap + nrand));
+    dcand = t.alloc<int32_t>((size_t)nb * (cap + nhard + nrand));
+    dex = t.alloc<double>((size_t)nb * (cap + nhard + nrand));
   }
 };
 
@@ -1830,10 +1901,13 @@ struct HarvestWs {
 // the grids' current state (so a query is what its tile step sees when it
 // runs): the query values (FP64) and B2, the `topn` nearest candidates by
 // exact FP64 distance among those the full scan's lower bounds put first,
-// and `nrand` random enabled candidates with their exact distances.
+// `nhard` members of the current encoder's shortlist that are not among the
+// listed nearest (the competitors a shortlist must rank below the band) and
+// `nrand` random enabled candidates, all with their exact distances.
 template <typename T>
 void harvest_queries(iq_plan* pl, std::vector<std::pair<int32_t, int32_t>> pool, int topn, int nrand,
                      std::mt19937_64& rng, HarvestWs& ws, ConvSet& set) {
+  const int nhard = ws.nhard;
   hipStream_t s = pl->ctx->stream;
   const IqGeo& g = pl->geo;
   const IqDev* D = pl->dd;
@@ -1841,7 +1915,7 @@ void harvest_queries(iq_plan* pl, std::vector<std::pair<int32_t, int32_t>> pool,
     return pl->steps[a.first].mask < pl->steps[b.first].mask;
   });
   const int cap = ws.cap, nb = ws.nb;
-  set.ns = topn + nrand;
+  set.ns = topn + nhard + nrand;
   std::uniform_int_distribution<size_t> pick(0, pl->enabled_host.size() - 1);
   IqQuery* dq = ws.dq;
   int32_t *hist = ws.hist, *list = ws.list, *cnt = ws.cnt, *dslot = ws.dslot, *dcand = ws.dcand;
@@ -1902,9 +1976,23 @@ void harvest_queries(iq_plan* pl, std::vector<std::pair<int32_t, int32_t>> pool,
     std::vector<int32_t> hcnt((size_t)nq), hlist((size_t)nq * cap);
     HC(hipMemcpyAsync(hcnt.data(), cnt, nq * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HC(hipMemcpyAsync(hlist.data(), list, (size_t)nq * cap * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    // the current encoder's shortlists of the same queries
+    std::vector<int32_t> hns((size_t)nq, 0);
+    std::vector<std::vector<int32_t>> hsl((size_t)nq);
+    if (nhard > 0) {
+      launch_shortlist(pl, hq, dq, ws.dt);
+      HC(hipMemcpyAsync(hns.data(), pl->hd.lrn.nslist, nq * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      HC(hipStreamSynchronize(s));
+      for (int i = 0; i < nq; i++) {
+        hsl[i].resize((size_t)std::max(hns[i], 0));
+        if (hns[i] > 0)
+          HC(hipMemcpyAsync(hsl[i].data(), pl->hd.lrn.slist + (int64_t)i * g.P, hns[i] * sizeof(int32_t),
+                            hipMemcpyDeviceToHost, s));
+      }
+    }
     HC(hipStreamSynchronize(s));
-    // listed and random candidates -> exact distances
-    const int w = cap + nrand;
+    // listed, hard and random candidates -> exact distances
+    const int w = cap + nhard + nrand;
     std::vector<int32_t> hc((size_t)nq * w);
     std::vector<int> nl((size_t)nq);
     for (int i = 0; i < nq; i++) {
@@ -1912,12 +2000,24 @@ void harvest_queries(iq_plan* pl, std::vector<std::pair<int32_t, int32_t>> pool,
       std::vector<int32_t> li(hlist.begin() + (size_t)i * cap, hlist.begin() + (size_t)i * cap + nl[i]);
       std::sort(li.begin(), li.end());
       for (int j = 0; j < w; j++) hc[(size_t)i * w + j] = j < nl[i] ? li[j] : pl->enabled_host[0];  // padding
+      // hard candidates: shortlist members outside the list (fewer than
+      // nhard: the rest random)
+      std::vector<int32_t> hard;
+      for (int32_t c : hsl[i])
+        if (!std::binary_search(li.begin(), li.end(), c)) hard.push_back(c);
+      std::shuffle(hard.begin(), hard.end(), rng);
       // random candidates outside the list (repeats allowed when the list
       // holds most of the enabled candidates)
-      const bool distinct = pl->enabled_host.size() >= 2 * (size_t)(nl[i] + nrand);
-      for (int j = nl[i]; j < nl[i] + nrand;) {
-        const int32_t c = pl->enabled_host[pick(rng)];
-        if (distinct && std::binary_search(li.begin(), li.end(), c)) continue;
+      const bool distinct = pl->enabled_host.size() >= 2 * (size_t)(nl[i] + nhard + nrand);
+      for (int j = nl[i]; j < nl[i] + nhard + nrand;) {
+        const int k = j - nl[i];
+        int32_t c;
+        if (k < nhard && k < (int)hard.size()) {
+          c = hard[k];
+        } else {
+          c = pl->enabled_host[pick(rng)];
+          if (distinct && std::binary_search(li.begin(), li.end(), c)) continue;
+        }
         hc[(size_t)i * w + j++] = c;
       }
     }
@@ -1950,7 +2050,7 @@ void harvest_queries(iq_plan* pl, std::vector<std::pair<int32_t, int32_t>> pool,
         set.cands.push_back(ci[ord[j]]);
         set.D.push_back(di[ord[j]]);
       }
-      for (int j = 0; j < nrand; j++) {
+      for (int j = 0; j < nhard + nrand; j++) {
         set.cands.push_back(ci[nl[i] + j]);
         set.D.push_back(di[nl[i] + j]);
       }
@@ -1968,7 +2068,7 @@ void harvest_queries(iq_plan* pl, std::vector<std::pair<int32_t, int32_t>> pool,
 // about nq_total plain tile steps, spread over up to 48 batches, are
 // harvested right before their batch runs.
 template <typename T>
-void conv_harvest(iq_plan* pl, int64_t nq_total, int topn, int nrand, std::mt19937_64& rng, ConvSet& set) {
+void conv_harvest(iq_plan* pl, int64_t nq_total, int topn, int nhard, int nrand, std::mt19937_64& rng, ConvSet& set) {
   hipStream_t s = pl->ctx->stream;
   std::vector<double> U((size_t)pl->nlog);
   std::uniform_real_distribution<double> uni(0.0, 1.0);
@@ -1996,7 +2096,7 @@ void conv_harvest(iq_plan* pl, int64_t nq_total, int topn, int nrand, std::mt199
   }
   HC(hipMemcpyAsync(pl->dq, qs.data(), qs.size() * sizeof(IqQuery), hipMemcpyHostToDevice, s));
   HarvestWs ws;
-  ws.init(pl, topn, nrand);
+  ws.init(pl, topn, nrand, nhard);
   plan_resets<T>(pl);
   size_t next = 0;
   int64_t left = nq_total;
@@ -2053,7 +2153,6 @@ double conv_val_recall(iq_plan* pl, const ConvSet& set, const std::vector<int>& 
   for (size_t b0 = 0; b0 < val.size(); b0 += (size_t)nb) {
     const int nq = (int)std::min<size_t>(nb, val.size() - b0);
     std::vector<IqQuery> hq((size_t)nq);
-    std::vector<int4> tiles;
     std::vector<int32_t> hband((size_t)nq * topn, 0), hnb((size_t)nq, 0);
     for (int i = 0; i < nq; i++) {
       const int qi = val[b0 + i];
@@ -2064,11 +2163,6 @@ double conv_val_recall(iq_plan* pl, const ConvSet& set, const std::vector<int>& 
       q.u = 0.5;
       q.forced = -1;
       hq[i] = q;
-      const int mk = set.mask[qi];
-      if (!tiles.empty() && tiles.back().x == mk && tiles.back().z < 32 && tiles.back().y + tiles.back().z == i)
-        tiles.back().z++;
-      else
-        tiles.push_back(make_int4(mk, i, 1, 0));
       const double* Di = set.D.data() + (size_t)qi * set.ns;
       const int32_t* ci = set.cands.data() + (size_t)qi * set.ns;
       const double thr = (1.0 + pl->tol) * Di[0];  // the topn list is ascending in D
@@ -2080,9 +2174,7 @@ double conv_val_recall(iq_plan* pl, const ConvSet& set, const std::vector<int>& 
       hnb[i] = k;
       total += k;
     }
-    const int nt = (int)tiles.size();
     HC(hipMemcpyAsync(dq, hq.data(), nq * sizeof(IqQuery), hipMemcpyHostToDevice, s));
-    HC(hipMemcpyAsync(dt, tiles.data(), nt * sizeof(int4), hipMemcpyHostToDevice, s));
     HC(hipMemcpyAsync(dband, hband.data(), hband.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     HC(hipMemcpyAsync(dnb, hnb.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, s, (int64_t*)dhits, (int64_t)1, (int64_t)0);
@@ -2094,12 +2186,7 @@ double conv_val_recall(iq_plan* pl, const ConvSet& set, const std::vector<int>& 
       HC(hipMemcpyAsync(pl->hd.b2 + (int64_t)i * (2 + pl->naux), set.b2.data() + qi, sizeof(double),
                         hipMemcpyHostToDevice, s));
     }
-    hipLaunchKernelGGL(k_learn_embed, dim3(nq), dim3(256), 0, s, D, (const IqQuery*)dq);
-    hipLaunchKernelGGL(k_learn_bfrag, dim3(nt), dim3(64), 0, s, D, (const IqQuery*)dq, (const int4*)dt);
-    hipLaunchKernelGGL(k_learn_bracket, dim3(nq), dim3(1024), 0, s, D, (const IqQuery*)dq);
-    hipLaunchKernelGGL(k_learn_mscan, dim3(nblk(pl->geo.P, 128)), dim3(256), 0, s, D, (const IqQuery*)dq,
-                       (const int4*)dt, nt);
-    hipLaunchKernelGGL(k_learn_pick, dim3(nq), dim3(1024), 0, s, D, (const IqQuery*)dq);
+    launch_shortlist(pl, hq, dq, dt);
     hipLaunchKernelGGL(k_cv_bandhits, dim3(nq), dim3(256), 0, s, D, (const IqQuery*)dq, (const int32_t*)dband,
                        (const int32_t*)dnb, topn, dhits);
     HCL();
@@ -2119,7 +2206,8 @@ void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
   IqDev& H = pl->hd;
   const IqGeo& g = pl->geo;
   if (!pl->lrn_ready) bad(IQ_ERR_ARG, "convolutional encoder: learn the linear encoders first (iq_plan_learn)");
-  if (o.nqueries < 1 || o.topn < 8 || o.nrandom < 0 || o.topn + o.nrandom > CVL_MAXS || o.steps < 0 || o.batch < 1 ||
+  if (o.nqueries < 1 || o.topn < 8 || o.nrandom < 0 || o.nhard < 0 || o.topn + o.nhard + o.nrandom > CVL_MAXS ||
+      o.steps < 0 || o.batch < 1 ||
       !(o.lr >= 0.0f) || !(o.lr_head >= 0.0f))
     bad(IQ_ERR_ARG, "convolutional encoder: invalid options");
   const auto t0 = std::chrono::steady_clock::now();
@@ -2143,7 +2231,7 @@ void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
   if (!pl->cv_theta) pl->cv_theta = pl->mem.alloc<float>(CV_NW);
   HC(hipMemcpyAsync(pl->cv_theta, th.data(), CV_NW * sizeof(float), hipMemcpyHostToDevice, s));
   ConvSet set;
-  conv_harvest<T>(pl, o.nqueries, o.topn, o.nrandom, rng, set);
+  conv_harvest<T>(pl, o.nqueries, o.topn, o.nhard, o.nrandom, rng, set);
   const auto t1 = std::chrono::steady_clock::now();
   // every fifth query is held out; batches of one mask each, visited in a
   // shuffled order every epoch
@@ -2221,6 +2309,7 @@ void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
     vl = vbatches.empty() ? 0.0 : val_loss();  // leaves B.f = the network's output
     rec = 0.0;
     if (!vq.empty()) {
+      refresh_wt(pl);
       conv_banks(pl, B);
       rec = conv_val_recall<T>(pl, set, vq, o.topn);
     }
@@ -2255,6 +2344,7 @@ void conv_learn(iq_plan* pl, const iq_conv_opts& o) {
   HC(hipMemcpyAsync(pl->cv_theta, best_th, CV_NW * sizeof(float), hipMemcpyDeviceToDevice, s));
   HC(hipMemcpyAsync((float*)H.lrn.W, best_W, wtot * sizeof(float), hipMemcpyDeviceToDevice, s));
   // banks from the trained network's output map
+  refresh_wt(pl);
   conv_forward(s, B, pl->cv_theta);
   for (int m = 0; m < (int)pl->masks.size(); m++) {
     hipLaunchKernelGGL(k_learn_bank, dim3(nblk(g.P, 256)), dim3(256), 0, s, (const IqDev*)pl->dd, m,
@@ -2387,6 +2477,7 @@ int iq_ctx_create(int device, iq_ctx** out) {
     c->device = device;
     HC(hipSetDevice(device));
     HC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HC(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
     scratch_setup(device);
   });
   if (rc != IQ_OK) { delete c; return rc; }

@@ -91,13 +91,16 @@ struct IqLearn {
   int32_t on;                 // shortlist from the bank instead of the full scan
   int32_t K;                  // shortlist size
   const float* W;             // encoders: IQ_EMB x V of each mask, row-major, at wofs[mask]
+  const float* WT;            // the same transposed (V x IQ_EMB per mask), for the query encodings
   const int64_t* wofs;        // per mask (-1: no encoder)
   const uint16_t* bank;       // P x IQ_EMB bf16 rows per encoded mask, at bofs[mask]
   const int64_t* bofs;
   float* eq;                  // per slot: the query's encoding
   uint4* bq;                  // B fragments of the query tiles, [tile][IQ_EMB / 16][64]
   uint32_t* brk;              // per slot: bracket lo, hi, bin shift
-  uint4* binfo;               // per batch position: FP32 B2 bits, bracket lo, hi, shift (bank scan columns)
+  uint4* binfo;               // per batch position: the bracket's top as a float, lo, hi, shift (bank scan columns)
+  int4* stage;                // bank scan: per block, stage_cap listed entries (column, rank, candidate, key)
+  int32_t stage_cap;
   int32_t* lcnt;              // per slot: candidates listed inside the bracket
   int2* list;                 // per slot: lcap (candidate, order key) pairs
   int64_t lcap;
@@ -191,6 +194,7 @@ struct IqDev {
   uint4* bl;                               // B fragments (lo)
   IqLearn lrn;
   int32_t rc_small;  // exact recheck: a wavefront per member up to this superset size, a work-item per member above
+  int32_t rc_lanes;  // 0: the exact recheck runs k_band_recheck_lanes (a lane per member), else k_band_recheck
 };
 
 // Order-preserving 32-bit key of a non-NaN float (-0 and +0 share a key).

@@ -1006,65 +1006,85 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 }
 
 // Exact recheck with a lane per member: the same ordered FMA chain (so the
-// same FP64 value), each lane walking its member's window.  The query values
-// and window offsets are the same for every lane and stream through scalar
-// loads; each lane keeps RCL_B window values of the next block in flight
-// while its chain runs over the current block.  A wavefront serves 64
-// members; grid (query, RCL_GROUPS wavefronts), members beyond loop.
-#define RCL_B 32
-#define RCL_GROUPS 16
-template <bool F32>
-__device__ __forceinline__ void rcl_fetch(const void* ti, int64_t base, const __attribute__((address_space(4))) int32_t* toff,
-                                          int j0, int V, double (&t)[RCL_B]) {
+// same FP64 value), each member's chain run by one lane.  A wavefront serves
+// RCL_MPW members; for every block of 64 masked voxels it loads the members'
+// window values one member at a time (lane l: voxel l of that window, so
+// each load is a few contiguous runs of the window), transposes them through
+// LDS, and lanes 0..RCL_MPW-1 then chain over their member's row, the query
+// values streaming through scalar loads.  The next block's loads are issued
+// before the current block's chains run.  Few members per wavefront keep
+// one round of loads per block and many wavefronts per query.
+// Grid: (query, RCL_GROUPS wavefronts), members beyond loop.
+#define RCL_GROUPS 32
+#define RCL_MPW 16
+template <typename E>
+__device__ __forceinline__ double rcl_chain(const IqDev* __restrict__ D, int64_t base, const int32_t* __restrict__ toff,
+                                            const __attribute__((address_space(4))) double* kv, int V, E (*tile)[65]) {
+  const int lane = threadIdx.x;
+  const auto ti = iq_glob((const E*)D->ti);
+  int64_t bj[RCL_MPW];
 #pragma unroll
-  for (int i = 0; i < RCL_B; i++) {
-    const int j = j0 + i < V ? j0 + i : V - 1;
-    t[i] = F32 ? (double)((const float*)ti)[base + toff[j]] : ((const double*)ti)[base + toff[j]];
+  for (int u = 0; u < RCL_MPW; u++)
+    bj[u] = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)base, u) |
+            ((int64_t)__builtin_amdgcn_readlane((int)(base >> 32), u) << 32);
+  E t[RCL_MPW];
+  {
+    const int32_t o = toff[lane < V ? lane : 0];
+#pragma unroll
+    for (int u = 0; u < RCL_MPW; u++) t[u] = ti[bj[u] + o];
   }
-}
-
-template <bool F32>
-__device__ __forceinline__ double rcl_chain(const IqDev* __restrict__ D, int64_t base,
-                                            const __attribute__((address_space(4))) int32_t* toff,
-                                            const __attribute__((address_space(4))) double* kv, int V) {
-  double cur[RCL_B], nxt[RCL_B];
-  rcl_fetch<F32>(D->ti, base, toff, 0, V, cur);
   double ab = 0.0;
-  for (int j0 = 0; j0 < V; j0 += RCL_B) {
-    if (j0 + RCL_B < V) rcl_fetch<F32>(D->ti, base, toff, j0 + RCL_B, V, nxt);
-    if (j0 + RCL_B <= V) {
+  for (int v0 = 0; v0 < V; v0 += 64) {
+    const int nv = V - v0 < 64 ? V - v0 : 64;
 #pragma unroll
-      for (int i = 0; i < RCL_B; i++) ab = __builtin_fma(cur[i], kv[j0 + i], ab);
-    } else {
-      for (int i = 0; i < V - j0; i++) ab = __builtin_fma(cur[i], kv[j0 + i], ab);
+    for (int u = 0; u < RCL_MPW; u++) tile[u][lane] = t[u];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int v1 = v0 + 64;
+    if (v1 < V) {  // the next block's window values, in flight during the chains
+      const int32_t o = toff[v1 + (v1 + lane < V ? lane : 0)];
+#pragma unroll
+      for (int u = 0; u < RCL_MPW; u++) t[u] = ti[bj[u] + o];
     }
-#pragma unroll
-    for (int i = 0; i < RCL_B; i++) cur[i] = nxt[i];
+    if (lane < RCL_MPW) {
+      const E* row = tile[lane];
+      if (nv == 64) {
+#pragma unroll 16
+        for (int i = 0; i < 64; i++) ab = __builtin_fma((double)row[i], kv[v0 + i], ab);
+      } else {
+        for (int i = 0; i < nv; i++) ab = __builtin_fma((double)row[i], kv[v0 + i], ab);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   return ab;
 }
 
 __global__ void __launch_bounds__(64)
 k_band_recheck_lanes(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
+  __shared__ double tile_mem[RCL_MPW * 65];  // the transposed block (doubles, or floats in its first half)
   const IqQuery q = qs[blockIdx.x];
   const int n = D->nsup[q.slot];
-  if ((int)blockIdx.y * 64 >= n) return;
+  if ((int)blockIdx.y * RCL_MPW >= n) return;
   const IqStep& st = D->steps[q.step];
   if (q.multi || st.mask < 0) return;
   const IqGeo& g = D->g;
   const int64_t slot = q.slot;
   const IqMask& m = D->masks[st.mask];
   const int V = m.V;
-  const auto toff = iq_const(D->toff + m.off);
+  const int32_t* toff = D->toff + m.off;
   const auto kv = iq_const(D->kv + slot * D->kv_stride);
   const double* a2map = D->a2maps[m.a2];
   const double b2 = D->b2[slot * (2 + D->naux)];
   double* dex = D->dexact + slot * g.P;
   const int lane = threadIdx.x;
   unsigned long long best = 0x7ff0000000000000ull;
-  for (int e0 = blockIdx.y * 64; e0 < n; e0 += gridDim.y * 64) {  // uniform over the wavefront
+  for (int e0 = blockIdx.y * RCL_MPW; e0 < n; e0 += gridDim.y * RCL_MPW) {  // uniform over the wavefront
     const int e = e0 + lane;
-    const bool live = e < n;
+    const bool live = lane < RCL_MPW && e < n;
     int64_t c = 0, base = 0;
     if (live) {
       c = D->sup[slot * g.P + e];
@@ -1072,7 +1092,8 @@ k_band_recheck_lanes(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs
       const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
       base = iq_lin3(g.ti, cx, cy, cz);
     }
-    const double ab = D->ti_f32 ? rcl_chain<true>(D, base, toff, kv, V) : rcl_chain<false>(D, base, toff, kv, V);
+    const double ab = D->ti_f32 ? rcl_chain<float>(D, base, toff, kv, V, (float(*)[65])tile_mem)
+                                : rcl_chain<double>(D, base, toff, kv, V, (double(*)[65])tile_mem);
     if (live) {
       const double d = __builtin_fabs((a2map[c] - 2.0 * ab) + b2);
       dex[c] = d;

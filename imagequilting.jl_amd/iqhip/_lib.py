@@ -83,6 +83,7 @@ class ConvOpts(ctypes.Structure):
         ("lr_head", ctypes.c_float),
         ("seed", ctypes.c_uint64),
         ("val_k", ctypes.c_int64),
+        ("nhard", ctypes.c_int),
     ]
 
 

@@ -366,12 +366,12 @@ class Plan:
 
     # -- convolutional encoder (no counterpart in the reference)
     def learn_conv(self, nqueries: int = 640, topn: int = 256, nrandom: int = 64, steps: int = 200, batch: int = 16,
-                   lr: float = 1e-3, lr_head: float = 0.0, seed: int = 0, val_k: int = 0) -> dict:
+                   lr: float = 1e-3, lr_head: float = 0.0, seed: int = 0, val_k: int = 0, nhard: int = 64) -> dict:
         """Train the convolutional encoder on (query, exact distance) pairs of
         the last run's grids and rebuild the banks from it (iq_plan_learn_conv);
         returns its statistics."""
         o = _lib.ConvOpts(int(nqueries), int(topn), int(nrandom), int(steps), int(batch), float(lr), float(lr_head),
-                          int(seed), int(val_k))
+                          int(seed), int(val_k), int(nhard))
         self.ctx.check(self.ctx._L.iq_plan_learn_conv(self.ctx.handle, self.handle, ctypes.byref(o)))
         return self.conv_stats()
 

@@ -288,8 +288,10 @@ int iq_plan_encoder(iq_plan* plan, int mask, float* W, int32_t* tvox, int64_t* V
  * realization (its own uniforms; the grids then hold that run) and takes
  * about `nqueries` plain tile steps, spread over the run, as they run: the
  * overlap values the step sees, its `topn` nearest candidates by exact FP64
- * distance (preselected by the full scan's lower bounds) and `nrandom`
- * random candidates; the loss of a query is
+ * distance (preselected by the full scan's lower bounds), `nhard` members
+ * of its shortlist under the linear encoders that are not among those
+ * nearest (the competitors the band must outrank) and `nrandom` random
+ * candidates; the loss of a query is
  * KL(p* || p~) with p* ~ exp(-(D - min D)/tau) over the exact distances,
  * p~ ~ exp(-(dd - min dd)/tau) over the bank scan's approximate distances
  * dd = A2 - 2 e_c.e_q, tau the larger of the band width tol * min D and the
@@ -320,13 +322,14 @@ int iq_plan_encoder(iq_plan* plan, int mask, float* W, int32_t* tvox, int64_t* V
 typedef struct iq_conv_opts {
   int64_t nqueries;  /* training queries (e.g. 640) */
   int topn;          /* exact nearest candidates per query (e.g. 256) */
-  int nrandom;       /* random candidates per query (e.g. 64); topn + nrandom <= 4096 */
+  int nrandom;       /* random candidates per query (e.g. 64); topn + nhard + nrandom <= 4096 */
   int steps;         /* Adam steps (e.g. 200) */
   int batch;         /* queries per step (e.g. 16) */
   float lr;          /* learning rate of the network (e.g. 1e-3) */
   float lr_head;     /* learning rate of the heads (0: the linear encoders' heads stay) */
   uint64_t seed;
   int64_t val_k;     /* shortlist size of the held-out recall check (0: the plan's K, else 2048) */
+  int nhard;         /* shortlist members of the linear encoders outside the nearest list, per query (e.g. 64) */
 } iq_conv_opts;
 int iq_plan_learn_conv(iq_ctx* ctx, iq_plan* plan, const iq_conv_opts* opts);
 int iq_plan_conv_stats(iq_plan* plan, double out[12]);

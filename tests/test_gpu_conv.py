@@ -109,16 +109,16 @@ def test_trained_network_gains_over_untrained_and_banks_match_oracle():
     U = np.random.default_rng(3).random((2, plan.npath))
     # untrained: random orthonormal heads, network at its initialisation (zero output layer)
     plan.learn(nsamples=2048, iters=0, seed=5)
-    st0 = plan.learn_conv(nqueries=64, topn=64, nrandom=16, steps=0, seed=7, val_k=48)
+    st0 = plan.learn_conv(nqueries=64, topn=64, nrandom=16, nhard=16, steps=0, seed=7, val_k=48)
     assert st0["kept_step"] == 0
     K = 48
     r_untrained = _recall(plan, U, K)
     plan.learn(nsamples=2048, iters=0, seed=5)
     print("train", flush=True)
-    st = plan.learn_conv(nqueries=256, topn=64, nrandom=16, steps=120, batch=8, lr=1e-3, lr_head=3e-3, seed=7,
-                         val_k=48)
+    st = plan.learn_conv(nqueries=256, topn=64, nrandom=16, nhard=16, steps=120, batch=8, lr=1e-3, lr_head=3e-3,
+                         seed=7, val_k=48)
     print("stats", st, flush=True)
-    assert st["pairs"] == st["queries"] * 80 and st["queries"] == 256
+    assert st["pairs"] == st["queries"] * 96 and st["queries"] == 256
     assert st["loss_last"] < st["loss_first"]
     # the kept parameters recall at least as much of the held-out bands as the start
     assert st["heldout_recall_kept"] >= st["heldout_recall_start"] and st["kept_step"] > 0
