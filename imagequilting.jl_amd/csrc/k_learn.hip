@@ -148,10 +148,23 @@ k_learn_embed(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const IqMask& m = D->masks[st.mask];
   const float* WT = D->lrn.WT + D->lrn.wofs[st.mask];  // V x 64: a voxel's 64 weights side by side
   const double* kv = D->kv + (int64_t)q.slot * D->kv_stride;
-  const int j = threadIdx.x & 63, p = threadIdx.x >> 6;
+  const int j = threadIdx.x & 63, p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const auto kvs = iq_const(kv);  // the same voxel for the whole wavefront: scalar loads
   float s = 0.0f;
-  for (int v = p; v < m.V; v += 16) s = __builtin_fmaf(WT[(int64_t)v * IQ_EMB + j], (float)kv[v], s);
-  part[p][j] = s;
+  int v = p;
+  for (; v + 16 * 7 < m.V; v += 16 * 8) {  // eight voxels' loads in flight
+    float wv[8];
+    double qv[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      wv[u] = WT[(int64_t)(v + 16 * u) * IQ_EMB + j];
+      qv[u] = kvs[v + 16 * u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) s = __builtin_fmaf(wv[u], (float)qv[u], s);
+  }
+  for (; v < m.V; v += 16) s = __builtin_fmaf(WT[(int64_t)v * IQ_EMB + j], (float)kvs[v], s);
+  part[threadIdx.x >> 6][j] = s;
   __syncthreads();
   if (p == 0) {
     float t = 0.0f;
@@ -227,11 +240,18 @@ k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const int64_t K = D->lrn.K < D->nen ? D->lrn.K : D->nen;
   if (threadIdx.x == 0) best = ~0ull;
   __syncthreads();
+  unsigned long long mine = ~0ull;
   for (int i = threadIdx.x; i < LKS; i += blockDim.x) {
     const int64_t c = (int64_t)i * g.P / LKS;
     smp[i] = D->disabled[c] ? 0xffffffffu : iq_f32_key(learn_score(D, st.mask, c, eq));
-    atomicMin(&best, ((unsigned long long)smp[i] << 32) | (unsigned)i);
+    const unsigned long long kv = ((unsigned long long)smp[i] << 32) | (unsigned)i;
+    mine = kv < mine ? kv : mine;
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long other = __shfl_xor(mine, o);
+    mine = other < mine ? other : mine;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMin(&best, mine);  // one LDS atomic per wavefront
   unsigned long long* bm = D->bitmap + slot * D->nw64;
   unsigned long long* sb = D->lrn.sbits + slot * D->nw64;
   for (int64_t w = threadIdx.x; w < D->nw64; w += blockDim.x) { bm[w] = 0ull; sb[w] = 0ull; }
@@ -406,6 +426,25 @@ k_learn_mscan(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const
   }
 }
 
+// Exclusive prefix sum of one int per work-item over a block of 1024
+// (wavefront scans, then the 16 wavefront totals); ends with a barrier.
+__device__ inline int block_exclusive_sum_1024(int x, int* wsum) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int inc = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int before = 0;
+  for (int i = 0; i < w; i++) before += wsum[i];
+  __syncthreads();
+  return before + inc - x;
+}
+
+#define LPK_SORT 4096
 // Per query: a 2048-bin histogram (in LDS) of the listed order keys over the
 // bracket, the threshold key where its count reaches K (the bracket's top
 // when the sample underestimated it), then the shortlist: the listed
@@ -414,6 +453,8 @@ __global__ void __launch_bounds__(1024)
 k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   __shared__ uint32_t thr;
   __shared__ int32_t hist[2048];
+  __shared__ int wsum[16];
+  __shared__ int32_t skey[LPK_SORT];
   const IqQuery q = qs[blockIdx.x];
   const IqStep& st = D->steps[q.step];
   if (!learn_applies(D, q, st)) return;
@@ -434,22 +475,24 @@ k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  {
+    // the first bin whose running count reaches K: each work-item holds two
+    // bins, a block-wide prefix sum gives the count before them
     const uint32_t* br = D->lrn.brk + slot * 4;
     const uint32_t lo = br[0], hi = br[1], shift = br[2];
-    int64_t run = 0;
-    uint32_t th = hi;
-    for (int b = 0; b < 2048; b++) {
-      run += hist[b];
-      if (run >= K) {
-        const uint64_t top = (uint64_t)lo + (((uint64_t)b + 1) << shift) - 1;
-        th = top < (uint64_t)hi ? (uint32_t)top : hi;
-        break;
-      }
+    if (threadIdx.x == 0) thr = hi;
+    const int b0 = 2 * (int)threadIdx.x;
+    const int h0 = hist[b0], h1 = hist[b0 + 1];
+    const int ex = block_exclusive_sum_1024(h0 + h1, wsum);
+    int b = -1;
+    if (ex < K && ex + h0 >= K) b = b0;
+    else if (ex + h0 < K && ex + h0 + h1 >= K) b = b0 + 1;
+    if (b >= 0) {  // one work-item at most
+      const uint64_t top = (uint64_t)lo + (((uint64_t)b + 1) << shift) - 1;
+      thr = top < (uint64_t)hi ? (uint32_t)top : hi;
     }
-    thr = th;
+    __syncthreads();
   }
-  __syncthreads();
   const int lane = threadIdx.x & 63;
   for (int64_t e0 = 0; e0 < n; e0 += blockDim.x) {
     const int64_t e = e0 + threadIdx.x;
@@ -479,6 +522,29 @@ k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     D->lrn.slist[slot * g.P] = (int32_t)c;
     D->lrn.nslist[slot] = 1;
     atomicOr(&D->lrn.sbits[slot * D->nw64 + (c >> 6)], 1ull << (c & 63));
+  }
+  __syncthreads();
+  // the shortlist in ascending candidate order (bitonic sort in LDS), so the
+  // bound kernel's neighbouring wavefronts read overlapping windows
+  const int ns = __hip_atomic_load(&D->lrn.nslist[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (ns > 1 && ns <= LPK_SORT) {
+    int32_t* sl = D->lrn.slist + slot * g.P;
+    int N = 2;
+    while (N < ns) N <<= 1;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) skey[i] = i < ns ? sl[i] : 0x7fffffff;
+    __syncthreads();
+    for (int k = 2; k <= N; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+          const int ixj = i ^ j;
+          if (ixj > i) {
+            const int32_t a = skey[i], b = skey[ixj];
+            if ((a > b) == ((i & k) == 0)) { skey[i] = b; skey[ixj] = a; }
+          }
+        }
+        __syncthreads();
+      }
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) sl[i] = skey[i];
   }
 }
 
@@ -520,7 +586,13 @@ k_short_bound(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const int32_t* sl = D->lrn.slist + slot * g.P;
   float* lo_out = D->lo32 + slot * g.P;
   float lmin = __builtin_inff();
-  for (int e = blockIdx.x * 8 + w; e < n; e += gridDim.x * 8) {
+  // group b takes a contiguous range of the (sorted) shortlist; ranges are
+  // dealt so that the groups of one XCD (b mod 8 when groups are dispatched
+  // round robin) cover neighbouring ranges, and its L2 sees nearby windows
+  const int G = gridDim.x;
+  const int r = (G % 8 == 0) ? ((int)blockIdx.x % 8) * (G / 8) + (int)blockIdx.x / 8 : (int)blockIdx.x;
+  const int ebeg = (int)((int64_t)n * r / G), eend = (int)((int64_t)n * (r + 1) / G);
+  for (int e = ebeg + w; e < eend; e += 8) {
     const int64_t c = sl[e];
     const int cx = (int)(c % g.dist[0]), cy = (int)((c / g.dist[0]) % g.dist[1]);
     const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
