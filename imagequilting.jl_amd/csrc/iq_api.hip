@@ -56,6 +56,15 @@ struct iq_ctx {
 };
 
 namespace {
+// LDS bytes of one slab's cut of n interior nodes: register-resident form
+// (at most 4 CB nodes, or any slab of a plan run by the large-slab kernel),
+// LDS-resident form (up to IQ_PR_NLDS), else none (global scratch)
+size_t cut_lds_bytes(int64_t n, bool big) {
+  if (n <= 4 * CB || (big && n <= CUT_BIG_MAX)) return (size_t)n * CUT_FAST_NODE_BYTES;
+  if (n <= IQ_PR_NLDS) return (size_t)n * CUT_NODE_BYTES;
+  return 0;
+}
+
 int fail(const iq_ctx* ctx, int code, const std::string& msg) {
   g_last_error = msg;
   if (ctx) const_cast<iq_ctx*>(ctx)->err = msg;
@@ -965,12 +974,15 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
   for (const IqStep& st : pl->steps)
     for (int q = 0; q < st.nslab; q++) {
       const IqSlab& sl = st.slab[q];
-      int64_t n = (int64_t)sl.sz[0] * sl.sz[1] * sl.sz[2] / sl.sz[sl.dim] * std::max(0, sl.sz[sl.dim] - 2);
-      if (n <= std::max<int64_t>(IQ_PR_NLDS, CUT_BIG_MAX))
-        pl->cut_lds_max = std::max(pl->cut_lds_max, (size_t)n * CUT_FAST_NODE_BYTES);
-      maxn = std::max(maxn, n);
+      maxn = std::max(maxn, (int64_t)sl.sz[0] * sl.sz[1] * sl.sz[2] / sl.sz[sl.dim] * std::max(0, sl.sz[sl.dim] - 2));
     }
   pl->cut_big = maxn > 4 * CB && maxn <= CUT_BIG_MAX;
+  for (const IqStep& st : pl->steps)
+    for (int q = 0; q < st.nslab; q++) {
+      const IqSlab& sl = st.slab[q];
+      const int64_t n = (int64_t)sl.sz[0] * sl.sz[1] * sl.sz[2] / sl.sz[sl.dim] * std::max(0, sl.sz[sl.dim] - 2);
+      pl->cut_lds_max = std::max(pl->cut_lds_max, cut_lds_bytes(n, pl->cut_big));
+    }
   pl->cut_wave = maxn <= CUT_WAVE_MAX;
   const int64_t cutper = (maxn > IQ_PR_NLDS && !pl->cut_big) ? maxn * CUT_NODE_BYTES : 0;
   size_t per_slot = (size_t)nmap * g.tilevox * 8 + (size_t)kmax * 4 + 64 + (size_t)P * (4 + 8 + 4 + 4 + 16) + nw64 * 8 +
@@ -2430,7 +2442,7 @@ int64_t cut_check(int dtype, const int64_t dims[3], int dim) {
 // err: device status word, zero before the launch.
 void cut_launch(hipStream_t s, const void* dA, const void* dB, int dtype, const int64_t dims[3], int dim, int64_t n,
                 uint8_t* keep, char* gscr, int32_t* err) {
-  const size_t lds = n > IQ_PR_NLDS ? 0 : (size_t)n * CUT_FAST_NODE_BYTES;
+  const size_t lds = cut_lds_bytes(n, false);
   if (lds > 64 * 1024) {
     HC(hipFuncSetAttribute((const void*)k_graphcut_host<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     HC(hipFuncSetAttribute((const void*)k_graphcut_host<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -275,13 +275,19 @@ __device__ void cut_slab(const SlabView<T>& A, const SlabView<T>& B, const int s
   }
 }
 
-// Register-resident form of cut_slab for slabs of at most NPT * CB interior
-// nodes: same schedule, same arithmetic, bit-identical result.  A work-item
-// owns nodes u = tid + k * CB; their coordinates, excess, sink residual and
-// flags stay in registers (only the owner touches them); heights (double
-// buffered) and the edge residual / push records live in LDS.  The gather of
-// round k runs at the start of round k+1 (push records double buffered), so a
-// round costs two barriers.
+// Register-resident form of cut_slab for slabs of at most NPT * TEAM
+// interior nodes: same schedule, same arithmetic, bit-identical result.  A
+// work-item owns nodes u = tid + k * TEAM; their coordinates, excess, sink
+// residual, flags, height and the residuals of their six outgoing edges
+// (towards the minus and the plus neighbour per axis) stay in registers:
+// only the owner reads them.  A push from u to w lowers u's outgoing
+// residual at once and leaves the amount in w's inbox (LDS, one slot per
+// direction a push can come from, so each slot has one writer); w adds it
+// to its excess and to its own outgoing residual on that edge at the start
+// of the next round, in the same fixed order as the reference schedule's
+// gather.  LDS holds only the inboxes (6 doubles per node) and the heights
+// (double buffered, 16-bit): 52 bytes per node, so a 1,536-node slab takes
+// 78 KB and two slabs fit in one compute unit.  A round costs two barriers.
 // Team synchronisation of the register-resident cut: a 64-wide team is one
 // wavefront (its LDS accesses complete in order; the fences only stop the
 // compiler from moving them across the step), wider teams use the
@@ -318,12 +324,7 @@ __device__ inline bool team_or(int p, int* tflag, int& par) {
   }
 }
 
-struct CutLds {
-  double *rf, *rb, *pr0, *pr1;
-  int32_t *h0, *h1;
-};
-
-#define CUT_FAST_NODE_BYTES (8 * 12 + 4 * 2)
+#define CUT_FAST_NODE_BYTES (8 * 6 + 2 * 2)
 
 template <typename T, int NPT, int TEAM>
 __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabView<T>& B, const int s[3], int dim, uint8_t side,
@@ -343,19 +344,21 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
   const int ist[3] = {1, si[0], si[0] * si[1]};
   __shared__ int tflag[2 * (TEAM / 64)];
   int par = 0;
-  CutLds S;
-  S.rf = (double*)mem;
-  S.rb = S.rf + 3 * n;
-  S.pr0 = S.rb + 3 * n;
-  S.pr1 = S.pr0 + 3 * n;
-  S.h0 = (int32_t*)(S.pr1 + 3 * n);
-  S.h1 = S.h0 + n;
+  // inbox of node w: [6 w + d] the amount its minus neighbour along d pushed
+  // up to it, [6 w + 3 + d] what its plus neighbour pushed down
+  double* inb = (double*)mem;
+  int16_t* h = (int16_t*)(inb + 6 * n);  // committed heights
+  int16_t* hn = h + n;                   // next heights
   // per owned node: flags (bit d: plus edge d, bit 3: sink edge, bits 4+d: minus edge d),
-  // neighbour indices (clamped to the node itself where absent), excess, sink residual
+  // excess, sink residual, outgoing residuals towards the minus (om) and plus
+  // (op) neighbours, height; neighbour indices are formed where used
+  // (clamped to the node itself where the edge is absent)
   int fl[NPT];
   int uu[NPT];
-  int vm[NPT][3], vp[NPT][3];
-  double e[NPT], rt[NPT];
+  double e[NPT], rt[NPT], om[NPT][3], op[NPT][3];
+  int hr[NPT];
+#define CUT_VM(k, d) ((fl[k] & (16 << (d))) ? uu[k] - ist[d] : uu[k])
+#define CUT_VP(k, d) ((fl[k] & (1 << (d))) ? uu[k] + ist[d] : uu[k])
 #pragma unroll
   for (int k = 0; k < NPT; k++) {
     const int u = threadIdx.x + k * TEAM;
@@ -363,21 +366,28 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
     fl[k] = 0;
     e[k] = 0.0;
     rt[k] = 0.0;
+    hr[k] = HMAX;
 #pragma unroll
-    for (int d = 0; d < 3; d++) { vm[k][d] = u < n ? u : 0; vp[k][d] = u < n ? u : 0; }
+    for (int d = 0; d < 3; d++) {
+      om[k][d] = 0.0;
+      op[k][d] = 0.0;
+    }
     if (u < n) {
       int ic[3] = {u % si[0], (u / si[0]) % si[1], u / (si[0] * si[1])};
       int pc[3] = {ic[0], ic[1], ic[2]};
       pc[dim] += 1;
       int f = 0;
       for (int d = 0; d < 3; d++) {
-        double c = 0.0;
-        if (ic[d] + 1 < si[d]) { f |= 1 << d; c = slab_cap<T>(A, B, s, pc[0], pc[1], pc[2], d); vp[k][d] = u + ist[d]; }
-        if (ic[d] > 0) { f |= 16 << d; vm[k][d] = u - ist[d]; }
-        S.rf[3 * u + d] = c;
-        S.rb[3 * u + d] = c;
-        S.pr0[3 * u + d] = 0.0;
-        S.pr1[3 * u + d] = 0.0;
+        // an edge's capacity is the same both ways; each end evaluates it
+        if (ic[d] + 1 < si[d]) { f |= 1 << d; op[k][d] = slab_cap<T>(A, B, s, pc[0], pc[1], pc[2], d); }
+        if (ic[d] > 0) {
+          int q[3] = {pc[0], pc[1], pc[2]};
+          q[d] -= 1;
+          f |= 16 << d;
+          om[k][d] = slab_cap<T>(A, B, s, q[0], q[1], q[2], d);
+        }
+        inb[6 * u + d] = 0.0;
+        inb[6 * u + 3 + d] = 0.0;
       }
       if (pc[dim] == 1) {
         int q[3] = {pc[0], pc[1], pc[2]};
@@ -391,23 +401,12 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
   team_sync<TEAM>();
 
   int64_t total = 0, levels = 0;
-  int32_t* h = S.h0;   // committed heights
-  int32_t* hn = S.h1;  // next heights
   for (;;) {
     // global relabel (exact residual distance to the sink)
 #pragma unroll
     for (int k = 0; k < NPT; k++)
-      if (uu[k] < n) h[uu[k]] = ((fl[k] & 8) && rt[k] > 0.0) ? 1 : HMAX;
+      if (uu[k] < n) h[uu[k]] = (int16_t)(((fl[k] & 8) && rt[k] > 0.0) ? 1 : HMAX);
     team_sync<TEAM>();
-    // residual reachability towards the sink does not change during the BFS: load it once
-    bool canm[NPT][3], canp[NPT][3];
-#pragma unroll
-    for (int k = 0; k < NPT; k++)
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        canm[k][d] = (fl[k] & (16 << d)) && S.rb[3 * vm[k][d] + d] > 0.0;
-        canp[k][d] = (fl[k] & (1 << d)) && S.rf[3 * uu[k] + d] > 0.0;
-      }
     // The labels are the residual hop distances to the sink (+1): the least
     // fixed point of h(u) = min over residual edges u->v of h(v) + 1 from
     // these starting values, which relaxing in any order reaches as well as
@@ -429,14 +428,14 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
           const int hu = h[u];
           int hm[3], hp[3];
 #pragma unroll
-          for (int d = 0; d < 3; d++) { hm[d] = h[vm[k][d]]; hp[d] = h[vp[k][d]]; }
+          for (int d = 0; d < 3; d++) { hm[d] = h[CUT_VM(k, d)]; hp[d] = h[CUT_VP(k, d)]; }
           int best = hu;
 #pragma unroll
           for (int d = 0; d < 3; d++) {
-            if (canm[k][d] && hm[d] + 1 < best) best = hm[d] + 1;
-            if (canp[k][d] && hp[d] + 1 < best) best = hp[d] + 1;
+            if ((fl[k] & (16 << d)) && om[k][d] > 0.0 && hm[d] + 1 < best) best = hm[d] + 1;
+            if ((fl[k] & (1 << d)) && op[k][d] > 0.0 && hp[d] + 1 < best) best = hp[d] + 1;
           }
-          if (best < hu) { h[u] = best; lc = 1; }
+          if (best < hu) { h[u] = (int16_t)best; lc = 1; }
         }
         if (!__any(lc)) break;
         ch = 1;
@@ -450,39 +449,45 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
     int act = 0;
 #pragma unroll
     for (int k = 0; k < NPT; k++)
-      if (uu[k] < n) act |= (e[k] > 0.0 && h[uu[k]] < HMAX);
+      if (uu[k] < n) {
+        hr[k] = h[uu[k]];
+        act |= (e[k] > 0.0 && hr[k] < HMAX);
+      }
     if (!team_or<TEAM>(act, tflag, par)) break;
     if (total > IQ_PR_LIMIT) {
       if (threadIdx.x == 0) atomicCAS(err, 0, IQ_KERR_ITER);
       break;
     }
-    double* prc = S.pr0;  // records written by this round's push
-    double* pro = S.pr1;  // records of the previous round (gathered, then cleared)
     for (int it = 0; it <= IQ_GR_FREQ; it++) {
-      // phase A: gather the previous round's pushes, then relabel
+      // phase A: take in the previous round's pushes, then relabel
       int a2 = 0;
+      int hnx[NPT];
 #pragma unroll
       for (int k = 0; k < NPT; k++) {
         const int u = uu[k];
+        hnx[k] = hr[k];
         if (u >= n) continue;
-        double qm[3], qp[3], rbm[3], rfp[3];
-        int hm[3], hp[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          qm[d] = pro[3 * vm[k][d] + d];
-          qp[d] = pro[3 * u + d];
-          rbm[d] = S.rb[3 * vm[k][d] + d];
-          rfp[d] = S.rf[3 * u + d];
-          hm[d] = h[vm[k][d]];
-          hp[d] = h[vp[k][d]];
-        }
-        const int hu = h[u];
+        const int hu = hr[k];
         if (it > 0) {
+          double im[3], ip[3];
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            im[d] = inb[6 * u + d];
+            ip[d] = inb[6 * u + 3 + d];
+          }
           double ex = e[k];
 #pragma unroll
           for (int d = 0; d < 3; d++) {
-            if ((fl[k] & (16 << d)) && qm[d] > 0.0) ex += qm[d];
-            if ((fl[k] & (1 << d)) && qp[d] < 0.0) ex += -qp[d];
+            if ((fl[k] & (16 << d)) && im[d] > 0.0) {
+              ex += im[d];
+              om[k][d] = om[k][d] + im[d];
+              inb[6 * u + d] = 0.0;
+            }
+            if ((fl[k] & (1 << d)) && ip[d] > 0.0) {
+              ex += ip[d];
+              op[k][d] = op[k][d] + ip[d];
+              inb[6 * u + 3 + d] = 0.0;
+            }
           }
           e[k] = ex;
         }
@@ -490,17 +495,24 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
         const bool active = e[k] > 0.0 && hu < HMAX;
         a2 |= active;
         if (active && it < IQ_GR_FREQ) {
+          int hm[3], hp[3];
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            hm[d] = h[CUT_VM(k, d)];
+            hp[d] = h[CUT_VP(k, d)];
+          }
           bool adm = false;
           int mh = HMAX;
           if ((fl[k] & 8) && rt[k] > 0.0) { if (hu == 1) adm = true; mh = 0; }
 #pragma unroll
           for (int d = 0; d < 3; d++) {
-            if ((fl[k] & (16 << d)) && rbm[d] > 0.0) { if (hu == hm[d] + 1) adm = true; if (hm[d] < mh) mh = hm[d]; }
-            if ((fl[k] & (1 << d)) && rfp[d] > 0.0) { if (hu == hp[d] + 1) adm = true; if (hp[d] < mh) mh = hp[d]; }
+            if ((fl[k] & (16 << d)) && om[k][d] > 0.0) { if (hu == hm[d] + 1) adm = true; if (hm[d] < mh) mh = hm[d]; }
+            if ((fl[k] & (1 << d)) && op[k][d] > 0.0) { if (hu == hp[d] + 1) adm = true; if (hp[d] < mh) mh = hp[d]; }
           }
           if (!adm) nh = (mh + 1 < HMAX) ? mh + 1 : HMAX;
         }
-        hn[u] = nh;
+        hn[u] = (int16_t)nh;
+        hnx[k] = nh;
       }
       if (it > 0) {
         total++;
@@ -509,29 +521,23 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
         team_sync<TEAM>();
       }
       if (it == IQ_GR_FREQ) break;
-      { int32_t* t = h; h = hn; hn = t; }
-      // phase B: push with the new heights; clear the gathered records
+      { int16_t* t = h; h = hn; hn = t; }
+#pragma unroll
+      for (int k = 0; k < NPT; k++) hr[k] = hnx[k];
+      // phase B: push with the new heights (an edge has one pusher per
+      // round, its higher end: the residuals it reads are its own)
 #pragma unroll
       for (int k = 0; k < NPT; k++) {
         const int u = uu[k];
         if (u >= n) continue;
-        pro[3 * u] = 0.0; pro[3 * u + 1] = 0.0; pro[3 * u + 2] = 0.0;
         double ex = e[k];
-        const int hu = h[u];
+        const int hu = hr[k];
         if (!(ex > 0.0 && hu < HMAX)) continue;
-        // both directions of every incident edge are read up front: an edge
-        // has one pusher per round (its higher end), so nothing read here is
-        // written by another work-item in this phase
-        double rbm[3], rfm[3], rfp[3], rbp[3];
         int hm[3], hp[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-          rbm[d] = S.rb[3 * vm[k][d] + d];
-          rfm[d] = S.rf[3 * vm[k][d] + d];
-          rfp[d] = S.rf[3 * u + d];
-          rbp[d] = S.rb[3 * u + d];
-          hm[d] = h[vm[k][d]];
-          hp[d] = h[vp[k][d]];
+          hm[d] = h[CUT_VM(k, d)];
+          hp[d] = h[CUT_VP(k, d)];
         }
         if ((fl[k] & 8) && hu == 1 && rt[k] > 0.0) {
           double r = rt[k];
@@ -542,40 +548,28 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           if (ex > 0.0 && (fl[k] & (16 << d)) && hu == hm[d] + 1) {
-            double r = rbm[d];
+            double r = om[k][d];
             if (r > 0.0) {
-              const int v = vm[k][d];
               double dl = ex < r ? ex : r;
               ex -= dl;
-              S.rb[3 * v + d] = r - dl;
-              S.rf[3 * v + d] = rfm[d] + dl;
-              prc[3 * v + d] = -dl;
+              om[k][d] = r - dl;
+              inb[6 * (u - ist[d]) + 3 + d] = dl;
             }
           }
           if (ex > 0.0 && (fl[k] & (1 << d)) && hu == hp[d] + 1) {
-            double r = rfp[d];
+            double r = op[k][d];
             if (r > 0.0) {
               double dl = ex < r ? ex : r;
               ex -= dl;
-              S.rf[3 * u + d] = r - dl;
-              S.rb[3 * u + d] = rbp[d] + dl;
-              prc[3 * u + d] = dl;
+              op[k][d] = r - dl;
+              inb[6 * (u + ist[d]) + d] = dl;
             }
           }
         }
         e[k] = ex;
       }
       team_sync<TEAM>();
-      { double* t = prc; prc = pro; pro = t; }
     }
-    // records of the last gathered round are cleared before the next cycle
-#pragma unroll
-    for (int k = 0; k < NPT; k++) {
-      const int u = uu[k];
-      if (u < n) { pro[3 * u] = 0.0; pro[3 * u + 1] = 0.0; pro[3 * u + 2] = 0.0;
-                   prc[3 * u] = 0.0; prc[3 * u + 1] = 0.0; prc[3 * u + 2] = 0.0; }
-    }
-    team_sync<TEAM>();
   }
 #pragma unroll
   for (int k = 0; k < NPT; k++) {
@@ -592,6 +586,8 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
     atomicAdd((unsigned long long*)rounds + 1, (unsigned long long)levels);
     atomicAdd((unsigned long long*)rounds + 2, 1ull);
   }
+#undef CUT_VM
+#undef CUT_VP
 }
 
 template <typename T>
