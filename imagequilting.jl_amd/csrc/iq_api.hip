@@ -1125,7 +1125,7 @@ void launch_mscan(iq_plan* pl, const IqQuery* q, const int4* tiles, int ntiles, 
 }
 
 // exact FP64 recheck of the band supersets of a batch: with a lane per
-// member (plans with long chains), or a wavefront per member
+// member, or (IQHIP_RECHECK=groups) a wavefront per member
 void launch_recheck(iq_plan* pl, const IqQuery* q, int nq, hipStream_t s) {
   if (pl->hd.rc_lanes == 0)
     hipLaunchKernelGGL(k_band_recheck_lanes, dim3(nq, RCL_GROUPS), dim3(64), 0, s, (const IqDev*)pl->dd, q);
