@@ -11,8 +11,12 @@ same plan with the same uniforms:
               interval bound, FP64 recheck of the band superset);
   learned     the shortlist of the K best approximate distances (bank scan of
               64-d bf16 encodings) replaces the full scan, reranked in FP64;
-              every check_every-th batch also runs the full scan, which decides
-              that batch and measures recall on its exact band.
+              every check_every-th batch (default 16) hands its first
+              check_queries queries (default 32, one MFMA tile) to the full
+              scan, which decides them and measures recall on their exact
+              bands; recall falling below 99 % returns the run to the full
+              scan.  With the conv encoder, one more run per encoder checks
+              every query of every batch (encoder_ablation_every_batch_checked).
 One step = one complete simulation of all realizations (every tile step:
 distances, band, weights, draw, min cuts, paste) plus the download of the
 output grids to the host.  value = learned voxels/s; the full scan's voxels/s,
@@ -193,7 +197,9 @@ def main():
     ap.add_argument("--workload", type=int, default=5, choices=sorted(WORKLOADS),
                     help="5 (default): learned + batched vs the full scan; 1-4: the full scan alone")
     ap.add_argument("--K", type=int, default=2048, help="shortlist size of the learned proposal")
-    ap.add_argument("--check-every", type=int, default=32, help="full-scan recall check every N-th batch")
+    ap.add_argument("--check-every", type=int, default=16, help="full-scan recall check every N-th batch")
+    ap.add_argument("--check-queries", type=int, default=32,
+                    help="queries of a check batch the full scan decides and measures recall on (0: all)")
     ap.add_argument("--full-steps", type=int, default=-1, help="timed steps of the full-scan leg (default: --steps)")
     ap.add_argument("--encoder", choices=("conv", "linear"), default="conv",
                     help="learned leg's encoder: the trained convolutional network (default) or the linear one alone")
@@ -292,7 +298,8 @@ def main():
             ablation = {"K": args.K, "checked_queries": st_conv["queries"],
                         "recall_linear_encoder": st_lin["recall"], "recall_conv_encoder": st_conv["recall"],
                         "fully_recalled_linear": st_lin["fully_recalled"], "fully_recalled_conv": st_conv["fully_recalled"]}
-        plan.set_proposal(True, K=args.K, check_every=args.check_every, min_recall=0.99)
+        plan.set_proposal(True, K=args.K, check_every=args.check_every, min_recall=0.99,
+                          check_queries=args.check_queries)
         l_el, l_dev = timed_leg(plan, U, args.warmup, args.steps, buf)
         st = plan.proposal_stats()  # of the last timed run (every run draws the same way)
         l_prof = profiled_run(plan, U[0])
@@ -310,7 +317,8 @@ def main():
             "avg_launch_us": bank_s / max(bank_n, 1) * 1e6, "bytes_per_launch": bank_b / max(bank_n, 1),
         }
         res["config"]["learned_proposal"] = {
-            "K": args.K, "check_every": args.check_every, "min_recall": 0.99,
+            "K": args.K, "check_every": args.check_every, "check_queries": args.check_queries or "all",
+            "min_recall": 0.99,
             "recall": st["recall"], "checked_queries": st["queries"], "fully_recalled_queries": st["fully_recalled"],
             "fell_back": st["fell_back"], "learn_seconds": learn_s,
             "encoder": ("3-layer 3x3x3 conv (1-32-32-1, residual) + 64-d linear heads, trained on the GPU"

@@ -375,6 +375,7 @@ struct iq_plan {
   // learned proposal
   bool lrn_ready = false, lrn_on = false, lrn_fell_back = false;
   int lrn_check = 8;
+  int lrn_check_q = 0;  // queries of a check batch the full scan decides (0: all)
   double lrn_min_recall = 0.99;
   int64_t lrn_stats[4] = {0, 0, 0, 0};
   std::vector<int64_t> lrn_wofs, lrn_bofs;
@@ -1207,17 +1208,26 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     hipLaunchKernelGGL(k_learn_pick, dim3(nq), dim3(1024), 0, s, D, q);
     HCL();
   }
-  if (has_plain && sel == SEL_LEARNED) {
+  // a check batch may hand only its first lrn_check_q queries to the full
+  // scan (which decides them and measures recall on them); the shortlist
+  // decides the rest as in a learned batch
+  const int nfull = sel == SEL_LEARNED ? 0 : (sel == SEL_CHECK && pl->lrn_check_q > 0 ? std::min(nq, pl->lrn_check_q) : nq);
+  if (has_plain && nfull < nq) {
     // shortlist -> FP32 bounds of its members -> band superset -> exact recheck
-    hipLaunchKernelGGL(k_short_bound, dim3(32, nq), dim3(512), (size_t)pl->lrn_vmax * 8, s, D, q);
+    const IqQuery* ql = q + nfull;
+    const int nl = nq - nfull;
+    hipLaunchKernelGGL(k_short_bound, dim3(32, nl), dim3(512), (size_t)pl->lrn_vmax * 8, s, D, ql);
     HCL();
-    hipLaunchKernelGGL(k_short_select, dim3(16, nq), dim3(256), 0, s, D, q);
+    hipLaunchKernelGGL(k_short_select, dim3(16, nl), dim3(256), 0, s, D, ql);
     HCL();
-    launch_recheck(pl, q, nq, s);
+    launch_recheck(pl, ql, nl, s);
     HCL();
-    hipLaunchKernelGGL(k_band_finalize, dim3(nq), dim3(FB), 0, s, D, q, 1);
+    hipLaunchKernelGGL(k_band_finalize, dim3(nl), dim3(FB), 0, s, D, ql, 1);
     HCL();
-  } else if (has_plain) {
+  }
+  if (has_plain && nfull > 0) {
+    // the full scan over the batch's first nf queries
+    const int nf = nfull;
     if (pl->fast) {
       if (pl->prof && pl->prof_used + 2 > (int64_t)pl->prof_ev.size()) {
         hipEvent_t a, b;
@@ -1227,9 +1237,9 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
         pl->prof_ev.push_back(b);
       }
       if (pl->mfma) {
-        const int ntile = (nq + 31) / 32;
+        const int ntile = (nf + 31) / 32;
         if (!fused_prep) {
-          hipLaunchKernelGGL(k_mfma_bprep, dim3(pl->hd.ktab.nstep, ntile), dim3(64), 0, s, D, q, nq);
+          hipLaunchKernelGGL(k_mfma_bprep, dim3(pl->hd.ktab.nstep, ntile), dim3(64), 0, s, D, q, nf);
           HCL();
         }
         const int nw = pl->mfma_nw_forced ? pl->mfma_nw : pl->nw_tiles[ntile > 1 ? 2 : 1];
@@ -1245,39 +1255,39 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
                             (2.0 * 32 * 32 * 16);
         }
         if (pl->mfma_split)
-          hipLaunchKernelGGL((k_scan_mfma<true>), grid, dim3(64 * nw), pl->lds_nw[nw], s, D, q, nq, pl->mfma_gerr,
+          hipLaunchKernelGGL((k_scan_mfma<true>), grid, dim3(64 * nw), pl->lds_nw[nw], s, D, q, nf, pl->mfma_gerr,
                              pl->kofs_nw[nw]);
         else
-          hipLaunchKernelGGL((k_scan_mfma<false>), grid, dim3(64 * nw), pl->lds_nw[nw], s, D, q, nq, pl->mfma_gerr,
+          hipLaunchKernelGGL((k_scan_mfma<false>), grid, dim3(64 * nw), pl->lds_nw[nw], s, D, q, nf, pl->mfma_gerr,
                              pl->kofs_nw[nw]);
         HCL();
       } else {
         if (pl->prof) HC(hipEventRecord(pl->prof_ev[pl->prof_used], s));
-        launch_scan_batch(pl, q, nq, s);
+        launch_scan_batch(pl, q, nf, s);
       }
       if (pl->prof) {
         HC(hipEventRecord(pl->prof_ev[pl->prof_used + 1], s));
         pl->prof_used += 2;
         double f = 0.0;
-        for (int i = 0; i < nq; i++) {
+        for (int i = 0; i < nf; i++) {
           const IqStep& st = pl->steps[hq[i].step];
           if (!hq[i].multi && st.mask >= 0) f += 2.0 * (double)P * (double)pl->masks[st.mask].V;
         }
         pl->batch_flop.push_back(f);
         pl->prof_flop += f;
       }
-      hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256 * SEL_PT), nq), dim3(256), 0, s, D, q);
+      hipLaunchKernelGGL(k_band_select, dim3(nblk(P, 256 * SEL_PT), nf), dim3(256), 0, s, D, q);
       HCL();
-      launch_recheck(pl, q, nq, s);
+      launch_recheck(pl, q, nf, s);
       HCL();
     } else {
-      hipLaunchKernelGGL(k_band_exact_all, dim3(nblk(P, 256), nq), dim3(256), 0, s, D, q);
+      hipLaunchKernelGGL(k_band_exact_all, dim3(nblk(P, 256), nf), dim3(256), 0, s, D, q);
       HCL();
     }
-    hipLaunchKernelGGL(k_band_finalize, dim3(nq), dim3(FB), 0, s, D, q, pl->fast ? 1 : 0);
+    hipLaunchKernelGGL(k_band_finalize, dim3(nf), dim3(FB), 0, s, D, q, pl->fast ? 1 : 0);
     HCL();
     if (sel == SEL_CHECK) {
-      hipLaunchKernelGGL(k_learn_recall, dim3(nq), dim3(256), 0, s, D, q);
+      hipLaunchKernelGGL(k_learn_recall, dim3(nf), dim3(256), 0, s, D, q);
       HCL();
     }
   }
@@ -2954,6 +2964,13 @@ int iq_plan_bank_rows(iq_ctx* ctx, iq_plan* plan, int mask, const int32_t* cands
       }
     }
   });
+}
+
+int iq_plan_set_check_sample(iq_plan* plan, int queries) {
+  if (!plan) return fail(nullptr, IQ_ERR_ARG, "null plan");
+  if (queries < 0) return fail(plan->ctx, IQ_ERR_ARG, "negative check sample");
+  plan->lrn_check_q = queries;
+  return IQ_OK;
 }
 
 int iq_plan_set_proposal(iq_plan* plan, int enable, int64_t K, int check_every, double min_recall) {

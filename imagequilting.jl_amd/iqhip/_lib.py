@@ -33,7 +33,7 @@ EXPORTS = (
     "iq_graphcut", "iq_plan_create", "iq_plan_destroy", "iq_plan_run", "iq_plan_results",
     "iq_plan_create_multi", "iq_plan_last_time", "iq_plan_profile", "iq_plan_scan_stats", "iq_plan_scan_tuning", "iq_plan_cut_time", "iq_plan_cut_stats",
     "iq_simulate", "iq_prepare", "iq_prepare_soft", "iq_finddisabled",
-    "iq_plan_learn", "iq_plan_set_proposal", "iq_plan_proposal_stats", "iq_plan_encoder", "iq_plan_select_stats",
+    "iq_plan_learn", "iq_plan_set_proposal", "iq_plan_set_check_sample", "iq_plan_proposal_stats", "iq_plan_encoder", "iq_plan_select_stats",
     "iq_plan_validate", "iq_scan_error_bound", "iq_plan_bank_stats", "iq_graphcut_noalloc",
     "iq_plan_learn_conv", "iq_plan_conv_stats", "iq_plan_conv_params", "iq_plan_conv_grad", "iq_plan_bank_rows",
 )
@@ -138,6 +138,7 @@ def load(path: str = LIB_PATH):
             "iq_prepare": [vp, c_int, vp, vp, vp],
             "iq_plan_learn": [vp, vp, i64, c_int, ctypes.c_uint64],
             "iq_plan_set_proposal": [vp, c_int, i64, c_int, f64],
+            "iq_plan_set_check_sample": [vp, c_int],
             "iq_plan_proposal_stats": [vp, vp],
             "iq_plan_encoder": [vp, c_int, vp, vp, vp, vp],
             "iq_plan_select_stats": [vp, vp, vp, vp],

@@ -341,11 +341,14 @@ class Plan:
         """Learn the per-mask encoders and encode every candidate (iq_plan_learn)."""
         self.ctx.check(self.ctx._L.iq_plan_learn(self.ctx.handle, self.handle, int(nsamples), int(iters), int(seed)))
 
-    def set_proposal(self, enable: bool = True, K: int = 4096, check_every: int = 8, min_recall: float = 0.99) -> None:
+    def set_proposal(self, enable: bool = True, K: int = 4096, check_every: int = 8, min_recall: float = 0.99,
+                     check_queries: int = 0) -> None:
         """Shortlist the K best approximate distances instead of the full scan;
-        every check_every-th batch the full scan decides and measures recall."""
+        every check_every-th batch the full scan decides and measures recall,
+        on its first check_queries queries (0: on all of them)."""
         self.ctx.check(self.ctx._L.iq_plan_set_proposal(self.handle, int(bool(enable)), int(K), int(check_every),
                                                         float(min_recall)))
+        self.ctx.check(self.ctx._L.iq_plan_set_check_sample(self.handle, int(check_queries)))
 
     def proposal_stats(self) -> dict:
         v = np.zeros(5, dtype=np.int64)

@@ -260,12 +260,17 @@ int iq_plan_select_stats(iq_plan* plan, int64_t* queries, int64_t* superset, int
  * iq_plan_learn: one linear encoder (64 outputs) per overlap-mask shape of
  * the plan, learned from nsamples training-image windows by iters rounds of
  * block subspace iteration (seeded; iters = 0 leaves a random orthonormal
- * encoder), and the bank of every candidate's encoding (256 bytes per
- * candidate and mask shape).
+ * encoder), and the bank of every candidate's encoding (64 bf16, 128 bytes
+ * per candidate and mask shape).
  * iq_plan_set_proposal: use the shortlist in iq_plan_run (enable), its size
  * K, a full-scan check every check_every-th batch (0: never), and return to
  * the full scan for the rest of a run once the cumulative recall of the
  * checks falls below min_recall.
+ * iq_plan_set_check_sample: in a check batch, only the first `queries`
+ * queries (0: all of them, the default) go to the full scan, which decides
+ * them and measures recall on their exact bands; the shortlist decides the
+ * others.  No counterpart in the reference: the sampled recall check of
+ * the learned proposal.
  * iq_plan_proposal_stats: for the last run, out[0] band members of the
  * checked queries, out[1] those inside the shortlist, out[2] checked
  * queries, out[3] queries with the whole band inside, out[4] 1 if the run
@@ -275,6 +280,7 @@ int iq_plan_select_stats(iq_plan* plan, int64_t* queries, int64_t* superset, int
  * go to tvox); any output may be NULL; *nmasks = number of mask shapes. */
 int iq_plan_learn(iq_ctx* ctx, iq_plan* plan, int64_t nsamples, int iters, uint64_t seed);
 int iq_plan_set_proposal(iq_plan* plan, int enable, int64_t K, int check_every, double min_recall);
+int iq_plan_set_check_sample(iq_plan* plan, int queries);
 int iq_plan_proposal_stats(iq_plan* plan, int64_t out[5]);
 int iq_plan_encoder(iq_plan* plan, int mask, float* W, int32_t* tvox, int64_t* V, int* nmasks);
 

@@ -74,6 +74,34 @@ def test_every_batch_checked_is_exact():
     plan.close()
 
 
+def test_sampled_checks():
+    # check batches that hand only their first queries to the full scan:
+    # recall is measured on those alone, and with the whole pattern database
+    # as the shortlist the run still equals the full scan bit for bit
+    T, plan, info = _plan(nreal=2)
+    U = np.random.default_rng(8).random((2, plan.npath))
+    g0, c0 = _run(plan, U)
+    plan.learn(nsamples=2048, iters=6, seed=1)
+    plan.set_proposal(True, K=64, check_every=1, min_recall=0.0, check_queries=0)
+    plan.run(U)
+    every = plan.proposal_stats()
+    plan.set_proposal(True, K=64, check_every=1, min_recall=0.0, check_queries=3)
+    print("sampled checks", flush=True)
+    plan.run(U)
+    sampled = plan.proposal_stats()
+    print("checked queries: all", every["queries"], "sampled", sampled["queries"], flush=True)
+    assert 0 < sampled["queries"] < every["queries"]
+    assert 0 <= sampled["in_shortlist"] <= sampled["band_members"]
+    plan.set_proposal(True, K=10 ** 9, check_every=2, min_recall=1.0, check_queries=3)
+    g1, c1 = _run(plan, U)
+    st = plan.proposal_stats()
+    assert not st["fell_back"] and st["in_shortlist"] == st["band_members"] > 0
+    assert np.array_equal(c0, c1)
+    for a, b in zip(g0, g1):
+        assert np.array_equal(a, b)
+    plan.close()
+
+
 def test_fallback_when_recall_drops():
     T, plan, info = _plan()
     U = np.random.default_rng(5).random((1, plan.npath))
