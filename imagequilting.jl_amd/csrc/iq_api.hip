@@ -1353,6 +1353,11 @@ void plan_run_finish(iq_plan* pl);
 // the shortlist tiles of each batch.
 void plan_tables(iq_plan* pl, const double* uniforms, const int64_t* forced, bool learned, std::vector<IqQuery>& qs,
                  std::vector<int4>& tiles, std::vector<RunBatch>& batches) {
+  // sampled checks: the batches plan_run will check (every lrn_check-th
+  // batch with plain queries) put an evenly strided sample of their queries
+  // first, so the sample spans the batch's mask shapes and realizations
+  const bool sampled = learned && pl->lrn_check > 0 && pl->lrn_check_q > 0;
+  int64_t nplain = 0;
   for (size_t k = 0; k < pl->nwaves; k++) {
     // lockstep: wave k of every grid, all realizations
     std::vector<IqQuery> wq;
@@ -1376,9 +1381,29 @@ void plan_tables(iq_plan* pl, const double* uniforms, const int64_t* forced, boo
       });
     for (size_t b0 = 0; b0 < wq.size(); b0 += (size_t)pl->nslots) {
       RunBatch bt{(int64_t)qs.size(), 0, false, false, 0, 0};
-      for (size_t k = b0; k < wq.size() && k < b0 + (size_t)pl->nslots; k++) {
+      const size_t b1 = std::min(wq.size(), b0 + (size_t)pl->nslots);
+      std::vector<size_t> order(b1 - b0);
+      for (size_t k = b0; k < b1; k++) order[k - b0] = k;
+      bool plain = false;
+      for (size_t k = b0; k < b1; k++) plain |= !wq[k].multi;
+      if (plain && sampled && nplain++ % pl->lrn_check == 0 && (size_t)pl->lrn_check_q < order.size()) {
+        const size_t n = order.size(), ns = (size_t)pl->lrn_check_q;
+        std::vector<char> taken(n, 0);
+        std::vector<size_t> re;
+        for (size_t j = 0; j < ns; j++) {
+          const size_t i = j * n / ns;
+          taken[i] = 1;
+          re.push_back(b0 + i);
+        }
+        for (size_t i = 0; i < n; i++)
+          if (!taken[i]) re.push_back(b0 + i);
+        order = re;
+      } else if (plain && !sampled) {
+        nplain++;
+      }
+      for (size_t k : order) {
         IqQuery q = wq[k];
-        q.slot = (int32_t)(k - b0);
+        q.slot = (int32_t)(qs.size() - bt.off);
         bt.plain |= !q.multi;
         bt.multi |= q.multi != 0;
         qs.push_back(q);

@@ -266,10 +266,10 @@ int iq_plan_select_stats(iq_plan* plan, int64_t* queries, int64_t* superset, int
  * K, a full-scan check every check_every-th batch (0: never), and return to
  * the full scan for the rest of a run once the cumulative recall of the
  * checks falls below min_recall.
- * iq_plan_set_check_sample: in a check batch, only the first `queries`
- * queries (0: all of them, the default) go to the full scan, which decides
- * them and measures recall on their exact bands; the shortlist decides the
- * others.  No counterpart in the reference: the sampled recall check of
+ * iq_plan_set_check_sample: in a check batch, only `queries` queries (0:
+ * all of them, the default), an evenly strided sample of the batch, go to
+ * the full scan, which decides them and measures recall on their exact
+ * bands; the shortlist decides the others.  No counterpart in the reference: the sampled recall check of
  * the learned proposal.
  * iq_plan_proposal_stats: for the last run, out[0] band members of the
  * checked queries, out[1] those inside the shortlist, out[2] checked
