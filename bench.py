@@ -11,11 +11,11 @@ same plan with the same uniforms:
               interval bound, FP64 recheck of the band superset);
   learned     the shortlist of the K best approximate distances (bank scan of
               64-d bf16 encodings) replaces the full scan, reranked in FP64;
-              every check_every-th batch (default 16) hands its first
-              check_queries queries (default 32, one MFMA tile) to the full
-              scan, which decides them and measures recall on their exact
-              bands; recall falling below 99 % returns the run to the full
-              scan.  With the conv encoder, one more run per encoder checks
+              every check_every-th batch (default 16) hands an evenly
+              strided sample of check_queries of its queries (default 32, one
+              MFMA tile) to the full scan, which decides them and measures
+              recall on their exact bands; recall falling below 99 % returns
+              the run to the full scan.  With the conv encoder, one more run per encoder checks
               every query of every batch (encoder_ablation_every_batch_checked).
 One step = one complete simulation of all realizations (every tile step:
 distances, band, weights, draw, min cuts, paste) plus the download of the
@@ -199,7 +199,8 @@ def main():
     ap.add_argument("--K", type=int, default=2048, help="shortlist size of the learned proposal")
     ap.add_argument("--check-every", type=int, default=16, help="full-scan recall check every N-th batch")
     ap.add_argument("--check-queries", type=int, default=32,
-                    help="queries of a check batch the full scan decides and measures recall on (0: all)")
+                    help="queries of a check batch (a strided sample) the full scan decides and measures recall on "
+                         "(0: all)")
     ap.add_argument("--full-steps", type=int, default=-1, help="timed steps of the full-scan leg (default: --steps)")
     ap.add_argument("--encoder", choices=("conv", "linear"), default="conv",
                     help="learned leg's encoder: the trained convolutional network (default) or the linear one alone")
