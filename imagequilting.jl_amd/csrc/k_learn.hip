@@ -204,17 +204,11 @@ k_learn_bfrag(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const
   }
 }
 
-// Approximate score of candidate c for a query, in FP32 as the MFMA bank
-// scan forms it: A2[c] + sum_j bf16(E[c][j]) bf16(-2 e_q[j]) (A2 from the
-// FP32 copy of the mask's A2 map, NaN for a disabled window).  The exact
-// distance is A2 - 2 AB + B2: B2 is the same for every candidate of a query,
-// so it is left out of the order.
-__device__ inline float learn_score(const IqDev* D, int mask, int64_t c, const float* eq) {
-  const uint16_t* row = D->lrn.bank + D->lrn.bofs[mask] + c * IQ_EMB;
-  float acc = 0.0f;
-  for (int j = 0; j < IQ_EMB; j++) acc = __builtin_fmaf(bf16_val(row[j]), bf16_val(iq_bf16_rne(-2.0f * eq[j])), acc);
-  return acc + D->a2f[D->masks[mask].a2][c];
-}
+// Approximate score of candidate c for a query, in FP32: A2[c] + sum_j
+// bf16(E[c][j]) bf16(-2 e_q[j]) (A2 from the FP32 copy of the mask's A2 map,
+// NaN for a disabled window), the terms added in order of j.  The exact
+// distance is A2 - 2 AB + B2: B2 is the same for every candidate of a
+// query, so it is left out of the order.
 
 // the float whose order key is k (iq_f32_key's inverse on non-NaN keys)
 __device__ inline float key_float(uint32_t k) { return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
@@ -238,12 +232,33 @@ k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const int64_t slot = q.slot;
   const float* eq = D->lrn.eq + slot * IQ_EMB;
   const int64_t K = D->lrn.K < D->nen ? D->lrn.K : D->nen;
+  // the query's bf16(-2 e_q) as floats, once per block
+  __shared__ float qe[IQ_EMB];
+  if (threadIdx.x < IQ_EMB) qe[threadIdx.x] = bf16_val(iq_bf16_rne(-2.0f * eq[threadIdx.x]));
   if (threadIdx.x == 0) best = ~0ull;
   __syncthreads();
+  const uint16_t* bank = D->lrn.bank + D->lrn.bofs[st.mask];
+  const float* a2f = D->a2f[D->masks[st.mask].a2];
   unsigned long long mine = ~0ull;
   for (int i = threadIdx.x; i < LKS; i += blockDim.x) {
     const int64_t c = (int64_t)i * g.P / LKS;
-    smp[i] = D->disabled[c] ? 0xffffffffu : iq_f32_key(learn_score(D, st.mask, c, eq));
+    // the approximate score, the row in eight 16-byte loads
+    const auto row = iq_glob((const iq_u32x4*)(bank + c * IQ_EMB));
+    iq_u32x4 rw[IQ_EMB / 8];
+#pragma unroll
+    for (int k = 0; k < IQ_EMB / 8; k++) rw[k] = row[k];
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < IQ_EMB / 8; k++) {
+      const uint32_t wv[4] = {rw[k].x, rw[k].y, rw[k].z, rw[k].w};
+#pragma unroll
+      for (int h = 0; h < 4; h++) {
+        acc = __builtin_fmaf(bf16_val(wv[h] & 0xffffu), qe[8 * k + 2 * h], acc);
+        acc = __builtin_fmaf(bf16_val(wv[h] >> 16), qe[8 * k + 2 * h + 1], acc);
+      }
+    }
+    const float sc = acc + a2f[c];
+    smp[i] = D->disabled[c] ? 0xffffffffu : iq_f32_key(sc);
     const unsigned long long kv = ((unsigned long long)smp[i] << 32) | (unsigned)i;
     mine = kv < mine ? kv : mine;
   }
