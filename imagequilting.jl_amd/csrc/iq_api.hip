@@ -1524,6 +1524,12 @@ void plan_resets(iq_plan* pl) {
   HCL();
 }
 
+// The recall checks' fallback rule: a recall below the threshold ends the
+// shortlist once this many queries have been checked, or at once when it
+// misses by more than LRN_GROSS times the allowed loss 1 - min_recall.
+#define LRN_DECIDE_QUERIES 256
+#define LRN_GROSS 4.0
+
 // Every launch of one run, in order: resets, then the batches.  Host decisions
 // between batches (the recall checks' fallback) read back device counters.
 template <typename T>
@@ -1546,12 +1552,18 @@ void plan_run_launches(iq_plan* pl, const std::vector<IqQuery>& qs, const std::v
                  tiles.data() + b.toff, b.nt);
     if (sel == SEL_CHECK) {
       // the full scan decided this batch: stop using the shortlist once its
-      // cumulative recall on the exact bands falls below the threshold
+      // cumulative recall on the exact bands falls below the threshold.  The
+      // recall pools band members, and a few queries with wide bands weigh
+      // heavily in it, so a small sample is noisy: a shortfall counts once
+      // LRN_DECIDE_QUERIES queries have been checked, or at once when the
+      // recall misses by more than LRN_GROSS times the allowed loss
       unsigned long long st[4];
       HC(hipMemcpyAsync(st, H.lrn.stats, sizeof(st), hipMemcpyDeviceToHost, s));
       HC(hipStreamSynchronize(s));
       for (int k = 0; k < 4; k++) pl->lrn_stats[k] = (int64_t)st[k];
-      if (st[0] > 0 && (double)st[1] < pl->lrn_min_recall * (double)st[0]) {
+      const double rec = st[0] > 0 ? (double)st[1] / (double)st[0] : 1.0;
+      const double loss = 1.0 - pl->lrn_min_recall;
+      if (rec < pl->lrn_min_recall && (st[2] >= LRN_DECIDE_QUERIES || rec < 1.0 - LRN_GROSS * loss)) {
         learned = false;
         pl->lrn_fell_back = true;
       }

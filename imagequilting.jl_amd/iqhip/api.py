@@ -345,7 +345,10 @@ class Plan:
                      check_queries: int = 0) -> None:
         """Shortlist the K best approximate distances instead of the full scan;
         every check_every-th batch the full scan decides and measures recall,
-        on its first check_queries queries (0: on all of them)."""
+        on an evenly strided sample of check_queries of its queries (0: on all
+        of them).  A cumulative recall below min_recall returns the run to the
+        full scan once 256 queries have been checked, or at once when it is
+        below 1 - 4 (1 - min_recall)."""
         self.ctx.check(self.ctx._L.iq_plan_set_proposal(self.handle, int(bool(enable)), int(K), int(check_every),
                                                         float(min_recall)))
         self.ctx.check(self.ctx._L.iq_plan_set_check_sample(self.handle, int(check_queries)))

@@ -265,7 +265,9 @@ int iq_plan_select_stats(iq_plan* plan, int64_t* queries, int64_t* superset, int
  * iq_plan_set_proposal: use the shortlist in iq_plan_run (enable), its size
  * K, a full-scan check every check_every-th batch (0: never), and return to
  * the full scan for the rest of a run once the cumulative recall of the
- * checks falls below min_recall.
+ * checks falls below min_recall: after at least 256 checked queries, or at
+ * once when the recall is below 1 - 4 (1 - min_recall) (so min_recall = 1
+ * falls back at the first band member missed).
  * iq_plan_set_check_sample: in a check batch, only `queries` queries (0:
  * all of them, the default), an evenly strided sample of the batch, go to
  * the full scan, which decides them and measures recall on their exact
