@@ -460,16 +460,21 @@ __device__ inline int block_exclusive_sum_1024(int x, int* wsum) {
 }
 
 #define LPK_SORT 4096
+#define LPK_UNROLL 4
 // Per query: a 2048-bin histogram (in LDS) of the listed order keys over the
 // bracket, the threshold key where its count reaches K (the bracket's top
 // when the sample underestimated it), then the shortlist: the listed
-// candidates at or below it (list and bitmap).  Grid: nq blocks of 1024.
+// candidates at or below it (list and bitmap), in ascending candidate order
+// when it fits LPK_SORT.  The block owns its query's shortlist, so members
+// take their positions from an LDS counter; list entries are read
+// LPK_UNROLL per work-item at a time.  Grid: nq blocks of 1024.
 __global__ void __launch_bounds__(1024)
 k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   __shared__ uint32_t thr;
   __shared__ int32_t hist[2048];
   __shared__ int wsum[16];
   __shared__ int32_t skey[LPK_SORT];
+  __shared__ int scount;
   const IqQuery q = qs[blockIdx.x];
   const IqStep& st = D->steps[q.step];
   if (!learn_applies(D, q, st)) return;
@@ -483,10 +488,18 @@ k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     const uint32_t* br = D->lrn.brk + slot * 4;
     const uint32_t lo = br[0], shift = br[2];
     for (int i = threadIdx.x; i < 2048; i += blockDim.x) hist[i] = 0;
+    if (threadIdx.x == 0) scount = 0;
     __syncthreads();
-    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
-      const uint32_t k = (uint32_t)list[e].y;
-      atomicAdd(&hist[k <= lo ? 0 : (k - lo) >> shift], 1);
+    for (int64_t e0 = threadIdx.x; e0 < n; e0 += LPK_UNROLL * blockDim.x) {
+      uint32_t k[LPK_UNROLL];
+#pragma unroll
+      for (int u = 0; u < LPK_UNROLL; u++) {
+        const int64_t e = e0 + u * blockDim.x;
+        k[u] = e < n ? (uint32_t)list[e].y : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < LPK_UNROLL; u++)
+        if (e0 + u * blockDim.x < n) atomicAdd(&hist[k[u] <= lo ? 0 : (k[u] - lo) >> shift], 1);
     }
     __syncthreads();
   }
@@ -509,44 +522,50 @@ k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     __syncthreads();
   }
   const int lane = threadIdx.x & 63;
-  for (int64_t e0 = 0; e0 < n; e0 += blockDim.x) {
-    const int64_t e = e0 + threadIdx.x;
-    int2 it = make_int2(0, 0);
-    bool sel = false;
-    if (e < n) {
-      it = list[e];
-      sel = (uint32_t)it.y <= thr;
+  int32_t* sl = D->lrn.slist + slot * g.P;
+  for (int64_t e0 = 0; e0 < n; e0 += LPK_UNROLL * blockDim.x) {
+    int2 it[LPK_UNROLL];
+#pragma unroll
+    for (int u = 0; u < LPK_UNROLL; u++) {
+      const int64_t e = e0 + u * blockDim.x + threadIdx.x;
+      it[u] = e < n ? list[e] : make_int2(-1, -1);  // candidate -1: past the list
     }
-    const unsigned long long bal = __ballot(sel);
-    if (bal) {
+#pragma unroll
+    for (int u = 0; u < LPK_UNROLL; u++) {
+      const bool sel = it[u].x >= 0 && (uint32_t)it[u].y <= thr;
+      const unsigned long long bal = __ballot(sel);
+      if (!bal) continue;
       int base = 0;
-      if (lane == 0) base = atomicAdd(&D->lrn.nslist[slot], __popcll(bal));
+      if (lane == 0) base = atomicAdd(&scount, __popcll(bal));
       base = __shfl(base, 0);
       if (sel) {
-        D->lrn.slist[slot * g.P + base + __popcll(bal & ((1ull << lane) - 1ull))] = it.x;
-        atomicOr(&D->lrn.sbits[slot * D->nw64 + (it.x >> 6)], 1ull << (it.x & 63));
+        const int idx = base + __popcll(bal & ((1ull << lane) - 1ull));
+        sl[idx] = it[u].x;
+        if (idx < LPK_SORT) skey[idx] = it[u].x;
+        atomicOr(&D->lrn.sbits[slot * D->nw64 + (it[u].x >> 6)], 1ull << (it[u].x & 63));
       }
     }
   }
+  __syncthreads();
+  int ns = scount;
   // an empty shortlist (no approximate distance fell inside the bracket,
   // e.g. ties resolved one ulp apart) takes the seed: the exact recheck and
   // the pick then run on it instead of finding no candidate at all
-  __syncthreads();
-  if (threadIdx.x == 0 && D->lrn.nslist[slot] == 0) {
-    const int64_t c = D->lrn.brk[slot * 4 + 3];
-    D->lrn.slist[slot * g.P] = (int32_t)c;
-    D->lrn.nslist[slot] = 1;
-    atomicOr(&D->lrn.sbits[slot * D->nw64 + (c >> 6)], 1ull << (c & 63));
+  if (ns == 0) {
+    if (threadIdx.x == 0) {
+      const int64_t c = D->lrn.brk[slot * 4 + 3];
+      sl[0] = (int32_t)c;
+      atomicOr(&D->lrn.sbits[slot * D->nw64 + (c >> 6)], 1ull << (c & 63));
+    }
+    ns = 1;
   }
-  __syncthreads();
+  if (threadIdx.x == 0) D->lrn.nslist[slot] = ns;
   // the shortlist in ascending candidate order (bitonic sort in LDS), so the
   // bound kernel's neighbouring wavefronts read overlapping windows
-  const int ns = __hip_atomic_load(&D->lrn.nslist[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (ns > 1 && ns <= LPK_SORT) {
-    int32_t* sl = D->lrn.slist + slot * g.P;
     int N = 2;
     while (N < ns) N <<= 1;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) skey[i] = i < ns ? sl[i] : 0x7fffffff;
+    for (int i = ns + (int)threadIdx.x; i < N; i += blockDim.x) skey[i] = 0x7fffffff;
     __syncthreads();
     for (int k = 2; k <= N; k <<= 1)
       for (int j = k >> 1; j > 0; j >>= 1) {
