@@ -8,8 +8,8 @@
 // synchronous: a relabel phase, a push phase (only the higher endpoint of an
 // edge can push on it, so every residual pair has one writer) and a gather
 // phase that adds incoming flow in a fixed neighbour order; a global relabel
-// (exact residual BFS distance to the sink) runs every IQ_GR_FREQ rounds and
-// decides termination.  The result is deterministic and matches the oracle's
+// (exact residual BFS distance to the sink) runs every iq_gr_freq(n) rounds
+// (8, or 16 above 1,024 interior nodes) and decides termination.  The result is deterministic and matches the oracle's
 // restatement bit for bit.  Cut mask = voxels that cannot reach the sink in
 // the residual graph (graphcut.jl:79-81).
 #include "iq_internal.h"
@@ -107,6 +107,7 @@ __device__ void cut_slab(const SlabView<T>& A, const SlabView<T>& B, const int s
   si[dim] = L - 2;
   const int n = si[0] * si[1] * si[2];
   const int HMAX = n + 1;
+  const int GR = iq_gr_freq(n);
   const int ist[3] = {1, si[0], si[0] * si[1]};
   CutState S;
   cut_bind(S, mem, n);
@@ -169,7 +170,7 @@ __device__ void cut_slab(const SlabView<T>& A, const SlabView<T>& B, const int s
       if (threadIdx.x == 0) atomicCAS(err, 0, IQ_KERR_ITER);
       break;
     }
-    for (int it = 0; it < IQ_GR_FREQ; it++) {
+    for (int it = 0; it < GR; it++) {
       // relabel phase (also clears the push records of the previous round)
       for (int u = threadIdx.x; u < n; u += CB) {
         S.pr[3 * u] = 0.0; S.pr[3 * u + 1] = 0.0; S.pr[3 * u + 2] = 0.0;
@@ -341,6 +342,7 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
   si[dim] = L - 2;
   const int n = si[0] * si[1] * si[2];
   const int HMAX = n + 1;
+  const int GR = iq_gr_freq(n);
   const int ist[3] = {1, si[0], si[0] * si[1]};
   __shared__ int tflag[2 * (TEAM / 64)];
   int par = 0;
@@ -458,7 +460,7 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
       if (threadIdx.x == 0) atomicCAS(err, 0, IQ_KERR_ITER);
       break;
     }
-    for (int it = 0; it <= IQ_GR_FREQ; it++) {
+    for (int it = 0; it <= GR; it++) {
       // phase A: take in the previous round's pushes, then relabel
       int a2 = 0;
       int hnx[NPT];
@@ -494,7 +496,7 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
         int nh = hu;
         const bool active = e[k] > 0.0 && hu < HMAX;
         a2 |= active;
-        if (active && it < IQ_GR_FREQ) {
+        if (active && it < GR) {
           int hm[3], hp[3];
 #pragma unroll
           for (int d = 0; d < 3; d++) {
@@ -520,7 +522,7 @@ __device__ __forceinline__ void cut_slab_fast(const SlabView<T>& A, const SlabVi
       } else {
         team_sync<TEAM>();
       }
-      if (it == IQ_GR_FREQ) break;
+      if (it == GR) break;
       { int16_t* t = h; h = hn; hn = t; }
 #pragma unroll
       for (int k = 0; k < NPT; k++) hr[k] = hnx[k];
