@@ -9,7 +9,9 @@
 #define IQ_MAX_AUX 8        // soft-data pairs
 #define IQ_SB 64            // block of the three-level cumulative sums of the draw
 #define IQ_GR_FREQ 8        // push-relabel rounds between global relabels (slabs of <= IQ_GR_BIG interior nodes)
-#define IQ_GR_FREQ_BIG 16   // the same for larger slabs
+#ifndef IQ_GR_FREQ_BIG
+#define IQ_GR_FREQ_BIG 24   // the same for larger slabs
+#endif
 #define IQ_GR_BIG 1024
 #define IQ_PR_LIMIT 200000  // push-relabel round guard (never reached on valid input)
 #define IQ_SORT_MAX 4096    // largest multi-valued band ranked in LDS
@@ -223,9 +225,9 @@ __device__ inline __attribute__((address_space(1))) T* iq_globw(T* p) {
 }
 
 // Push-relabel rounds between global relabels of a slab with n interior
-// nodes: relabels cost about as much as rounds, and larger slabs need fewer
-// of them per round (1,536-node slabs: 0.92 M against 1.03 M cycles per cut
-// at 16 rounds instead of 8; slabs of ~200 nodes gain nothing).
+// nodes: larger slabs do better with fewer relabels per round (1,536-node
+// slabs: 1,499, 1,226, 1,188 and 1,194 us per launch of 362 cuts at 8, 16,
+// 24 and 32 rounds; slabs of ~200 nodes gain nothing).
 __host__ __device__ inline int iq_gr_freq(int64_t n) { return n > IQ_GR_BIG ? IQ_GR_FREQ_BIG : IQ_GR_FREQ; }
 
 __host__ __device__ inline int64_t iq_lin3(const int32_t d[3], int64_t x, int64_t y, int64_t z) {

@@ -9,7 +9,7 @@
 // edge can push on it, so every residual pair has one writer) and a gather
 // phase that adds incoming flow in a fixed neighbour order; a global relabel
 // (exact residual BFS distance to the sink) runs every iq_gr_freq(n) rounds
-// (8, or 16 above 1,024 interior nodes) and decides termination.  The result is deterministic and matches the oracle's
+// (8, or 24 above 1,024 interior nodes) and decides termination.  The result is deterministic and matches the oracle's
 // restatement bit for bit.  Cut mask = voxels that cannot reach the sink in
 // the residual graph (graphcut.jl:79-81).
 #include "iq_internal.h"

@@ -386,7 +386,7 @@ static double slab_cap(const double* A, const double* B, int is_f32, const int64
  * contracted into the terminals and a preflow push-relabel runs on the
  * interior slices.  The schedule is synchronous and deterministic (relabel,
  * push, gather phases; global relabel every `gr_freq` rounds; gr_freq <= 0
- * takes the kernel's rule: 8 rounds, 16 above 1,024 interior nodes), the
+ * takes the kernel's rule: 8 rounds, 24 above 1,024 interior nodes), the
  * same schedule the HIP kernel executes, so the two agree bit for bit.
  */
 int ora_graphcut(const double* A, const double* B, int is_f32, const int64_t s[3], int dim, int gr_freq,
@@ -404,7 +404,7 @@ int ora_graphcut(const double* A, const double* B, int is_f32, const int64_t s[3
   si[dim] = L - 2;
   int64_t n = si[0] * si[1] * si[2];
   const int64_t HMAX = n + 1;
-  if (gr_freq <= 0) gr_freq = n > 1024 ? 16 : 8;
+  if (gr_freq <= 0) gr_freq = n > 1024 ? 24 : 8;
   int64_t sst[3] = {1, s[0], s[0] * s[1]};
   int64_t ist[3] = {1, si[0], si[0] * si[1]};
   double* e = (double*)calloc((size_t)n, sizeof(double));

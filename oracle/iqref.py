@@ -29,7 +29,7 @@ ORA_ERR_NOCAND = -3
 ORA_ERR_NOTINDB = -4
 ORA_ERR_ITER = -5
 
-GR_FREQ = 0  # global-relabel period of the push-relabel schedule; 0: 8 rounds, 16 above 1,024 interior nodes (the HIP kernel's rule)
+GR_FREQ = 0  # global-relabel period of the push-relabel schedule; 0: 8 rounds, 24 above 1,024 interior nodes (the HIP kernel's rule)
 
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _f64p = ctypes.POINTER(ctypes.c_double)
