@@ -1708,8 +1708,8 @@ void plan_learn(iq_plan* pl, int64_t nsamp, int iters, uint64_t seed) {
     H.lrn.slist = pl->mem.alloc<int32_t>((size_t)pl->nslots * g.P);
     H.lrn.nslist = pl->mem.alloc<int32_t>(pl->nslots);
     for (int m = 0; m < nm; m++) pl->lrn_vmax = std::max(pl->lrn_vmax, pl->masks[m].V);
-    const size_t bound_lds = (size_t)pl->lrn_vmax * 8;  // query values and window offsets
-    if (bound_lds > 160 * 1024) bad(IQ_ERR_UNSUPPORTED, "learned proposal: overlap masks of more than 20,480 voxels");
+    const size_t bound_lds = (size_t)pl->lrn_vmax * 8;  // query values and window offsets (+ 2 KB of unit starts)
+    if (bound_lds > 156 * 1024) bad(IQ_ERR_UNSUPPORTED, "learned proposal: overlap masks of more than 19,968 voxels");
     if (bound_lds > 64 * 1024)
       HC(hipFuncSetAttribute((const void*)k_short_bound, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bound_lds));
     pl->lrn_wofs = wofs;

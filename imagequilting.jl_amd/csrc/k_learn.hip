@@ -582,16 +582,54 @@ k_learn_pick(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   }
 }
 
-// FP32 bounds of the shortlist: one wavefront per member sums a.q over the
-// masked voxels in FP32 (lanes take strided voxels, then a shuffle tree) with
-// the bound |AB32 - AB| <= gamma_V sum|a||b| <= gamma_V sqrt(A2 B2), giving an
-// interval [lo, hi] around the exact FP64 distance as in the full scan's
-// epilogue; lo goes to lo32, the least hi to the query's reduction word.
+// FP32 sum a.q of one candidate window over the masked voxels: each lane
+// sums its strided voxels in ascending order, eight window values in flight
+// at a time (sixteen measured slower), then a shuffle tree.
+template <typename E>
+__device__ __forceinline__ float bound_window_sum(const __attribute__((address_space(1))) E* ti, int64_t base,
+                                                  const int32_t* tl, const float* qv, int V, int lane) {
+  float acc = 0.0f;
+  int v = lane;
+  for (; v + 64 * 7 < V; v += 64 * 8) {
+    float t[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) t[u] = (float)ti[base + tl[v + 64 * u]];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc = __builtin_fmaf(t[u], qv[v + 64 * u], acc);
+  }
+  for (; v < V; v += 64) acc = __builtin_fmaf((float)ti[base + tl[v]], qv[v], acc);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  return acc;
+}
+
+// four floats at a 4-byte aligned address: one 16-byte global load (the
+// hardware takes dword-aligned multi-dword loads)
+typedef float iq_f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+
+// the first candidate of c's unit: same row, x rounded down to a multiple of 4
+__device__ inline int64_t bound_unit(int64_t c, int d0) { return c - ((c % d0) & 3); }
+
+// FP32 bounds of the shortlist: a wavefront sums a.q over the masked voxels
+// in FP32 with the bound |AB32 - AB| <= gamma_V sum|a||b| <= gamma_V
+// sqrt(A2 B2), giving an interval [lo, hi] around the exact FP64 distance as
+// in the full scan's epilogue; lo goes to lo32, the least hi to the query's
+// reduction word.  Members whose candidates share a row and an aligned block
+// of four x positions form a unit: their windows are one window shifted by
+// 0..3 voxels in x, so a 16-byte load at each masked voxel of the block's
+// first window feeds all of them, and the lines a unit brings through L2 are
+// shared (a unit whose block runs past the row's last candidate, and FP64
+// training images, take their members one window at a time).  The shortlist
+// is sorted, so a unit's members are adjacent in it.
 // Grid: (groups, query), 8 wavefronts per group (the staged query takes
-// 8 V bytes of LDS per group, so larger groups keep more wavefronts per CU).
-__global__ void __launch_bounds__(512)
+// 8 V bytes of LDS per group, so larger groups keep more wavefronts per CU);
+// a group lists the unit starts of its range LSB_WIN entries at a time in
+// LDS, and a wavefront takes one unit at a time.
+#define LSB_WIN 512
+__global__ void __launch_bounds__(LSB_WIN)
 k_short_bound(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   extern __shared__ float qv[];  // the query values (V floats), then the window offsets (V ints)
+  __shared__ int ustart[LSB_WIN + 1];
+  __shared__ int wcnt[LSB_WIN / 64];
   const IqQuery q = qs[blockIdx.y];
   const IqStep& st = D->steps[q.step];
   if (!learn_applies(D, q, st)) return;
@@ -619,6 +657,7 @@ k_short_bound(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const int n = D->lrn.nslist[slot];
   const int32_t* sl = D->lrn.slist + slot * g.P;
   float* lo_out = D->lo32 + slot * g.P;
+  const int d0 = g.dist[0];
   float lmin = __builtin_inff();
   // group b takes a contiguous range of the (sorted) shortlist; ranges are
   // dealt so that the groups of one XCD (b mod 8 when groups are dispatched
@@ -626,43 +665,80 @@ k_short_bound(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   const int G = gridDim.x;
   const int r = (G % 8 == 0) ? ((int)blockIdx.x % 8) * (G / 8) + (int)blockIdx.x / 8 : (int)blockIdx.x;
   const int ebeg = (int)((int64_t)n * r / G), eend = (int)((int64_t)n * (r + 1) / G);
-  for (int e = ebeg + w; e < eend; e += 8) {
-    const int64_t c = sl[e];
-    const int cx = (int)(c % g.dist[0]), cy = (int)((c / g.dist[0]) % g.dist[1]);
-    const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
-    const int64_t base = iq_lin3(g.ti, cx, cy, cz);
-    // each lane sums its strided voxels in ascending order, eight window
-    // values in flight at a time (sixteen measured slower: 841 against 759 us
-    // per launch at the 2.9M-candidate workload)
-    float acc = 0.0f;
-    int v = lane;
-    if (f32) {
-      for (; v + 64 * 7 < V; v += 64 * 8) {
-        float t[8];
+  for (int wb = ebeg; wb < eend; wb += LSB_WIN) {
+    const int we = eend - wb < LSB_WIN ? eend : wb + LSB_WIN;
+    // unit starts of [wb, we), in order (the window's first entry starts one)
+    const int e = wb + (int)threadIdx.x;
+    bool first = false;
+    if (e < we) first = e == wb || bound_unit(sl[e], d0) != bound_unit(sl[e - 1], d0);
+    const unsigned long long bal = __ballot(first);
+    if (lane == 0) wcnt[w] = __popcll(bal);
+    __syncthreads();
+    int before = 0, nu = 0;
 #pragma unroll
-        for (int u = 0; u < 8; u++) t[u] = ti32[base + tl[v + 64 * u]];
-#pragma unroll
-        for (int u = 0; u < 8; u++) acc = __builtin_fmaf(t[u], qv[v + 64 * u], acc);
-      }
-      for (; v < V; v += 64) acc = __builtin_fmaf(ti32[base + tl[v]], qv[v], acc);
-    } else {
-      for (; v + 64 * 7 < V; v += 64 * 8) {
-        float t[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) t[u] = (float)ti64[base + tl[v + 64 * u]];
-#pragma unroll
-        for (int u = 0; u < 8; u++) acc = __builtin_fmaf(t[u], qv[v + 64 * u], acc);
-      }
-      for (; v < V; v += 64) acc = __builtin_fmaf((float)ti64[base + tl[v]], qv[v], acc);
+    for (int i = 0; i < LSB_WIN / 64; i++) {
+      before += i < w ? wcnt[i] : 0;
+      nu += wcnt[i];
     }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    const double a2 = a2map[c];
-    const double sq = __builtin_sqrt(a2 * b2) * (1.0 + 0x1p-30);
-    const double d = __builtin_fabs((a2 - 2.0 * (double)acc) + b2);
-    const double er = (2.0 * (gerr + g64) * sq + 4.0 * u64 * (a2 + b2 + 2.0 * sq) + 1e3 * V * 0x1p-125) * (1.0 + 0x1p-30) +
-                      4.0 * u64 * d;
-    if (lane == 0) lo_out[c] = iq_f32_down(d - er > 0.0 ? d - er : 0.0);
-    lmin = fminf(lmin, iq_f32_up(d + er));
+    if (first) ustart[before + __popcll(bal & ((1ull << lane) - 1ull))] = e;
+    if (threadIdx.x == 0) ustart[nu] = we;
+    __syncthreads();
+    for (int k = w; k < nu; k += LSB_WIN / 64) {
+      const int s0 = ustart[k], s1 = ustart[k + 1];
+      const int64_t c0 = bound_unit(sl[s0], d0);
+      int jm = 0;  // members j of the unit: candidates c0 + j
+      for (int i = s0; i < s1; i++) jm |= 1 << (int)(sl[i] - c0);
+      const int x0 = (int)(c0 % d0), cy = (int)((c0 / d0) % g.dist[1]);
+      const int cz = (int)(c0 / ((int64_t)d0 * g.dist[1]));
+      const int64_t base = iq_lin3(g.ti, x0, cy, cz);
+      float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (f32 && x0 + 3 < d0) {
+        int v = lane;
+        for (; v + 64 * 3 < V; v += 64 * 4) {
+          iq_f32x4u t[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) t[u] = *iq_glob((const iq_f32x4u*)((const float*)D->ti + base + tl[v + 64 * u]));
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const float qq = qv[v + 64 * u];
+            acc[0] = __builtin_fmaf(t[u].x, qq, acc[0]);
+            acc[1] = __builtin_fmaf(t[u].y, qq, acc[1]);
+            acc[2] = __builtin_fmaf(t[u].z, qq, acc[2]);
+            acc[3] = __builtin_fmaf(t[u].w, qq, acc[3]);
+          }
+        }
+        for (; v < V; v += 64) {
+          const iq_f32x4u t = *iq_glob((const iq_f32x4u*)((const float*)D->ti + base + tl[v]));
+          const float qq = qv[v];
+          acc[0] = __builtin_fmaf(t.x, qq, acc[0]);
+          acc[1] = __builtin_fmaf(t.y, qq, acc[1]);
+          acc[2] = __builtin_fmaf(t.z, qq, acc[2]);
+          acc[3] = __builtin_fmaf(t.w, qq, acc[3]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (jm & (1 << j))
+            for (int o = 32; o > 0; o >>= 1) acc[j] += __shfl_xor(acc[j], o);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (jm & (1 << j))
+            acc[j] = f32 ? bound_window_sum(ti32, base + j, tl, qv, V, lane) : bound_window_sum(ti64, base + j, tl, qv, V, lane);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (!(jm & (1 << j))) continue;
+        const int64_t c = c0 + j;
+        const double a2 = a2map[c];
+        const double sq = __builtin_sqrt(a2 * b2) * (1.0 + 0x1p-30);
+        const double d = __builtin_fabs((a2 - 2.0 * (double)acc[j]) + b2);
+        const double er = (2.0 * (gerr + g64) * sq + 4.0 * u64 * (a2 + b2 + 2.0 * sq) + 1e3 * V * 0x1p-125) * (1.0 + 0x1p-30) +
+                          4.0 * u64 * d;
+        if (lane == 0) lo_out[c] = iq_f32_down(d - er > 0.0 ? d - er : 0.0);
+        lmin = fminf(lmin, iq_f32_up(d + er));
+      }
+    }
+    __syncthreads();  // the next window's unit starts overwrite these
   }
   if (lane == 0 && lmin < __builtin_inff()) atomicMin(&D->gmin_hi[slot], __float_as_uint(lmin));
 }
