@@ -1010,50 +1010,61 @@ k_band_recheck(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
 // RCL_MPW members; for every block of 64 masked voxels it loads the members'
 // window values one member at a time (lane l: voxel l of that window, so
 // each load is a few contiguous runs of the window), transposes them through
-// LDS, and lanes 0..RCL_MPW-1 then chain over their member's row, the query
-// values streaming through scalar loads.  The next block's loads are issued
-// before the current block's chains run.  Few members per wavefront keep
+// LDS with the block's 64 query values (lane l loads value l), and lanes
+// 0..RCL_MPW-1 then chain over their member's row, reading the query values
+// from LDS (one address for all lanes).  The next block's loads are issued
+// before the current block's chains run.  (Query values read by scalar
+// loads inside the chain: 280 us per launch at the 2.9M-candidate workload;
+// the scalar cache does not hold a query's 38 KB, and every 16 chain steps
+// waited on a scalar load.)  Few members per wavefront keep
 // one round of loads per block and many wavefronts per query.
 // Grid: (query, RCL_GROUPS wavefronts), members beyond loop.
 #define RCL_GROUPS 32
 #define RCL_MPW 16
 template <typename E>
 __device__ __forceinline__ double rcl_chain(const IqDev* __restrict__ D, int64_t base, const int32_t* __restrict__ toff,
-                                            const __attribute__((address_space(4))) double* kv, int V, E (*tile)[65]) {
+                                            const double* kvp, int V, E (*tile)[65], double* kt) {
   const int lane = threadIdx.x;
   const auto ti = iq_glob((const E*)D->ti);
+  const auto kv = iq_glob(kvp);
   int64_t bj[RCL_MPW];
 #pragma unroll
   for (int u = 0; u < RCL_MPW; u++)
     bj[u] = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)base, u) |
             ((int64_t)__builtin_amdgcn_readlane((int)(base >> 32), u) << 32);
   E t[RCL_MPW];
+  double kq;
   {
-    const int32_t o = toff[lane < V ? lane : 0];
+    const int i = lane < V ? lane : 0;
+    const int32_t o = toff[i];
 #pragma unroll
     for (int u = 0; u < RCL_MPW; u++) t[u] = ti[bj[u] + o];
+    kq = kv[i];
   }
   double ab = 0.0;
   for (int v0 = 0; v0 < V; v0 += 64) {
     const int nv = V - v0 < 64 ? V - v0 : 64;
 #pragma unroll
     for (int u = 0; u < RCL_MPW; u++) tile[u][lane] = t[u];
+    kt[lane] = kq;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int v1 = v0 + 64;
-    if (v1 < V) {  // the next block's window values, in flight during the chains
-      const int32_t o = toff[v1 + (v1 + lane < V ? lane : 0)];
+    if (v1 < V) {  // the next block's window and query values, in flight during the chains
+      const int i = v1 + (v1 + lane < V ? lane : 0);
+      const int32_t o = toff[i];
 #pragma unroll
       for (int u = 0; u < RCL_MPW; u++) t[u] = ti[bj[u] + o];
+      kq = kv[i];
     }
     if (lane < RCL_MPW) {
       const E* row = tile[lane];
       if (nv == 64) {
 #pragma unroll 16
-        for (int i = 0; i < 64; i++) ab = __builtin_fma((double)row[i], kv[v0 + i], ab);
+        for (int i = 0; i < 64; i++) ab = __builtin_fma((double)row[i], kt[i], ab);
       } else {
-        for (int i = 0; i < nv; i++) ab = __builtin_fma((double)row[i], kv[v0 + i], ab);
+        for (int i = 0; i < nv; i++) ab = __builtin_fma((double)row[i], kt[i], ab);
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1066,6 +1077,7 @@ __device__ __forceinline__ double rcl_chain(const IqDev* __restrict__ D, int64_t
 __global__ void __launch_bounds__(64)
 k_band_recheck_lanes(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   __shared__ double tile_mem[RCL_MPW * 65];  // the transposed block (doubles, or floats in its first half)
+  __shared__ double kt[64];                  // the block's query values
   const IqQuery q = qs[blockIdx.x];
   const int n = D->nsup[q.slot];
   if ((int)blockIdx.y * RCL_MPW >= n) return;
@@ -1076,7 +1088,7 @@ k_band_recheck_lanes(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs
   const IqMask& m = D->masks[st.mask];
   const int V = m.V;
   const int32_t* toff = D->toff + m.off;
-  const auto kv = iq_const(D->kv + slot * D->kv_stride);
+  const double* kvp = D->kv + slot * D->kv_stride;
   const double* a2map = D->a2maps[m.a2];
   const double b2 = D->b2[slot * (2 + D->naux)];
   double* dex = D->dexact + slot * g.P;
@@ -1092,8 +1104,8 @@ k_band_recheck_lanes(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs
       const int cz = (int)(c / ((int64_t)g.dist[0] * g.dist[1]));
       base = iq_lin3(g.ti, cx, cy, cz);
     }
-    const double ab = D->ti_f32 ? rcl_chain<float>(D, base, toff, kv, V, (float(*)[65])tile_mem)
-                                : rcl_chain<double>(D, base, toff, kv, V, (double(*)[65])tile_mem);
+    const double ab = D->ti_f32 ? rcl_chain<float>(D, base, toff, kvp, V, (float(*)[65])tile_mem, kt)
+                                : rcl_chain<double>(D, base, toff, kvp, V, (double(*)[65])tile_mem, kt);
     if (live) {
       const double d = __builtin_fabs((a2map[c] - 2.0 * ab) + b2);
       dex[c] = d;
