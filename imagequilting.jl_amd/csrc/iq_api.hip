@@ -460,9 +460,9 @@ std::vector<std::vector<int64_t>> build_waves(const HostGrid& G, const IqStep* s
 // choice changes speed only: both sizes compute the same bounds.
 template <typename T>
 void tune_scan(iq_plan* pl) {
-  const int nwa[2] = {2, 4};
+  const int nwa[3] = {2, 3, 4};
   for (int t = 1; t <= 2; t++) pl->nw_tiles[t] = pl->lds_nw[4] ? 4 : 2;
-  if (!pl->mfma || pl->mfma_nw_forced || !pl->lds_nw[2] || !pl->lds_nw[4]) return;
+  if (!pl->mfma || pl->mfma_nw_forced || !pl->lds_nw[2] || !pl->lds_nw[3] || !pl->lds_nw[4]) return;
   const IqGeo& g = pl->geo;
   hipStream_t s = pl->ctx->stream;
   std::vector<IqQuery> hq;
@@ -877,11 +877,12 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
         return l;
       };
       if (const char* envnw = std::getenv("IQHIP_SCAN_NW")) {
-        pl->mfma_nw = std::atoi(envnw) >= 4 ? 4 : 2;
+        const int e = std::atoi(envnw);
+        pl->mfma_nw = e >= 4 ? 4 : (e == 3 ? 3 : 2);
         pl->mfma_nw_forced = true;
       }
       size_t lmax = 0;
-      for (int nw : {2, 4}) {
+      for (int nw : {2, 3, 4}) {
         if (pl->mfma_nw_forced && nw != pl->mfma_nw) continue;
         if (lds_of(nw) > 150 * 1024) continue;
         pl->lds_nw[nw] = lds_of(nw);
@@ -893,7 +894,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
         pl->mfma_split = !bf16_exact(H.ti32, tivox);
         pl->mfma_gerr = scan_gerr(H.ktab.nstep, pl->mfma_split);
         H.chunks = pl->mem.upload(chunks.data(), chunks.size(), s);
-        for (int nw : {2, 4}) {
+        for (int nw : {2, 3, 4}) {
           if (!pl->lds_nw[nw]) continue;
           std::vector<int32_t> kofs(chunks.size());
           for (int b = 0; b < H.ktab.nbox; b++) {

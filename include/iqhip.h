@@ -226,10 +226,11 @@ int iq_scan_error_bound(int64_t nstep, int split, double* gerr);
 int iq_plan_cut_time(iq_plan* plan, int64_t* launches, double* seconds);
 
 /* Scan autotuning result of iq_plan_create (no counterpart in the reference):
- * wavefronts per MFMA-scan workgroup chosen for launches of one and of two or
- * more 32-query tiles (nw[0], nw[1]; 0 when the plan has no MFMA scan), and
- * the measured microseconds per launch us[tiles-1][0: 2 wavefronts, 1: 4]
- * (0 where not measured).  Diagnostic only. */
+ * wavefronts per MFMA-scan workgroup (2, 3 or 4) chosen for launches of one
+ * and of two or more 32-query tiles (nw[0], nw[1]; 0 when the plan has no
+ * MFMA scan), and the measured microseconds per launch us[tiles-1][0: 2
+ * wavefronts, 1: 4] (0 where not measured; the 3-wavefront timings are not
+ * reported).  Diagnostic only. */
 int iq_plan_scan_tuning(iq_plan* plan, int nw[2], float us[4]);
 
 /* Shortlist bank-scan launches of the last run (learned proposal; HIP

@@ -11,8 +11,8 @@ from iqhip import synth
 pytestmark = pytest.mark.gpu
 
 
-def _plan(nreal=1, seed=0, tile=(12, 12, 4)):
-    T = synth.grf((64, 64, 16), (20, 20, 4), np.random.default_rng(40), np.float32)
+def _plan(nreal=1, seed=0, tile=(12, 12, 4), dtype=np.float32):
+    T = synth.grf((64, 64, 16), (20, 20, 4), np.random.default_rng(40), np.float32).astype(dtype)
     plan, info = iqhip.make_plan(T, tile, (72, 72, 16), overlap=(0.25, 0.25, 0.25), tol=0.1, nreal=nreal,
                                  rng=np.random.default_rng(seed))
     return T, plan, info
@@ -44,12 +44,14 @@ def test_encoder_matches_cpu_pca():
     plan.close()
 
 
-@pytest.mark.parametrize("tile", [(12, 12, 4), (16, 16, 4)], ids=["rows3", "rows4"])
-def test_full_shortlist_is_the_full_scan(tile):
+@pytest.mark.parametrize("tile,dtype", [((12, 12, 4), np.float32), ((16, 16, 4), np.float32), ((12, 12, 4), np.float64)],
+                         ids=["rows3", "rows4", "float64"])
+def test_full_shortlist_is_the_full_scan(tile, dtype):
     # K = every candidate: the shortlist path must reproduce the full scan
     # exactly, with overlap masks of rows of 3 and 12 voxels (12-wide tiles)
-    # and of 4 and 16 voxels (16-wide tiles)
-    T, plan, info = _plan(nreal=2, tile=tile)
+    # and of 4 and 16 voxels (16-wide tiles), and on a Float64 training image
+    # (values exact in FP32), whose FP32 bounds convert both operands
+    T, plan, info = _plan(nreal=2, tile=tile, dtype=dtype)
     U = np.random.default_rng(3).random((2, plan.npath))
     g0, c0 = _run(plan, U)
     plan.learn(nsamples=2048, iters=6, seed=1)

@@ -1,8 +1,9 @@
 """Alternative kernels chosen by environment knobs give the default kernels'
 results bit for bit: the exact recheck with a wavefront per superset member
-(IQHIP_RECHECK=groups; the default runs a lane per member) and the min cut of
+(IQHIP_RECHECK=groups; the default runs a lane per member), the min cut of
 slabs with more than 1,024 interior nodes on 512 work-items
-(IQHIP_CUT_TEAM=512; the default uses 1,024).  Tile 32x32x8 with overlap 1/4
+(IQHIP_CUT_TEAM=512; the default uses 1,024) and the MFMA scan with 3 or 2
+wavefronts per group (IQHIP_SCAN_NW; the default is timed per plan).  Tile 32x32x8 with overlap 1/4
 gives 6 x 32 x 8 = 1,536-node slabs, the largest the large-slab kernel
 takes."""
 import os
@@ -35,7 +36,8 @@ def _run(env, T, U):
     return r["grids"], r["chosen"], r["cuts"], (rounds, cuts)
 
 
-@pytest.mark.parametrize("env", [{"IQHIP_RECHECK": "groups"}, {"IQHIP_CUT_TEAM": "512"}], ids=["recheck", "cut512"])
+@pytest.mark.parametrize("env", [{"IQHIP_RECHECK": "groups"}, {"IQHIP_CUT_TEAM": "512"}, {"IQHIP_SCAN_NW": "3"},
+                                 {"IQHIP_SCAN_NW": "2"}], ids=["recheck", "cut512", "scan3", "scan2"])
 def test_alternative_kernels_match_default(env):
     T = synth.grf((48, 48, 12), (10, 10, 3), np.random.default_rng(31), np.float32)
     probe, _ = iqhip.make_plan(T, (32, 32, 8), (56, 56, 14), overlap=(0.25, 0.25, 0.25), tol=0.1, nreal=2,
