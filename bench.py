@@ -236,7 +236,9 @@ def main():
     full_steps = args.steps if args.full_steps < 0 else args.full_steps
     nU = args.warmup + max(args.steps, full_steps)
     U = [rng.random(sum(plan.real_npath)) for _ in range(nU)]
-    buf = np.zeros(sum(int(np.prod(p)) for p in plan.real_padsize), dtype=plan.dtype)
+    # the output grids land in page-locked host memory (a pageable buffer
+    # would add a staging copy to every step's download)
+    buf = plan.ctx.pinned_empty(sum(int(np.prod(p)) for p in plan.real_padsize), plan.dtype)
     voxels = float(sum(np.prod(z) for z in simsizes))
     t32 = np.asarray(w["trainimg"], dtype=np.float32)
     bf16_exact = bool(((np.ascontiguousarray(t32).view(np.uint32) & 0xFFFF) == 0).all())

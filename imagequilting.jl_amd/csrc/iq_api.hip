@@ -2563,6 +2563,25 @@ int iq_ctx_synchronize(iq_ctx* ctx) {
   return guard(ctx, [&] { HC(hipStreamSynchronize(ctx->stream)); });
 }
 
+int iq_host_alloc(iq_ctx* ctx, size_t bytes, void** out) {
+  if (!ctx || !out) return fail(ctx, IQ_ERR_ARG, "null context or output");
+  return guard(ctx, [&] {
+    *out = nullptr;
+    HC(hipSetDevice(ctx->device));
+    if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+      *out = nullptr;
+      bad(IQ_ERR_NOMEM, "page-locked host allocation failed");
+    }
+  });
+}
+
+int iq_host_free(iq_ctx* ctx, void* p) {
+  if (!ctx) return fail(nullptr, IQ_ERR_ARG, "null context");
+  return guard(ctx, [&] {
+    if (p) HC(hipHostFree(p));
+  });
+}
+
 int iq_imfilter(iq_ctx* ctx, const double* img, const int64_t idims[3], const double* krn, const int64_t kdims[3],
                 double* out) {
   if (!ctx) return fail(nullptr, IQ_ERR_ARG, "null context");

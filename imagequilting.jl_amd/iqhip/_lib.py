@@ -29,6 +29,7 @@ STATUS = {
 # every symbol include/iqhip.h declares
 EXPORTS = (
     "iq_device_count", "iq_ctx_create", "iq_ctx_destroy", "iq_last_error", "iq_ctx_synchronize",
+    "iq_host_alloc", "iq_host_free",
     "iq_imfilter", "iq_fastdistance", "iq_band_pick", "iq_relaxation", "iq_taumodel", "iq_sample",
     "iq_graphcut", "iq_plan_create", "iq_plan_destroy", "iq_plan_run", "iq_plan_results",
     "iq_plan_create_multi", "iq_plan_last_time", "iq_plan_profile", "iq_plan_scan_stats", "iq_plan_scan_tuning", "iq_plan_cut_time", "iq_plan_cut_stats",
@@ -107,6 +108,8 @@ def load(path: str = LIB_PATH):
             "iq_ctx_create": [c_int, P(vp)],
             "iq_ctx_destroy": [vp],
             "iq_ctx_synchronize": [vp],
+            "iq_host_alloc": [vp, ctypes.c_size_t, P(vp)],
+            "iq_host_free": [vp, vp],
             "iq_imfilter": [vp, vp, vp, vp, vp, vp],
             "iq_fastdistance": [vp, vp, c_int, vp, vp, vp, vp, vp],
             "iq_band_pick": [vp, vp, i64, f64, vp, vp],

@@ -58,6 +58,30 @@ class Context:
     def check(self, rc: int):
         _lib.check(rc, self.handle)
 
+    def pinned_empty(self, n: int, dtype) -> np.ndarray:
+        """A flat array of n elements in page-locked host memory
+        (iq_host_alloc), freed with the array: plan results copy into it at
+        the full PCIe rate."""
+        dtype = np.dtype(dtype)
+        h = _Pinned(self, max(int(n) * dtype.itemsize, 1))
+        raw = (ctypes.c_char * h.nbytes).from_address(h.ptr)
+        raw._owner = h  # the allocation lives as long as the array's buffer
+        return np.frombuffer(raw, dtype=dtype, count=int(n))
+
+
+class _Pinned:
+    def __init__(self, ctx: "Context", nbytes: int):
+        p = ctypes.c_void_p()
+        ctx.check(ctx._L.iq_host_alloc(ctx.handle, nbytes, ctypes.byref(p)))
+        self.ctx, self.ptr, self.nbytes = ctx, p.value, nbytes
+
+    def __del__(self):
+        try:
+            if self.ctx.handle:
+                self.ctx._L.iq_host_free(self.ctx.handle, ctypes.c_void_p(self.ptr))
+        except Exception:
+            pass
+
 
 _DEFAULT: Optional[Context] = None
 

@@ -60,6 +60,12 @@ int iq_ctx_create(int device, iq_ctx** out);
 int iq_ctx_destroy(iq_ctx* ctx);
 const char* iq_last_error(const iq_ctx* ctx); /* ctx may be NULL: last error of any context on this thread */
 int iq_ctx_synchronize(iq_ctx* ctx);
+/* Page-locked host memory on the context's device (no counterpart in the
+ * reference): iq_plan_results copies into such a buffer at the full PCIe
+ * rate (a pageable buffer goes through a staging copy).  iq_host_free
+ * releases a buffer iq_host_alloc returned (NULL is ignored). */
+int iq_host_alloc(iq_ctx* ctx, size_t bytes, void** out);
+int iq_host_free(iq_ctx* ctx, void* p);
 
 /* ---- per-function entry points (host buffers in, host buffers out) ---- */
 
