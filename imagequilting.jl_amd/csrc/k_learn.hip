@@ -214,8 +214,9 @@ k_learn_bfrag(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs, const
 __device__ inline float key_float(uint32_t k) { return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
 
 // Per query: a bracket [lo, hi] of order keys around the K-th smallest
-// approximate distance from a strided sample of LKS candidates (sorted in
-// LDS; hi = the sample quantile at three times K/P), the bin width of a
+// approximate distance from a strided sample of LKS candidates (lo = the
+// sample minimum, hi = the sample quantile at three times K/P, selected in
+// LDS), the bin width of a
 // 2048-bin histogram over the bracket, fresh per-query state (candidate
 // list, shortlist, bitmaps), and the bank scan's per-column word of the
 // query at its batch position (the bracket's top as a float, lo, hi,
@@ -270,22 +271,56 @@ k_learn_bracket(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   unsigned long long* bm = D->bitmap + slot * D->nw64;
   unsigned long long* sb = D->lrn.sbits + slot * D->nw64;
   for (int64_t w = threadIdx.x; w < D->nw64; w += blockDim.x) { bm[w] = 0ull; sb[w] = 0ull; }
-  __syncthreads();
-  for (int k = 2; k <= LKS; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
+  // the sample's qi-th smallest key (0-based) by radix selection, eight bits
+  // a pass from the top: a histogram of the keys that share the digits fixed
+  // so far, then the bin that holds the remaining rank
+  const int64_t qi = (3 * K * LKS) / g.P + 1;
+  __shared__ uint32_t rhist[256];
+  __shared__ uint32_t rsel[2];  // the digits fixed so far, the rank left inside them
+  if (threadIdx.x == 0) { rsel[0] = 0u; rsel[1] = (uint32_t)(qi < LKS ? qi : 0); }
+  if (qi < LKS) {
+    for (int sh = 24; sh >= 0; sh -= 8) {
+      for (int i = threadIdx.x; i < 256; i += blockDim.x) rhist[i] = 0u;
+      __syncthreads();
+      const uint32_t pre = rsel[0];
+      const uint32_t pmask = sh == 24 ? 0u : (0xffffffffu << (sh + 8));
       for (int i = threadIdx.x; i < LKS; i += blockDim.x) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const uint32_t a = smp[i], b = smp[ixj];
-          if ((a > b) == ((i & k) == 0)) { smp[i] = b; smp[ixj] = a; }
+        const uint32_t k = smp[i];
+        if ((k & pmask) == pre) atomicAdd(&rhist[(k >> sh) & 255u], 1u);
+      }
+      __syncthreads();
+      if (threadIdx.x < 64) {  // one wavefront: four bins a lane, a wavefront prefix sum
+        const int l = threadIdx.x;
+        uint32_t c[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) c[u] = rhist[4 * l + u];
+        const uint32_t sum = c[0] + c[1] + c[2] + c[3];
+        uint32_t inc = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = __shfl_up(inc, o);
+          if (l >= o) inc += y;
+        }
+        const uint32_t r = rsel[1];
+        uint32_t acc = inc - sum;
+        if (acc <= r && r < inc) {  // one lane
+          for (int u = 0; u < 4; u++) {
+            if (r < acc + c[u]) {
+              rsel[0] = pre | ((uint32_t)(4 * l + u) << sh);
+              rsel[1] = r - acc;
+              break;
+            }
+            acc += c[u];
+          }
         }
       }
       __syncthreads();
     }
+  } else {
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
-    const uint32_t lo = smp[0];
-    const int64_t qi = (3 * K * LKS) / g.P + 1;
-    const uint32_t hi = qi >= LKS ? 0xfffffffeu : smp[qi];
+    const uint32_t lo = (uint32_t)(best >> 32);
+    const uint32_t hi = qi >= LKS ? 0xfffffffeu : rsel[0];
     const uint32_t span = hi - lo;
     uint32_t shift = 0;
     while (shift < 32 && (span >> shift) >= 2048u) shift++;
