@@ -370,6 +370,7 @@ struct iq_plan {
   int multi_bx = 16;       // candidate block width of the tiled maps (32 or 16; 16 rows)
   bool cut_big = false;    // largest slab needs the 512/1024-lane register-resident cut
   int cut_team = 1024;     // its work-items per slab (IQHIP_CUT_TEAM=512: 512)
+  int max_nslab = 1;       // most cut slabs of any step: the cut launches' grid width
   bool cut_wave = false;   // every slab fits the one-wavefront cut
   std::vector<int32_t> enabled_host;
   // learned proposal
@@ -979,6 +980,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
       maxn = std::max(maxn, (int64_t)sl.sz[0] * sl.sz[1] * sl.sz[2] / sl.sz[sl.dim] * std::max(0, sl.sz[sl.dim] - 2));
     }
   pl->cut_big = maxn > 4 * CB && maxn <= CUT_BIG_MAX;
+  for (const IqStep& st : pl->steps) pl->max_nslab = std::max(pl->max_nslab, st.nslab);
   for (const IqStep& st : pl->steps)
     for (int q = 0; q < st.nslab; q++) {
       const IqSlab& sl = st.slab[q];
@@ -1318,13 +1320,13 @@ void run_batch(iq_plan* pl, const IqQuery* hq, const IqQuery* q, int nq, bool ha
     HC(hipEventRecord(pl->cut_ev[pl->cut_used], s));
   }
   if (pl->cut_wave)
-    hipLaunchKernelGGL((k_graphcut_wave<T>), dim3(IQ_MAX_SLAB, nq), dim3(64), pl->cut_lds_max, s, D, q);
+    hipLaunchKernelGGL((k_graphcut_wave<T>), dim3(pl->max_nslab, nq), dim3(64), pl->cut_lds_max, s, D, q);
   else if (pl->cut_big && pl->cut_team == 1024)
-    hipLaunchKernelGGL((k_graphcut_big<T, 1024>), dim3(IQ_MAX_SLAB, nq), dim3(1024), pl->cut_lds_max, s, D, q);
+    hipLaunchKernelGGL((k_graphcut_big<T, 1024>), dim3(pl->max_nslab, nq), dim3(1024), pl->cut_lds_max, s, D, q);
   else if (pl->cut_big)
-    hipLaunchKernelGGL((k_graphcut_big<T, CB_BIG>), dim3(IQ_MAX_SLAB, nq), dim3(CB_BIG), pl->cut_lds_max, s, D, q);
+    hipLaunchKernelGGL((k_graphcut_big<T, CB_BIG>), dim3(pl->max_nslab, nq), dim3(CB_BIG), pl->cut_lds_max, s, D, q);
   else
-    hipLaunchKernelGGL((k_graphcut<T>), dim3(IQ_MAX_SLAB, nq), dim3(CB), pl->cut_lds_max, s, D, q);
+    hipLaunchKernelGGL((k_graphcut<T>), dim3(pl->max_nslab, nq), dim3(CB), pl->cut_lds_max, s, D, q);
   HCL();
   if (pl->prof) {
     HC(hipEventRecord(pl->cut_ev[pl->cut_used + 1], s));
