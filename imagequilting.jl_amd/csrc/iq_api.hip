@@ -370,7 +370,7 @@ struct iq_plan {
   int multi_bx = 16;       // candidate block width of the tiled maps (32 or 16; 16 rows)
   bool cut_big = false;    // largest slab needs the 512/1024-lane register-resident cut
   int cut_team = 1024;     // its work-items per slab (IQHIP_CUT_TEAM=512: 512)
-  int max_nslab = 1;       // most cut slabs of any step: the cut launches' grid width
+  int max_nslab = 1;       // most min cuts of any step: the cut launches' grid width
   bool cut_wave = false;   // every slab fits the one-wavefront cut
   std::vector<int32_t> enabled_host;
   // learned proposal
@@ -667,6 +667,10 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
         any = 1;
       }
     }
+    // slabs two slices thick (no interior: their cut is fixed) go last
+    std::stable_partition(st.slab, st.slab + st.nslab, [](const IqSlab& x) { return x.sz[x.dim] > 2; });
+    st.ncut = 0;
+    while (st.ncut < st.nslab && st.slab[st.ncut].sz[st.slab[st.ncut].dim] > 2) st.ncut++;
     st.mask = -1;
     if (any) {
       auto it = mask_index.find(bits);
@@ -980,7 +984,7 @@ void plan_build(iq_ctx* ctx, const iq_plan_desc* descs, int ngrid, iq_plan* pl) 
       maxn = std::max(maxn, (int64_t)sl.sz[0] * sl.sz[1] * sl.sz[2] / sl.sz[sl.dim] * std::max(0, sl.sz[sl.dim] - 2));
     }
   pl->cut_big = maxn > 4 * CB && maxn <= CUT_BIG_MAX;
-  for (const IqStep& st : pl->steps) pl->max_nslab = std::max(pl->max_nslab, st.nslab);
+  for (const IqStep& st : pl->steps) pl->max_nslab = std::max(pl->max_nslab, st.ncut);
   for (const IqStep& st : pl->steps)
     for (int q = 0; q < st.nslab; q++) {
       const IqSlab& sl = st.slab[q];

@@ -57,6 +57,7 @@ struct IqStep {
   int32_t hard;       // hard-data points in the tile (iqsim.jl:213)
   int32_t hoff, hcnt; // hard voxels of the tile: entries [hoff, hoff+hcnt) of the hard tables
   int32_t nslab;
+  int32_t ncut;       // slabs [0, ncut) have interior nodes and need a min cut; the others are two slices thick
   IqSlab slab[IQ_MAX_SLAB];
 };
 

@@ -612,7 +612,7 @@ k_graphcut(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   extern __shared__ char cut_lds[];
   const IqQuery q = qs[blockIdx.y];
   const IqStep& st = D->steps[q.step];
-  if ((int)blockIdx.x >= st.nslab) return;
+  if ((int)blockIdx.x >= st.ncut) return;
   const int64_t slot = q.slot;
   const int64_t pk = D->pick[slot];
   if (pk < 0) return;
@@ -656,7 +656,7 @@ k_graphcut_wave(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   extern __shared__ char cut_lds[];
   const IqQuery q = qs[blockIdx.y];
   const IqStep& st = D->steps[q.step];
-  if ((int)blockIdx.x >= st.nslab) return;
+  if ((int)blockIdx.x >= st.ncut) return;
   const int64_t slot = q.slot;
   const int64_t pk = D->pick[slot];
   if (pk < 0) return;
@@ -693,7 +693,7 @@ k_graphcut_big(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
   extern __shared__ char cut_lds[];
   const IqQuery q = qs[blockIdx.y];
   const IqStep& st = D->steps[q.step];
-  if ((int)blockIdx.x >= st.nslab) return;
+  if ((int)blockIdx.x >= st.ncut) return;
   const int64_t slot = q.slot;
   const int64_t pk = D->pick[slot];
   if (pk < 0) return;
@@ -780,8 +780,17 @@ k_paste(const IqDev* __restrict__ D, const IqQuery* __restrict__ qs) {
     for (int s = 0; s < ns; s++) {
       const IqSlab& sl = st.slab[s];
       int a = x - sl.lo[0], b = y - sl.lo[1], c = z - sl.lo[2];
-      if (a >= 0 && b >= 0 && c >= 0 && a < sl.sz[0] && b < sl.sz[1] && c < sl.sz[2])
-        k |= keep[(int64_t)s * g.tilevox + a + sl.sz[0] * (b + sl.sz[1] * c)];
+      if (a >= 0 && b >= 0 && c >= 0 && a < sl.sz[0] && b < sl.sz[1] && c < sl.sz[2]) {
+        if (s < st.ncut) {
+          k |= keep[(int64_t)s * g.tilevox + a + sl.sz[0] * (b + sl.sz[1] * c)];
+        } else {
+          // two slices, no interior: the slice next to the source keeps the
+          // grid, the other takes the window (negated for a next slab), as
+          // the cut kernels write for such a slab
+          const int cd = sl.dim == 0 ? a : (sl.dim == 1 ? b : c);
+          k |= (uint8_t)((cd == sl.sz[sl.dim] - 1 ? 0 : 1) ^ sl.side);
+        }
+      }
     }
     int64_t gi = iq_lin3(R.pad, st.origin[0] + x, st.origin[1] + y, st.origin[2] + z);
     if (!k) sim[gi] = ti[iq_lin3(g.ti, rc[0] + x, rc[1] + y, rc[2] + z)];
